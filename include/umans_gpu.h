@@ -1,0 +1,160 @@
+/* umans_gpu.h — C ABI of the MI355X (gfx950) backend for the UMANS per-step agent loop.
+ *
+ * One call of umans_step() performs what the reference's WorldBase::DoStep does
+ * (src/Engine/core/worldBase.cpp:79-128) for every agent of a world, on one GPU:
+ * neighbour search (replacing AgentKDTree + World*::ComputeNeighbors), preferred
+ * velocity (agent.cpp:52-59), Policy::ComputeNewVelocity (policy.cpp:43-76) with the
+ * cost functions named by their registry names (costFunctionFactory.cpp:41-61),
+ * contact forces (policy.cpp:107-141) and integration with toric wrap
+ * (agent.cpp:71-93, worldToric.cpp:81-108).
+ *
+ * Conventions
+ *   - Every function returns 0 on success or a negative UMANS_E* code; the message of
+ *     the last failure on a context is available from umans_last_error(). Nothing
+ *     throws across this boundary (the reference's C API uses bool/nullptr + stderr,
+ *     src/Library/APIFunctions.h:54-112).
+ *   - Host arrays are caller-owned and copied in or out; device buffers belong to the
+ *     context. 2-vectors are interleaved (x0, y0, x1, y1, ...).
+ *   - Calls on one context are serialised by the caller; umans_step() is asynchronous
+ *     on the context's stream; downloads synchronise.
+ */
+#ifndef UMANS_GPU_H
+#define UMANS_GPU_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define UMANS_ABI_VERSION 1
+
+enum {
+	UMANS_OK = 0,
+	UMANS_EINVAL = -22,    /* bad argument */
+	UMANS_ENOMEM = -12,    /* device allocation failed */
+	UMANS_EHIP = -1000,    /* HIP runtime error (message in umans_last_error) */
+	UMANS_ESTATE = -1001   /* call out of order (e.g. step before upload) */
+};
+
+/* Cost-function kinds. Registry names (costFunctionFactory.cpp:47-60):
+ * GoalReachingForce, PowerLaw, SocialForcesAvoidance, ORCA, Karamouzas, RVO,
+ * Moussaid, TtcaDca, Paris, VanToll. */
+enum {
+	UMANS_COST_GOAL_REACHING = 0, UMANS_COST_POWER_LAW = 1, UMANS_COST_SOCIAL_FORCES = 2,
+	UMANS_COST_ORCA = 3, UMANS_COST_KARAMOUZAS = 4, UMANS_COST_RVO = 5,
+	UMANS_COST_MOUSSAID = 6, UMANS_COST_TTCA_DCA = 7, UMANS_COST_PARIS = 8,
+	UMANS_COST_VAN_TOLL = 9, UMANS_COST_KIND_COUNT = 10
+};
+
+/* Policy::OptimizationMethod (policy.h:100-110) */
+enum { UMANS_METHOD_GRADIENT = 0, UMANS_METHOD_SAMPLING = 1, UMANS_METHOD_GLOBAL = 2 };
+
+#define UMANS_MAX_COSTS 4
+#define UMANS_MAX_PARAMS 8
+
+/* One Policy (policy.h:89-181) in packed form. The sampling fields are
+ * SamplingParameters (policy.h:36-87): s_type 0 regular / 1 random; s_base 0 zero /
+ * 1 current velocity; s_base_dir 0 unit / 1 current velocity / 2 preferred velocity;
+ * s_radius 0 preferred speed / 1 maximum speed / 2 maximum acceleration.
+ * cost_params[c] holds the cost function's own parameters in the order given by
+ * umans_cost_param_name(); cost_range[c] is CostFunction::range_ (costFunction.h:132). */
+typedef struct {
+	int32_t method;
+	int32_t s_type, s_base, s_base_dir, s_radius;
+	float s_angle;
+	int32_t s_speed_samples, s_angle_samples, s_random_samples, s_include_base;
+	float relaxation_time, contact_force_scale;
+	int32_t n_cost;
+	int32_t cost_kind[UMANS_MAX_COSTS];
+	float cost_coeff[UMANS_MAX_COSTS];
+	float cost_range[UMANS_MAX_COSTS];
+	float cost_params[UMANS_MAX_COSTS][UMANS_MAX_PARAMS];
+} umans_policy;
+
+/* World description. world_type 0 = WorldInfinite (worldInfinite.cpp), 1 = WorldToric
+ * (worldToric.cpp) centred on the origin. max_neighbors caps each agent's neighbour
+ * list at the first k entries of the reference's list in canonical order (ascending
+ * double d2, ties by agent ID; toric segments concatenated); <= 0 means no cap
+ * (lists longer than the backend maximum, 128, are truncated and counted in
+ * umans_stats.overflow_agents). cell_size 0 = automatic (max interaction range);
+ * block_size 0 = automatic. */
+typedef struct {
+	int32_t world_type;
+	float width, height;
+	float delta_time;
+	int32_t max_neighbors;
+	float cell_size;
+	int32_t block_size;
+	int32_t device;            /* HIP device ordinal */
+} umans_world_desc;
+
+/* Per-agent host arrays (length n). Inputs for umans_upload; outputs for
+ * umans_download (any output pointer may be null). rng is the per-agent
+ * std::minstd_rand0 state (agent.cpp:38); null on upload = seeded from the ID. */
+typedef struct {
+	int32_t* id;
+	float* pos;
+	float* vel;
+	float* goal;
+	float* viewdir;
+	float* radius;
+	float* pref_speed;
+	float* max_speed;
+	float* max_accel;
+	int32_t* policy;
+	uint32_t* rng;
+	float* pref_vel;           /* download only: preferred velocity of the last step */
+} umans_agents;
+
+typedef struct {
+	int64_t steps;             /* steps run on this context */
+	int64_t overflow_agents;   /* agents whose neighbour list hit the cap, last step */
+	int64_t max_list;          /* longest uncapped neighbour list, last step */
+	double sim_time;           /* WorldBase::time_ */
+} umans_stats;
+
+typedef struct umans_ctx umans_ctx;
+
+/* Context lifetime. Replaces CrowdSimulator construction (crowdSimulator.cpp:30-35). */
+int umans_create(const umans_world_desc* desc, umans_ctx** out);
+void umans_destroy(umans_ctx* ctx);
+const char* umans_last_error(const umans_ctx* ctx);
+
+/* Policies (CrowdSimulator::AddPolicy, crowdSimulator.h:91). Policy i gets ID i. */
+int umans_set_policies(umans_ctx* ctx, const umans_policy* policies, int n);
+
+/* Agents: replaces WorldBase::AddAgent/Agent::setGoal (worldBase.cpp:161-187,
+ * agent.cpp:120-127) for the whole population at once. */
+int umans_upload(umans_ctx* ctx, const umans_agents* agents, int n);
+int umans_download(umans_ctx* ctx, umans_agents* agents);
+int umans_num_agents(const umans_ctx* ctx);
+
+/* nsteps × WorldBase::DoStep (worldBase.cpp:79-128); asynchronous. */
+int umans_step(umans_ctx* ctx, int nsteps);
+int umans_synchronize(umans_ctx* ctx);
+int umans_get_stats(umans_ctx* ctx, umans_stats* out);
+
+/* Neighbour lists of the last step, in upload order (the reference's
+ * Agent::getNeighbors after ComputeNeighbors, agent.cpp:43-50): count[n] and, per
+ * agent, up to `stride` entries of neighbour ID, double squared distance to the
+ * (displaced) query point, and toric segment (0 = (0,0), 1 = +W, 2 = -W, 3 = +H,
+ * 4 = -H; worldToric.cpp:64-76). Requires umans_set_record_neighbors(ctx, 1)
+ * before the step. */
+int umans_set_record_neighbors(umans_ctx* ctx, int enable);
+int umans_dump_neighbors(umans_ctx* ctx, int stride, int32_t* count, int32_t* ids,
+                         double* d2, int32_t* seg);
+
+/* Registry (CostFunctionFactory, costFunctionFactory.h:35-80 and the parameter
+ * readers of each cost function's parseParameters). Host-only; no GPU needed. */
+int umans_cost_id(const char* name);                           /* -1 if unknown */
+const char* umans_cost_name(int kind);
+int umans_cost_param_index(int kind, const char* param);       /* -1 if unknown */
+const char* umans_cost_param_name(int kind, int index);        /* null past the end */
+int umans_cost_defaults(int kind, float* range, float params[UMANS_MAX_PARAMS]);
+int umans_policy_init(umans_policy* p, int method);            /* reference defaults */
+
+#ifdef __cplusplus
+}
+#endif
+#endif
