@@ -1,0 +1,144 @@
+"""Throughput of the per-step agent loop on one MI355X.
+
+    python bench.py [--steps K] [--warmup W] [--workload mixed|bidirectional|circle]
+
+A step is one WorldBase::DoStep (neighbour grid rebuild, neighbour search, new
+velocity, contact forces, integration) over every agent, on synthetic agents already
+resident in HBM. Default workload: the large mixed crowd, 1M agents in a 2000 x 2000 m
+toric world (40% PowerLaw + sampling, 30% SocialForces + gradient, 30% ORCA, range 4 m,
+at most 12 neighbours, dt 0.05 s). Prints one JSON line with the aggregate
+agent-steps/s, the roofline of the dominant kernel (the fused step kernel, timed with
+HIP events on the library's stream) and the reference's own CPU path timed on the host
+cores on a bounded sample of the same workload.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+from umans_amd import roofline as RL  # noqa: E402
+from umans_amd import scenarios as S  # noqa: E402
+
+
+def workload(name):
+    if name == "mixed":
+        return S.mixed(), RL.flops_mixed()
+    if name == "bidirectional":
+        return S.bidirectional(), RL.flops_karamouzas_sampling()
+    if name == "circle":
+        return S.circle(50), RL.flops_orca(10)
+    raise SystemExit(f"unknown workload {name}")
+
+
+def cpu_baseline(sc, steps, threads):
+    """Time the CPU path on the host cores: the compiled reference (oracle/_ref) when it
+    is present, else the oracle's FP32 restatement (OpenMP)."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import ref_py as R
+    from umans_amd.registry import to_reference_xml_policy
+    if R.available() and sc.name.startswith("mixed"):
+        pols = [to_reference_xml_policy("sampling", [("PowerLaw", 1.0, {"k": 1.5, "tau0": 3.0, "range": 4.0}),
+                                                     ("GoalReachingForce", 1.0, {})],
+                                        sampling={"SamplingType": "regular", "SamplingBase": "zero",
+                                                  "SamplingBaseDirection": "preferred velocity",
+                                                  "SamplingAngle": 360, "SpeedSamples": 6, "AngleSamples": 12,
+                                                  "SamplingRadius": "maximum speed"}, relaxation_time=0.5),
+                to_reference_xml_policy("gradient", [("SocialForcesAvoidance", 1.0, {"V0": 2000.0, "sigma": 0.08,
+                                                                                    "range": 4.0, "agentMass": 80.0}),
+                                                     ("GoalReachingForce", 1.0, {})], relaxation_time=0.5),
+                to_reference_xml_policy("global", [("ORCA", 1.0, {"timeHorizon": 3.0, "range": 4.0})])]
+        sim = R.RefSim(sc.world_type, sc.width, sc.height, sc.dt, pols, sc.agents, threads=threads)
+        sim.step(1)   # first step outside the timing, as the warm-up of the GPU leg
+        t = sim.time_steps(steps)
+        sim.close()
+        kind = "reference"
+    else:
+        import oracle_py as O
+        wd = O.make_world(sc.world_type, sc.width, sc.height, sc.dt, sc.max_neighbors)
+        a = dict(sc.agents)
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            o = O.step(wd, sc.policies, a, precision=0, nthreads=threads)
+            a["pos"], a["vel"], a["viewdir"] = o["pos"], o["vel"], o["viewdir"]
+        t = time.perf_counter() - t0
+        kind = "port"
+    return {"value": sc.n * steps / t, "unit": "agent-steps/s", "cores": threads, "kind": kind,
+            "sample": f"{sc.n} agents x {steps} steps of the same workload ({t:.1f} s)"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=50)
+    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--workload", default="mixed")
+    ap.add_argument("--cpu-steps", type=int, default=3)
+    ap.add_argument("--cpu-threads", type=int, default=min(16, os.cpu_count() or 1))
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py runs on one GPU only (asked for {args.gpus}); the multi-GPU design is "
+              "documented, not run.", file=sys.stderr)
+        return 2
+
+    from umans_amd import gpu
+    sc, flops_per_agent = workload(args.workload)
+    w = gpu.World(sc.world_type, sc.width, sc.height, sc.dt, sc.policies, sc.max_neighbors)
+    w.upload(sc.agents)
+    w.step(args.warmup)
+    w.synchronize()
+    w.set_profiling(True)
+    w.synchronize()
+    t0 = time.perf_counter()
+    w.step(args.steps)
+    w.synchronize()
+    t = time.perf_counter() - t0
+    prof = w.profile()
+    st = w.stats()
+    w.close()
+
+    n = sc.n
+    value = n * args.steps / t
+    step_s = prof["step_ms"] / 1e3 / max(prof["steps"], 1)
+    grid_s = prof["grid_ms"] / 1e3 / max(prof["steps"], 1)
+    achieved = RL.STEP_BYTES * n / step_s / 1e9
+    valu = flops_per_agent * n / step_s / 1e12
+    out = {
+        "metric": "agent-steps/s at 1M agents" if sc.name.startswith("mixed") else "agent-steps/s",
+        "value": value,
+        "unit": "agent-steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": t * 1e3 / args.steps,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic (seeded generator)",
+        "config": {"workload": sc.name, "agents": n, "world": "toric" if sc.world_type == 1 else "infinite",
+                   "size_m": sc.width, "dt": sc.dt, "max_neighbors": sc.max_neighbors,
+                   "policies": "40% PowerLaw+sampling 12x6, 30% SocialForces+gradient, 30% ORCA"
+                   if sc.name.startswith("mixed") else sc.name, "parallelism": "single GPU"},
+        "roofline": {"bound": "hbm", "kernel": "k_step", "achieved": achieved, "peak": RL.HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": achieved / RL.HBM_PEAK_GBS, "traffic": None,
+                     "bytes_per_agent": RL.STEP_BYTES, "avg_launch_ms": step_s * 1e3,
+                     "valu": {"achieved": valu, "peak": RL.FP32_PEAK_TFLOPS, "unit": "TFLOP/s",
+                              "frac": valu / RL.FP32_PEAK_TFLOPS, "flops_per_agent": flops_per_agent}},
+        "phases_ms": {"grid": grid_s * 1e3, "step": step_s * 1e3},
+        "neighbor_overflow_agents": st["overflow_agents"],
+    }
+    if not args.no_cpu_baseline:
+        print(f"# GPU leg done ({value:.3e} agent-steps/s); timing the CPU baseline", file=sys.stderr, flush=True)
+        out["cpu_baseline"] = cpu_baseline(sc, args.cpu_steps, args.cpu_threads)
+        out["cpu_baseline"]["gpu_over_cpu"] = value / out["cpu_baseline"]["value"]
+    print(json.dumps(out))
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

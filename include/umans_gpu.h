@@ -135,6 +135,18 @@ int umans_step(umans_ctx* ctx, int nsteps);
 int umans_synchronize(umans_ctx* ctx);
 int umans_get_stats(umans_ctx* ctx, umans_stats* out);
 
+/* Per-phase GPU timers (hipEvents on the context's stream), accumulated over the
+ * steps run while enabled: grid = cell keys + scan + reorder (+ bounding box in
+ * infinite worlds), step = the fused per-agent kernel. The reference has only a
+ * wall-clock timer around a whole run (crowdSimulator.cpp:109,145-153). */
+typedef struct {
+	int64_t steps;
+	double grid_ms;
+	double step_ms;
+} umans_profile;
+int umans_set_profiling(umans_ctx* ctx, int enable);   /* enabling resets the totals */
+int umans_get_profile(umans_ctx* ctx, umans_profile* out);  /* synchronises */
+
 /* Neighbour lists of the last step, in upload order (the reference's
  * Agent::getNeighbors after ComputeNeighbors, agent.cpp:43-50): count[n] and, per
  * agent, up to `stride` entries of neighbour ID, double squared distance to the

@@ -1,0 +1,467 @@
+// Kernels of one WorldBase::DoStep (src/Engine/core/worldBase.cpp:79-128) on gfx950.
+//
+// Per step, on the context's stream:
+//   k_fill         clear cell counters (a kernel, never a memset, so the step can be
+//                  captured in a graph)
+//   k_bbox/k_grid  infinite worlds only: bounding box -> uniform cell grid
+//   k_count        cell key of every agent + its rank inside the cell (atomic)
+//   k_scan_*       exclusive scan of the counts -> cell ranges
+//   k_reorder      scatter the state into cell order; preferred velocity
+//                  (Agent::ComputePreferredVelocity, agent.cpp:52-59)
+//   k_step         per agent (one lane): k-nearest-within-radius query in canonical
+//                  order, new velocity, contact forces, integration, toric wrap
+// The sorted order of one step is the input order of the next, so agents that moved
+// little keep their slot and the scatter stays nearly coalesced.
+#pragma once
+#include "costs.h"
+
+namespace umans {
+
+// ------------------------------------------------------------------ utilities
+__global__ void k_fill(unsigned* p, int n, unsigned v) {
+	const int i = blockIdx.x * blockDim.x + threadIdx.x;
+	if (i < n) p[i] = v;
+}
+
+UD int ordered_key(float f) {
+	const int i = __float_as_int(f);
+	return i >= 0 ? i : (i ^ 0x7fffffff);
+}
+UD float from_ordered_key(int k) { return __int_as_float(k >= 0 ? k : (k ^ 0x7fffffff)); }
+
+UD int wave_min_i(int v) {
+	for (int o = 32; o > 0; o >>= 1) v = min(v, __shfl_xor(v, o));
+	return v;
+}
+UD int wave_max_i(int v) {
+	for (int o = 32; o > 0; o >>= 1) v = max(v, __shfl_xor(v, o));
+	return v;
+}
+
+// bbox[0..3] = ordered keys of min x, min y, max x, max y (initialised by k_bbox_init)
+__global__ void k_bbox_init(int* bbox) {
+	if (threadIdx.x == 0) {
+		bbox[0] = 0x7fffffff; bbox[1] = 0x7fffffff;
+		bbox[2] = (int)0x80000000; bbox[3] = (int)0x80000000;
+	}
+}
+__global__ __launch_bounds__(256) void k_bbox(const float2* pos, int n, int* bbox) {
+	int mnx = 0x7fffffff, mny = 0x7fffffff, mxx = (int)0x80000000, mxy = (int)0x80000000;
+	for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+		const float2 p = pos[i];
+		if (isfinite(p.x) && isfinite(p.y)) {
+			const int kx = ordered_key(p.x), ky = ordered_key(p.y);
+			mnx = min(mnx, kx); mny = min(mny, ky); mxx = max(mxx, kx); mxy = max(mxy, ky);
+		}
+	}
+	mnx = wave_min_i(mnx); mny = wave_min_i(mny); mxx = wave_max_i(mxx); mxy = wave_max_i(mxy);
+	if ((threadIdx.x & 63) == 0) {
+		atomicMin(&bbox[0], mnx); atomicMin(&bbox[1], mny);
+		atomicMax(&bbox[2], mxx); atomicMax(&bbox[3], mxy);
+	}
+}
+// Infinite world: a grid over the bounding box with cells of at least `cell`
+// metres and at most `cap` cells.
+__global__ void k_grid_setup(const int* bbox, float cell, int cap, GridInfo* g) {
+	if (threadIdx.x != 0) return;
+	GridInfo r;
+	const bool empty = bbox[0] > bbox[2];
+	const float x0 = empty ? 0.0f : from_ordered_key(bbox[0]);
+	const float y0 = empty ? 0.0f : from_ordered_key(bbox[1]);
+	const float x1 = empty ? 0.0f : from_ordered_key(bbox[2]);
+	const float y1 = empty ? 0.0f : from_ordered_key(bbox[3]);
+	double cs = cell > 0.0f ? (double)cell : 1.0;
+	double w = (double)x1 - (double)x0, h = (double)y1 - (double)y0;
+	double nx = floor(w / cs) + 1.0, ny = floor(h / cs) + 1.0;
+	while (nx * ny > (double)cap) {
+		cs *= 1.25;
+		nx = floor(w / cs) + 1.0;
+		ny = floor(h / cs) + 1.0;
+	}
+	r.ox = x0; r.oy = y0;
+	r.invx = (float)(1.0 / cs); r.invy = r.invx;
+	r.ncx = (int)nx; r.ncy = (int)ny;
+	r.ncells = r.ncx * r.ncy;
+	r.pad = 0;
+	*g = r;
+}
+
+// Cell index along one axis: monotone non-decreasing in x, clamped into the grid
+// (non-finite coordinates land in cell 0).
+UD int bin_axis(float x, float o, float inv, int nc) {
+	float t = floorf((x - o) * inv);
+	t = fminf(fmaxf(t, 0.0f), (float)(nc - 1));
+	return (int)t;
+}
+
+__global__ __launch_bounds__(256) void k_count(const float2* pos, int n, const GridInfo* grid,
+                                              unsigned* cell_count, unsigned* cell_of, unsigned* rank) {
+	const int k = blockIdx.x * blockDim.x + threadIdx.x;
+	if (k >= n) return;
+	const GridInfo g = *grid;
+	const float2 p = pos[k];
+	const int c = bin_axis(p.y, g.oy, g.invy, g.ncy) * g.ncx + bin_axis(p.x, g.ox, g.invx, g.ncx);
+	cell_of[k] = (unsigned)c;
+	rank[k] = atomicAdd(&cell_count[c], 1u);
+}
+
+// ---------------------------------------------------------------- scan (3 passes)
+constexpr int kScanThreads = 256;
+constexpr int kScanItems = 4;
+constexpr int kScanTile = kScanThreads * kScanItems;
+
+UD unsigned block_exclusive_scan(unsigned v, unsigned* sh, unsigned& total) {
+	// wave-level inclusive scan, then across the block's 4 waves
+	const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+	unsigned x = v;
+	for (int o = 1; o < 64; o <<= 1) {
+		const unsigned y = __shfl_up(x, o);
+		if (lane >= o) x += y;
+	}
+	if (lane == 63) sh[wave] = x;
+	__syncthreads();
+	unsigned wofs = 0, tot = 0;
+	for (int w = 0; w < kScanThreads / 64; ++w) {
+		if (w < wave) wofs += sh[w];
+		tot += sh[w];
+	}
+	__syncthreads();
+	total = tot;
+	return wofs + x - v;
+}
+
+__global__ __launch_bounds__(kScanThreads) void k_scan_local(const unsigned* in, unsigned* out, unsigned* sums, int n) {
+	__shared__ unsigned sh[kScanThreads / 64];
+	const int base = blockIdx.x * kScanTile + threadIdx.x * kScanItems;
+	unsigned v[kScanItems];
+	unsigned s = 0;
+	for (int i = 0; i < kScanItems; ++i) {
+		v[i] = (base + i < n) ? in[base + i] : 0u;
+		s += v[i];
+	}
+	unsigned total;
+	unsigned run = block_exclusive_scan(s, sh, total);
+	for (int i = 0; i < kScanItems; ++i) {
+		if (base + i < n) out[base + i] = run;
+		run += v[i];
+	}
+	if (threadIdx.x == 0) sums[blockIdx.x] = total;
+}
+__global__ __launch_bounds__(kScanThreads) void k_scan_sums(unsigned* sums, int nb) {
+	__shared__ unsigned sh[kScanThreads / 64];
+	unsigned carry = 0;
+	for (int b0 = 0; b0 < nb; b0 += kScanThreads) {
+		const int i = b0 + threadIdx.x;
+		const unsigned v = i < nb ? sums[i] : 0u;
+		unsigned total;
+		const unsigned ex = block_exclusive_scan(v, sh, total);
+		if (i < nb) sums[i] = carry + ex;
+		carry += total;
+		__syncthreads();
+	}
+}
+__global__ __launch_bounds__(kScanThreads) void k_scan_add(unsigned* out, const unsigned* sums, int n) {
+	const int base = blockIdx.x * kScanTile;
+	const unsigned add = sums[blockIdx.x];
+	for (int i = threadIdx.x; i < kScanTile; i += kScanThreads)
+		if (base + i < n) out[base + i] += add;
+}
+
+// --------------------------------------------------------------------- reorder
+__global__ __launch_bounds__(256) void k_reorder(State src, State dst, int n, const unsigned* cell_of,
+                                                const unsigned* rank, const unsigned* cell_start) {
+	const int k = blockIdx.x * blockDim.x + threadIdx.x;
+	if (k >= n) return;
+	const unsigned d = cell_start[cell_of[k]] + rank[k];
+	const float2 p = src.pos[k];
+	const float2 g = src.goal[k];
+	const float4 prm = src.prm[k];
+	dst.pos[d] = p;
+	dst.vel[d] = src.vel[k];
+	dst.goal[d] = g;
+	dst.viewdir[d] = src.viewdir[k];
+	dst.prm[d] = prm;
+	dst.meta[d] = src.meta[k];
+	// Agent::ComputePreferredVelocity + hasReachedGoal (agent.cpp:52-59, 99-102)
+	const V2 toGoal = v2(g) - v2(p);
+	V2 pref{0.0f, 0.0f};
+	if (!(sqr_mag(toGoal) <= prm.x * prm.x)) pref = normalized(toGoal) * prm.y;
+	dst.pref[d] = f2(pref);
+}
+
+// ------------------------------------------------------------ neighbour search
+// All agents j with double d2 = (qx - x_j)^2 + (qy - y_j)^2 < double(float(r*r)), the
+// query point q being float(position + displacement), the querying agent excluded
+// (AgentKDTree.cpp:35-59, nanoflann.hpp:249-253 and :432-440). Toric worlds run one
+// query per axis displacement whose condition holds, (0,0), +W, -W, +H, -H, and
+// concatenate the results (worldToric.cpp:61-79). Each segment is kept in canonical
+// order (d2 ascending, ties by agent ID); the list keeps the first kcap entries of the
+// concatenation. Returns the list length; `total` is the uncapped length.
+template <int KMAX, int BS>
+UD int find_neighbors(const StepArgs& A, const GridInfo& g, int self, V2 p, float range,
+                      double* d2s, unsigned* nbs, int& total) {
+	const State& S = A.in;
+	const double r2 = (double)(range * range);
+	const float rs = range * 1.0001f + 1e-3f;
+	const int kcap = A.kcap;
+	int filled = 0;
+	total = 0;
+	for (int s = 0; s < 5; ++s) {
+		float dx = 0.0f, dy = 0.0f;
+		if (s > 0) {
+			if (A.world_type != 1) break;
+			if (s == 1) { if (!((double)(p.x - range) < -0.5 * (double)A.W)) continue; dx = A.W; }
+			else if (s == 2) { if (!((double)(p.x + range) > 0.5 * (double)A.W)) continue; dx = -A.W; }
+			else if (s == 3) { if (!((double)(p.y - range) < -0.5 * (double)A.H)) continue; dy = A.H; }
+			else { if (!((double)(p.y + range) > 0.5 * (double)A.H)) continue; dy = -A.H; }
+		}
+		const V2 q{p.x + dx, p.y + dy};
+		// cell range: a margin covers the rounding of r2 and of the float bounds
+		const float m = rs + (fabsf(q.x) + fabsf(q.y)) * 1e-6f;
+		const int cx0 = bin_axis(q.x - m, g.ox, g.invx, g.ncx), cx1 = bin_axis(q.x + m, g.ox, g.invx, g.ncx);
+		const int cy0 = bin_axis(q.y - m, g.oy, g.invy, g.ncy), cy1 = bin_axis(q.y + m, g.oy, g.invy, g.ncy);
+		const double qx = (double)q.x, qy = (double)q.y;
+		const int seg_base = filled;
+		const int cap = kcap - filled;
+		int seg_cnt = 0;
+		for (int cy = cy0; cy <= cy1; ++cy) {
+			const int row = cy * g.ncx;
+			const unsigned b = A.cell_start[row + cx0], e = A.cell_start[row + cx1 + 1];
+			for (unsigned j = b; j < e; ++j) {
+				if ((int)j == self) continue;
+				const float2 pj = S.pos[j];
+				const double ddx = qx - (double)pj.x;
+				const double ddy = qy - (double)pj.y;
+				const double d2 = ddx * ddx + ddy * ddy;
+				if (!(d2 < r2)) continue;
+				++total;
+				if (cap <= 0) continue;
+				int at;
+				if (seg_cnt < cap) {
+					at = seg_base + seg_cnt;
+					++seg_cnt;
+				} else {
+					const int last = seg_base + cap - 1;
+					const double dl = d2s[last * BS];
+					if (d2 > dl) continue;
+					if (d2 == dl && S.meta[j].x > S.meta[nbs[last * BS] & kIndexMask].x) continue;
+					at = last;
+				}
+				while (at > seg_base) {
+					const double dp = d2s[(at - 1) * BS];
+					const unsigned np = nbs[(at - 1) * BS];
+					if (d2 > dp) break;
+					if (d2 == dp && S.meta[j].x > S.meta[np & kIndexMask].x) break;
+					d2s[at * BS] = dp;
+					nbs[at * BS] = np;
+					--at;
+				}
+				d2s[at * BS] = d2;
+				nbs[at * BS] = j | ((unsigned)s << kSegShift);
+			}
+		}
+		filled += seg_cnt;
+	}
+	return filled;
+}
+
+// --------------------------------------------------------------- per-agent state
+template <int KMAX>
+struct Prep {
+	V2 target[4];            // force-based costs: target velocity
+	float ttcp[4];           // Karamouzas: TTC at the preferred velocity
+	Line lines[KMAX];        // ORCA half-planes
+	int n_lines;
+};
+
+// CostFunction::GetCost of cost c at velocity v
+template <int KMAX>
+UD float cost_of(const DevPolicy& P, int c, const Self& a, const NbList& L, const Prep<KMAX>& pr, V2 v) {
+	const int kind = P.kind[c];
+	const float* prm = P.params[c];
+	const float range = P.range[c];
+	switch (kind) {
+	case KIND_GOAL_REACHING: case KIND_POWER_LAW: case KIND_SOCIAL_FORCES:
+		return 0.5f * sqr_mag(v - pr.target[c]) / a.dt;     // ForceBasedFunction.cpp:26-30
+	case KIND_ORCA: return orca_cost(pr.lines, pr.n_lines, a, v);
+	case KIND_KARAMOUZAS: return karamouzas_cost(prm, range, a, L, v, pr.ttcp[c]);
+	case KIND_RVO: return rvo_cost(prm, range, a, L, v);
+	case KIND_MOUSSAID: return moussaid_cost(prm, range, a, L, v);
+	case KIND_TTCA_DCA: return ttcadca_cost(prm, range, a, L, v);
+	case KIND_PARIS: return paris_cost(prm, range, a, L, v);
+	case KIND_VAN_TOLL: return vantoll_cost(prm, range, a, L, v);
+	}
+	return 0.0f;
+}
+
+// CostFunction::GetGradient: analytic for force-based costs (ForceBasedFunction.cpp:32-36)
+// and TtcaDca; central differences with delta 0.1 otherwise (costFunction.cpp:36-54).
+template <int KMAX>
+UD V2 gradient_of(const DevPolicy& P, int c, const Self& a, const NbList& L, const Prep<KMAX>& pr, V2 v) {
+	const int kind = P.kind[c];
+	if (is_force_based(kind)) return (v - pr.target[c]) / a.dt;
+	if (kind == KIND_TTCA_DCA) return ttcadca_gradient(P.params[c], P.range[c], a, L, v);
+	const float d = 0.1f;
+	const V2 dx{d, 0.0f}, dy{0.0f, d};
+	const float cl = cost_of<KMAX>(P, c, a, L, pr, v - dx), cr = cost_of<KMAX>(P, c, a, L, pr, v + dx);
+	const float gx = (cr - cl) / (2.0f * d);
+	const float cb = cost_of<KMAX>(P, c, a, L, pr, v - dy), ct = cost_of<KMAX>(P, c, a, L, pr, v + dy);
+	const float gy = (ct - cb) / (2.0f * d);
+	return V2{gx, gy};
+}
+
+// CostFunction::ApproximateGlobalMinimumBySampling (costFunction.cpp:66-164): the first
+// strict minimum of sum(coeff * cost) wins; (0,0) if no candidate beats MaxFloat.
+template <int KMAX>
+UD V2 sample_best(const DevPolicy& P, const Self& a, const NbList& L, const Prep<KMAX>& pr, uint32_t& rng) {
+	const V2 base = P.s_base == 1 ? a.vel : V2{0.0f, 0.0f};
+	float radius = a.pref_speed;
+	if (P.s_radius == 1) radius = a.max_speed;
+	else if (P.s_radius == 2) radius = smin(2.0f * a.max_speed, a.max_accel * a.dt);
+	V2 bdir{1.0f, 0.0f};
+	if (P.s_base_dir == 1) bdir = normalized(a.vel);
+	else if (P.s_base_dir == 2) bdir = normalized(a.pref);
+	V2 best{0.0f, 0.0f};
+	float bestCost = kMaxFloat;
+	auto eval = [&](V2 v) {
+		float tc = 0.0f;
+		for (int c = 0; c < P.n_cost; ++c) tc += P.coeff[c] * cost_of<KMAX>(P, c, a, L, pr, v);
+		if (tc < bestCost) { best = v; bestCost = tc; }
+	};
+	if (P.s_type == 1) {
+		for (int i = 0; i < P.s_random_samples; ++i) {
+			const float ra = rng_uniform(rng, -P.max_angle, P.max_angle);
+			const float rl = rng_uniform(rng, 0.0f, radius);
+			eval(base + rotate_ccw(bdir, ra) * rl);
+		}
+	} else {
+		const int S = P.s_speed_samples, NA = P.s_angle_samples;
+		const float dL = radius / (float)(P.s_include_base ? (S - 1) : S);
+		const float start = -P.max_angle;
+		const float dA = (P.max_angle - start) / (float)P.s_angle_div;
+		for (int s = 1; s <= S; ++s) {
+			const float len = dL * (float)(P.s_include_base ? s - 1 : s);
+			float ang = start;
+			for (int k = 0; k < NA; ++k, ang += dA) {
+				const V2 dir = (k < kMaxAngleSamples) ? rotate_cs(bdir, P.ang_cos[k], P.ang_sin[k])
+				                                      : rotate_ccw(bdir, ang);
+				eval(base + dir * len);
+				if (P.s_include_base && s == 1) break;
+			}
+		}
+	}
+	return best;
+}
+
+// ------------------------------------------------------------------ the step
+template <int KMAX, int BS>
+__global__ __launch_bounds__(BS) void k_step(StepArgs A) {
+	__shared__ double s_d2[KMAX * BS];
+	__shared__ unsigned s_nb[KMAX * BS];
+	const int tid = threadIdx.x;
+	const int k = blockIdx.x * BS + tid;
+	const bool active = k < A.n;
+	int total = 0, cnt = 0;
+	const State& S = A.in;
+	const GridInfo g = *A.grid;
+	int4 meta = make_int4(0, 0, 0, 0);
+	Self a;
+	if (active) {
+		meta = S.meta[k];
+		const float4 prm = S.prm[k];
+		a.pos = v2(S.pos[k]);
+		a.vel = v2(S.vel[k]);
+		a.pref = v2(S.pref[k]);
+		a.radius = prm.x; a.pref_speed = prm.y; a.max_speed = prm.z; a.max_accel = prm.w;
+		a.dt = A.dt;
+		a.relaxation = A.pol[meta.y].relaxation;
+		cnt = find_neighbors<KMAX, BS>(A, g, k, a.pos, A.pol[meta.y].query_range, s_d2 + tid, s_nb + tid, total);
+	}
+	// stats: one atomic per wave
+	{
+		const unsigned long long ov = __ballot(active && total > A.kcap);
+		const int mx = wave_max_i(total);
+		if ((tid & 63) == 0) {
+			const int c = __popcll(ov);
+			if (c) atomicAdd(&A.stats[0], (unsigned long long)c);
+			atomicMax(&A.stats[1], (unsigned long long)mx);
+		}
+	}
+	if (!active) return;
+	const DevPolicy& P = A.pol[meta.y];
+	if (A.rec_count) {
+		A.rec_count[k] = cnt;
+		const int m = cnt < A.rec_stride ? cnt : A.rec_stride;
+		for (int e = 0; e < m; ++e) {
+			const unsigned u = s_nb[e * BS + tid];
+			const long long o = (long long)k * A.rec_stride + e;
+			A.rec_id[o] = S.meta[u & kIndexMask].x;
+			A.rec_d2[o] = s_d2[e * BS + tid];
+			A.rec_seg[o] = (int)(u >> kSegShift);
+		}
+	}
+	const NbList L{s_nb + tid, BS, cnt, S.pos, S.vel, S.prm, S.pref, A.W, A.H, a.pos};
+	uint32_t rng = (uint32_t)meta.w;
+
+	// Policy::ComputeNewVelocity (policy.cpp:43-76)
+	V2 next{0.0f, 0.0f};
+	if (!is_zero(a.pref)) {
+		Prep<KMAX> pr;
+		pr.n_lines = 0;
+		Line proj[KMAX];
+		for (int c = 0; c < P.n_cost; ++c) {
+			const int kind = P.kind[c];
+			if (is_force_based(kind)) pr.target[c] = target_velocity(P, c, a, L);
+			else if (kind == KIND_ORCA) pr.n_lines = orca_lines(P.params[c], P.range[c], a, L, pr.lines);
+			else if (kind == KIND_KARAMOUZAS)
+				pr.ttcp[c] = time_to_first_collision(L, a.pos, a.pref, a.radius, P.range[c], true);
+		}
+		V2 acc;
+		if (P.mode == MODE_GRADIENT) {
+			V2 tg{0.0f, 0.0f};
+			for (int c = 0; c < P.n_cost; ++c) tg = tg + P.coeff[c] * gradient_of<KMAX>(P, c, a, L, pr, a.vel);
+			acc = -1.0f * tg;
+		} else {
+			V2 best;
+			if (P.mode == MODE_GLOBAL_ORCA) best = orca_solve(pr.lines, pr.n_lines, a, proj);
+			else if (P.mode == MODE_GLOBAL_FORCE) best = pr.target[0];
+			else best = sample_best<KMAX>(P, a, L, pr, rng);
+			acc = (best - a.vel) / smax(a.relaxation, a.dt);
+		}
+		acc = clamp_vec(acc, a.max_accel);
+		next = clamp_vec(a.vel + acc * a.dt, a.max_speed);
+	}
+
+	// Policy::ComputeContactForces (policy.cpp:107-141), agents only
+	V2 contact{0.0f, 0.0f};
+	if (P.contact_scale > 0.0f) {
+		for (int e = 0; e < cnt; ++e) {
+			const Nb n = L.get(e);
+			const V2 diff = a.pos - n.pos;
+			const float inter = a.radius + n.radius - mag(diff);
+			if (inter > 0.0f) contact = contact + normalized(diff) * (P.contact_scale * inter);
+		}
+	}
+
+	// Agent::UpdateVelocityAndPosition (agent.cpp:71-93) + toric wrap (worldToric.cpp:81-108)
+	const V2 vel = next + ((contact / 80.0f) * a.dt);
+	V2 view = v2(S.viewdir[k]);
+	const V2 vd = (2.0f * vel + a.pref) / 3.0f;
+	if ((double)sqr_mag(vd) > 0.01) view = normalized(vd);
+	V2 pos = a.pos + (vel * a.dt);
+	if (A.world_type == 1) {
+		const float hw = 0.5f * A.W, hh = 0.5f * A.H;
+		if (pos.x > hw) pos.x -= A.W; else if (pos.x < -hw) pos.x += A.W;
+		if (pos.y > hh) pos.y -= A.H; else if (pos.y < -hh) pos.y += A.H;
+	}
+	A.out.pos[k] = f2(pos);
+	A.out.vel[k] = f2(vel);
+	A.out.viewdir[k] = f2(view);
+	A.out.goal[k] = S.goal[k];
+	A.out.prm[k] = S.prm[k];
+	A.out.pref[k] = f2(a.pref);
+	meta.w = (int)rng;
+	A.out.meta[k] = meta;
+}
+
+} // namespace umans

@@ -1,0 +1,87 @@
+// Types shared by the host side and the kernels of libumans_gpu.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace umans {
+
+enum {
+	KIND_GOAL_REACHING = 0, KIND_POWER_LAW = 1, KIND_SOCIAL_FORCES = 2, KIND_ORCA = 3,
+	KIND_KARAMOUZAS = 4, KIND_RVO = 5, KIND_MOUSSAID = 6, KIND_TTCA_DCA = 7,
+	KIND_PARIS = 8, KIND_VAN_TOLL = 9
+};
+
+// How a policy turns costs into a new velocity (Policy::ComputeNewVelocity,
+// policy.cpp:43-76, with getBestVelocityGlobal resolved on the host).
+enum {
+	MODE_GRADIENT = 0,       // a = -sum coeff * gradient (policy.cpp:78-89)
+	MODE_SAMPLE = 1,         // best = argmin over candidates (costFunction.cpp:66-164)
+	MODE_GLOBAL_ORCA = 2,    // best = ORCA LP solution (ORCA.cpp:88-95)
+	MODE_GLOBAL_FORCE = 3    // best = target velocity of the force model (ForceBasedFunction.cpp:38-41)
+};
+
+constexpr unsigned kSegShift = 28;
+constexpr unsigned kIndexMask = (1u << kSegShift) - 1u;
+constexpr int kMaxAngleSamples = 64;
+
+// A policy as the kernels read it. The sampling fields are the effective ones (for a
+// global policy that falls back to sampling: SamplingParameters::
+// ApproximateGlobalOptimization, policy.h:74-86).
+struct DevPolicy {
+	int mode;
+	int s_type, s_base, s_base_dir, s_radius;
+	int s_speed_samples, s_angle_samples, s_random_samples, s_include_base;
+	int s_angle_div;         // angle == 360 ? angle_samples : angle_samples - 1
+	float max_angle;         // (float)(angle / 360.0 * PI), costFunction.cpp:90
+	float relaxation, contact_scale;
+	float query_range;       // Policy::getInteractionRange, policy.cpp:34-41
+	int n_cost;
+	int orca_cost;           // index of the ORCA cost or -1
+	int kind[4];
+	float coeff[4];          // effective coefficients (1 for a single-cost global fallback)
+	float range[4];
+	float params[4][8];
+	float ang_cos[kMaxAngleSamples];   // regular sampling: cosf/sinf of the float angle
+	float ang_sin[kMaxAngleSamples];   // sequence start, start + dA, ... (host libm)
+};
+
+struct GridInfo {
+	float ox, oy;            // origin of cell (0, 0)
+	float invx, invy;        // cells per metre
+	int ncx, ncy;
+	int ncells;
+	int pad;
+};
+
+// Cell-sorted (after reorder) agent state, structure of arrays.
+struct State {
+	float2* pos;
+	float2* vel;
+	float2* goal;
+	float2* viewdir;
+	float2* pref;            // preferred velocity of the current step
+	float4* prm;             // radius, preferred speed, maximum speed, maximum acceleration
+	int4* meta;              // agent ID, policy, upload row, RNG state
+};
+
+struct StepArgs {
+	State in;                // cell-sorted state of this step (read only)
+	State out;               // next state, same order
+	const DevPolicy* pol;
+	const GridInfo* grid;
+	const unsigned* cell_start;   // ncells + 1
+	int n;
+	int world_type;
+	float W, H;
+	float dt;
+	int kcap;                // neighbour cap (<= KMAX of the kernel instance)
+	// optional neighbour record (null when not recording)
+	int* rec_count;
+	int* rec_id;
+	double* rec_d2;
+	int* rec_seg;
+	int rec_stride;
+	unsigned long long* stats;    // [0] overflow agents, [1] max list length
+};
+
+} // namespace umans

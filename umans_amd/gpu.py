@@ -1,0 +1,226 @@
+"""ctypes binding of libumans_gpu.so (the C ABI in include/umans_gpu.h).
+
+`World` mirrors what the reference's CrowdSimulator + WorldBase expose for the per-step
+loop (crowdSimulator.h:58-96, worldBase.h:191-229): policies by registry name and
+parameters, agents added in bulk, DoStep, and state/neighbour read-back. Every call
+goes through the HIP library; if the library is missing this module raises.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libumans_gpu.so")
+
+_F = C.POINTER(C.c_float)
+_I = C.POINTER(C.c_int32)
+_U = C.POINTER(C.c_uint32)
+_D = C.POINTER(C.c_double)
+
+EXPORTS = [
+    "umans_create", "umans_destroy", "umans_last_error", "umans_set_policies", "umans_upload",
+    "umans_download", "umans_num_agents", "umans_step", "umans_synchronize", "umans_get_stats",
+    "umans_set_record_neighbors", "umans_dump_neighbors", "umans_cost_id", "umans_cost_name",
+    "umans_cost_param_index", "umans_cost_param_name", "umans_cost_defaults", "umans_policy_init",
+    "umans_set_profiling", "umans_get_profile",
+]
+
+
+class WorldDesc(C.Structure):
+    _fields_ = [("world_type", C.c_int32), ("width", C.c_float), ("height", C.c_float),
+                ("delta_time", C.c_float), ("max_neighbors", C.c_int32), ("cell_size", C.c_float),
+                ("block_size", C.c_int32), ("device", C.c_int32)]
+
+
+class PolicyC(C.Structure):
+    _fields_ = [("method", C.c_int32), ("s_type", C.c_int32), ("s_base", C.c_int32),
+                ("s_base_dir", C.c_int32), ("s_radius", C.c_int32), ("s_angle", C.c_float),
+                ("s_speed_samples", C.c_int32), ("s_angle_samples", C.c_int32),
+                ("s_random_samples", C.c_int32), ("s_include_base", C.c_int32),
+                ("relaxation_time", C.c_float), ("contact_force_scale", C.c_float),
+                ("n_cost", C.c_int32), ("cost_kind", C.c_int32 * 4), ("cost_coeff", C.c_float * 4),
+                ("cost_range", C.c_float * 4), ("cost_params", (C.c_float * 8) * 4)]
+
+
+class AgentsC(C.Structure):
+    _fields_ = [("id", _I), ("pos", _F), ("vel", _F), ("goal", _F), ("viewdir", _F), ("radius", _F),
+                ("pref_speed", _F), ("max_speed", _F), ("max_accel", _F), ("policy", _I), ("rng", _U),
+                ("pref_vel", _F)]
+
+
+class Stats(C.Structure):
+    _fields_ = [("steps", C.c_int64), ("overflow_agents", C.c_int64), ("max_list", C.c_int64),
+                ("sim_time", C.c_double)]
+
+
+class Profile(C.Structure):
+    _fields_ = [("steps", C.c_int64), ("grid_ms", C.c_double), ("step_ms", C.c_double)]
+
+
+class UmansError(RuntimeError):
+    pass
+
+
+_lib = None
+
+
+def lib():
+    """Load libumans_gpu.so (raises if it has not been built)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise UmansError(f"{LIB_PATH} is missing: build it with `python -m umans_amd.build`")
+        L = C.CDLL(LIB_PATH)
+        vp = C.c_void_p
+        L.umans_create.argtypes = [C.POINTER(WorldDesc), C.POINTER(vp)]
+        L.umans_destroy.argtypes = [vp]
+        L.umans_destroy.restype = None
+        L.umans_last_error.argtypes = [vp]
+        L.umans_last_error.restype = C.c_char_p
+        L.umans_set_policies.argtypes = [vp, C.POINTER(PolicyC), C.c_int]
+        L.umans_upload.argtypes = [vp, C.POINTER(AgentsC), C.c_int]
+        L.umans_download.argtypes = [vp, C.POINTER(AgentsC)]
+        L.umans_num_agents.argtypes = [vp]
+        L.umans_step.argtypes = [vp, C.c_int]
+        L.umans_synchronize.argtypes = [vp]
+        L.umans_get_stats.argtypes = [vp, C.POINTER(Stats)]
+        L.umans_set_record_neighbors.argtypes = [vp, C.c_int]
+        L.umans_dump_neighbors.argtypes = [vp, C.c_int, _I, _I, _D, _I]
+        L.umans_cost_id.argtypes = [C.c_char_p]
+        L.umans_cost_name.argtypes = [C.c_int]
+        L.umans_cost_name.restype = C.c_char_p
+        L.umans_cost_param_index.argtypes = [C.c_int, C.c_char_p]
+        L.umans_cost_param_name.argtypes = [C.c_int, C.c_int]
+        L.umans_cost_param_name.restype = C.c_char_p
+        L.umans_cost_defaults.argtypes = [C.c_int, _F, _F]
+        L.umans_policy_init.argtypes = [C.POINTER(PolicyC), C.c_int]
+        L.umans_set_profiling.argtypes = [vp, C.c_int]
+        L.umans_get_profile.argtypes = [vp, C.POINTER(Profile)]
+        _lib = L
+    return _lib
+
+
+def _p(a, t):
+    return None if a is None else a.ctypes.data_as(t)
+
+
+def policy_struct(packed):
+    q = PolicyC()
+    for k in ("method", "s_type", "s_base", "s_base_dir", "s_radius", "s_speed_samples",
+              "s_angle_samples", "s_random_samples", "s_include_base", "n_cost"):
+        setattr(q, k, int(packed[k]))
+    for k in ("s_angle", "relaxation_time", "contact_force_scale"):
+        setattr(q, k, float(packed[k]))
+    for c in range(4):
+        q.cost_kind[c] = int(packed["cost_kind"][c])
+        q.cost_coeff[c] = float(packed["cost_coeff"][c])
+        q.cost_range[c] = float(packed["cost_range"][c])
+        for j in range(8):
+            q.cost_params[c][j] = float(packed["cost_params"][c][j])
+    return q
+
+
+class World:
+    """One simulated world on one GPU (a context of the C ABI)."""
+
+    def __init__(self, world_type, width, height, dt, policies, max_neighbors=0, cell_size=0.0, device=0):
+        L = lib()
+        self._h = C.c_void_p()
+        desc = WorldDesc(int(world_type), float(width), float(height), float(dt), int(max_neighbors),
+                         float(cell_size), 0, int(device))
+        rc = L.umans_create(C.byref(desc), C.byref(self._h))
+        if rc != 0:
+            raise UmansError(f"umans_create failed ({rc})")
+        self.desc = desc
+        self._check(L.umans_set_policies(self._h, (PolicyC * len(policies))(*[policy_struct(p) for p in policies]),
+                                         len(policies)))
+        self.n = 0
+
+    def _check(self, rc):
+        if rc != 0:
+            msg = lib().umans_last_error(self._h)
+            raise UmansError(f"umans call failed ({rc}): {msg.decode() if msg else ''}")
+
+    def close(self):
+        if getattr(self, "_h", None):
+            lib().umans_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def upload(self, agents):
+        n = len(agents["id"])
+        keep = {}
+
+        def arr(k, t, dtype, required=True):
+            if k not in agents or agents[k] is None:
+                if required:
+                    raise KeyError(k)
+                return None
+            keep[k] = np.ascontiguousarray(agents[k], dtype=dtype)
+            return _p(keep[k], t)
+        a = AgentsC(arr("id", _I, np.int32), arr("pos", _F, np.float32), arr("vel", _F, np.float32, False),
+                    arr("goal", _F, np.float32), arr("viewdir", _F, np.float32, False),
+                    arr("radius", _F, np.float32), arr("pref_speed", _F, np.float32),
+                    arr("max_speed", _F, np.float32), arr("max_accel", _F, np.float32),
+                    arr("policy", _I, np.int32), arr("rng", _U, np.uint32, False), None)
+        self._check(lib().umans_upload(self._h, C.byref(a), n))
+        self.n = n
+
+    def step(self, nsteps=1):
+        self._check(lib().umans_step(self._h, int(nsteps)))
+
+    def synchronize(self):
+        self._check(lib().umans_synchronize(self._h))
+
+    def record_neighbors(self, enable=True):
+        self._check(lib().umans_set_record_neighbors(self._h, 1 if enable else 0))
+
+    def download(self):
+        n = self.n
+        out = {k: np.zeros((n, 2), np.float32) for k in ("pos", "vel", "viewdir", "goal", "pref_vel")}
+        for k in ("radius", "pref_speed", "max_speed", "max_accel"):
+            out[k] = np.zeros(n, np.float32)
+        out["id"] = np.zeros(n, np.int32)
+        out["policy"] = np.zeros(n, np.int32)
+        out["rng"] = np.zeros(n, np.uint32)
+        a = AgentsC(_p(out["id"], _I), _p(out["pos"], _F), _p(out["vel"], _F), _p(out["goal"], _F),
+                    _p(out["viewdir"], _F), _p(out["radius"], _F), _p(out["pref_speed"], _F),
+                    _p(out["max_speed"], _F), _p(out["max_accel"], _F), _p(out["policy"], _I),
+                    _p(out["rng"], _U), _p(out["pref_vel"], _F))
+        self._check(lib().umans_download(self._h, C.byref(a)))
+        return out
+
+    def neighbors(self, stride=128):
+        n = self.n
+        cnt = np.zeros(n, np.int32)
+        ids = np.full((n, stride), -1, np.int32)
+        d2 = np.zeros((n, stride), np.float64)
+        seg = np.zeros((n, stride), np.int32)
+        self._check(lib().umans_dump_neighbors(self._h, stride, _p(cnt, _I), _p(ids, _I), _p(d2, _D), _p(seg, _I)))
+        return cnt, ids, d2, seg
+
+    def set_profiling(self, enable=True):
+        self._check(lib().umans_set_profiling(self._h, 1 if enable else 0))
+
+    def profile(self):
+        p = Profile()
+        self._check(lib().umans_get_profile(self._h, C.byref(p)))
+        return {"steps": p.steps, "grid_ms": p.grid_ms, "step_ms": p.step_ms}
+
+    def stats(self):
+        s = Stats()
+        self._check(lib().umans_get_stats(self._h, C.byref(s)))
+        return {"steps": s.steps, "overflow_agents": s.overflow_agents, "max_list": s.max_list,
+                "sim_time": s.sim_time}
+
+
+def world_from_scenario(sc, cell_size=0.0, device=0):
+    w = World(sc.world_type, sc.width, sc.height, sc.dt, sc.policies, sc.max_neighbors, cell_size, device)
+    w.upload(sc.agents)
+    return w
