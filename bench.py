@@ -31,6 +31,8 @@ def workload(name):
         return S.bidirectional(), RL.flops_karamouzas_sampling()
     if name == "circle":
         return S.circle(50), RL.flops_orca(10)
+    if name == "sweep":
+        return S.sweep(), RL.flops_powerlaw_sampling(30, 72)
     raise SystemExit(f"unknown workload {name}")
 
 
@@ -70,16 +72,42 @@ def cpu_baseline(sc, steps, threads):
             "sample": f"{sc.n} agents x {steps} steps of the same workload ({t:.1f} s)"}
 
 
+def copy_probe(mib):
+    """hipMemcpy device-to-device of `mib` MiB, three times, through the HIP runtime."""
+    import ctypes as C
+    hip = C.CDLL("libamdhip64.so")
+    nbytes = mib << 20
+    src, dst = C.c_void_p(), C.c_void_p()
+    assert hip.hipMalloc(C.byref(src), C.c_size_t(nbytes)) == 0
+    assert hip.hipMalloc(C.byref(dst), C.c_size_t(nbytes)) == 0
+    assert hip.hipMemset(src, 1, C.c_size_t(nbytes)) == 0
+    for _ in range(3):
+        assert hip.hipMemcpy(dst, src, C.c_size_t(nbytes), 3) == 0   # hipMemcpyDeviceToDevice
+    assert hip.hipDeviceSynchronize() == 0
+    hip.hipFree(src)
+    hip.hipFree(dst)
+    print(json.dumps({"copy_probe_bytes": nbytes, "copies": 3}))
+    return 0
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=50)
+    ap.add_argument("--steps", type=int, default=100)
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--workload", default="mixed")
     ap.add_argument("--cpu-steps", type=int, default=3)
     ap.add_argument("--cpu-threads", type=int, default=min(16, os.cpu_count() or 1))
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-autotune", action="store_true")
+    ap.add_argument("--graph", action="store_true", help="replay a captured hipGraph per step "
+                    "(no per-kernel timing then)")
+    ap.add_argument("--copy-probe-mib", type=int, default=0,
+                    help="only run a device-to-device copy of this size (3 times): the known-size "
+                         "reference that scales the FETCH_SIZE / WRITE_SIZE counters")
     args = ap.parse_args()
+    if args.copy_probe_mib:
+        return copy_probe(args.copy_probe_mib)
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for {args.gpus}); the multi-GPU design is "
               "documented, not run.", file=sys.stderr)
@@ -87,11 +115,18 @@ def main():
 
     from umans_amd import gpu
     sc, flops_per_agent = workload(args.workload)
-    w = gpu.World(sc.world_type, sc.width, sc.height, sc.dt, sc.policies, sc.max_neighbors)
+    batched = isinstance(sc, S.Batch)
+    if batched:
+        w = gpu.World(0, 0, 0, 0.1, sc.policies, max_neighbors=sc.max_neighbors, batch=sc.worlds)
+    else:
+        w = gpu.World(sc.world_type, sc.width, sc.height, sc.dt, sc.policies, sc.max_neighbors)
     w.upload(sc.agents)
+    tuning = None if args.no_autotune else w.autotune(trial_steps=3)
+    if args.graph:
+        w.set_graph(True)
     w.step(args.warmup)
     w.synchronize()
-    w.set_profiling(True)
+    w.set_profiling(not args.graph)
     w.synchronize()
     t0 = time.perf_counter()
     w.step(args.steps)
@@ -103,12 +138,15 @@ def main():
 
     n = sc.n
     value = n * args.steps / t
+    if args.graph:      # no per-kernel events inside a graph: attribute the whole step
+        prof = {"steps": args.steps, "step_ms": t * 1e3, "grid_ms": 0.0}
     step_s = prof["step_ms"] / 1e3 / max(prof["steps"], 1)
     grid_s = prof["grid_ms"] / 1e3 / max(prof["steps"], 1)
     achieved = RL.STEP_BYTES * n / step_s / 1e9
     valu = flops_per_agent * n / step_s / 1e12
     out = {
-        "metric": "agent-steps/s at 1M agents" if sc.name.startswith("mixed") else "agent-steps/s",
+        "metric": ("agent-steps/s at 1M agents" if sc.name.startswith("mixed") else
+                   "batched agent-steps/s" if batched else "agent-steps/s"),
         "value": value,
         "unit": "agent-steps/s",
         "n_gpus": 1,
@@ -120,10 +158,15 @@ def main():
         "vs_baseline": None,
         "dtype": "f32",
         "data": "synthetic (seeded generator)",
-        "config": {"workload": sc.name, "agents": n, "world": "toric" if sc.world_type == 1 else "infinite",
-                   "size_m": sc.width, "dt": sc.dt, "max_neighbors": sc.max_neighbors,
-                   "policies": "40% PowerLaw+sampling 12x6, 30% SocialForces+gradient, 30% ORCA"
-                   if sc.name.startswith("mixed") else sc.name, "parallelism": "single GPU"},
+        "config": ({"workload": sc.name, "agents": n, "worlds": len(sc.worlds), "world": "infinite",
+                    "dt": 0.1, "max_neighbors": sc.max_neighbors,
+                    "policies": "per world PowerLaw(k~U[1,3], tau0~U[1,5]) + GoalReachingForce, sampling 12x6",
+                    "parallelism": "single GPU, all worlds in each launch"} if batched else
+                   {"workload": sc.name, "agents": n, "world": "toric" if sc.world_type == 1 else "infinite",
+                    "size_m": sc.width, "dt": sc.dt, "max_neighbors": sc.max_neighbors,
+                    "policies": "40% PowerLaw+sampling 12x6, 30% SocialForces+gradient, 30% ORCA"
+                    if sc.name.startswith("mixed") else sc.name, "parallelism": "single GPU"}),
+        "tuning": tuning, "graph": bool(args.graph),
         "roofline": {"bound": "hbm", "kernel": "k_step", "achieved": achieved, "peak": RL.HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": achieved / RL.HBM_PEAK_GBS, "traffic": None,
                      "bytes_per_agent": RL.STEP_BYTES, "avg_launch_ms": step_s * 1e3,
@@ -132,7 +175,7 @@ def main():
         "phases_ms": {"grid": grid_s * 1e3, "step": step_s * 1e3},
         "neighbor_overflow_agents": st["overflow_agents"],
     }
-    if not args.no_cpu_baseline:
+    if not args.no_cpu_baseline and not batched:
         print(f"# GPU leg done ({value:.3e} agent-steps/s); timing the CPU baseline", file=sys.stderr, flush=True)
         out["cpu_baseline"] = cpu_baseline(sc, args.cpu_steps, args.cpu_threads)
         out["cpu_baseline"]["gpu_over_cpu"] = value / out["cpu_baseline"]["value"]

@@ -105,6 +105,7 @@ typedef struct {
 	int32_t* policy;
 	uint32_t* rng;
 	float* pref_vel;           /* download only: preferred velocity of the last step */
+	int32_t* world;            /* world of each agent in a batch (null: world 0) */
 } umans_agents;
 
 typedef struct {
@@ -121,6 +122,14 @@ int umans_create(const umans_world_desc* desc, umans_ctx** out);
 void umans_destroy(umans_ctx* ctx);
 const char* umans_last_error(const umans_ctx* ctx);
 
+/* Batched worlds: n_worlds independent worlds (each with its own type, size and
+ * delta time) stepped together, every phase one launch over all their agents. The
+ * reference runs one world per CrowdSimulator (crowdSimulator.h:37); a batch is the
+ * same as n_worlds simulators stepped in lock-step. Neighbour cap and block size are
+ * taken from worlds[0]. */
+int umans_create_batch(const umans_world_desc* worlds, int n_worlds, umans_ctx** out);
+int umans_num_worlds(const umans_ctx* ctx);
+
 /* Policies (CrowdSimulator::AddPolicy, crowdSimulator.h:91). Policy i gets ID i. */
 int umans_set_policies(umans_ctx* ctx, const umans_policy* policies, int n);
 
@@ -134,6 +143,23 @@ int umans_num_agents(const umans_ctx* ctx);
 int umans_step(umans_ctx* ctx, int nsteps);
 int umans_synchronize(umans_ctx* ctx);
 int umans_get_stats(umans_ctx* ctx, umans_stats* out);
+
+/* Capture one step (every kernel, counters cleared by kernels, no memsets) in a
+ * hipGraph and replay it for each step. Off by default; not used while profiling. */
+int umans_set_graph(umans_ctx* ctx, int enable);
+
+/* Launch tuning: the cell width as a factor of the largest interaction range and
+ * the step-kernel block size. Neither changes any result (neighbour lists are in
+ * canonical order whatever the grid). umans_autotune times trial_steps steps per
+ * candidate from a saved copy of the state, restores the state, and keeps the
+ * fastest; it does not advance the simulation. */
+typedef struct {
+	float cell_factor;
+	int32_t block_size;
+	float ms_per_step;         /* autotune: time of the chosen configuration */
+} umans_tuning;
+int umans_autotune(umans_ctx* ctx, int trial_steps, umans_tuning* out);
+int umans_set_tuning(umans_ctx* ctx, const umans_tuning* tuning);
 
 /* Per-phase GPU timers (hipEvents on the context's stream), accumulated over the
  * steps run while enabled: grid = cell keys + scan + reorder (+ bounding box in

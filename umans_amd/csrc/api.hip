@@ -14,7 +14,8 @@ using namespace umans;
 
 namespace {
 
-constexpr int kMaxPolicies = 1024;
+constexpr int kMaxPolicies = 1 << kPolicyBits;
+constexpr int kMaxWorlds = (1 << (31 - kPolicyBits)) - 1;
 constexpr int kUncappedMax = 128;
 
 template <typename T> hipError_t dalloc(T** p, size_t count) {
@@ -40,10 +41,63 @@ struct DeviceState {
 	}
 };
 
+hipError_t copy_state(const State& dst, const State& src, int n, hipStream_t st) {
+	hipError_t e;
+	if ((e = hipMemcpyAsync(dst.pos, src.pos, n * sizeof(float2), hipMemcpyDeviceToDevice, st))) return e;
+	if ((e = hipMemcpyAsync(dst.vel, src.vel, n * sizeof(float2), hipMemcpyDeviceToDevice, st))) return e;
+	if ((e = hipMemcpyAsync(dst.goal, src.goal, n * sizeof(float2), hipMemcpyDeviceToDevice, st))) return e;
+	if ((e = hipMemcpyAsync(dst.viewdir, src.viewdir, n * sizeof(float2), hipMemcpyDeviceToDevice, st))) return e;
+	if ((e = hipMemcpyAsync(dst.pref, src.pref, n * sizeof(float2), hipMemcpyDeviceToDevice, st))) return e;
+	if ((e = hipMemcpyAsync(dst.prm, src.prm, n * sizeof(float4), hipMemcpyDeviceToDevice, st))) return e;
+	return hipMemcpyAsync(dst.meta, src.meta, n * sizeof(int4), hipMemcpyDeviceToDevice, st);
+}
+
+int kmax_bucket(int kcap) {
+	if (kcap <= 16) return 16;
+	if (kcap <= 32) return 32;
+	if (kcap <= 64) return 64;
+	return 128;
+}
+
+// Block sizes instantiated per neighbour-list capacity (the LDS list is KMAX x block
+// entries of 12 B).
+bool block_supported(int kmax, int bs) {
+	switch (kmax) {
+	case 16: return bs == 64 || bs == 128 || bs == 256;
+	case 32: return bs == 64 || bs == 128;
+	default: return bs == 64;
+	}
+}
+int default_block(int kmax) { return kmax <= 32 ? 128 : 64; }
+
+template <int KMAX, int BS>
+void launch_step_t(const StepArgs& a, hipStream_t st) {
+	const int blocks = (a.n + BS - 1) / BS;
+	if (blocks > 0) hipLaunchKernelGGL((k_step<KMAX, BS>), dim3(blocks), dim3(BS), 0, st, a);
+}
+
+void launch_step(int kmax, int bs, const StepArgs& a, hipStream_t st) {
+	switch (kmax) {
+	case 16:
+		if (bs == 64) launch_step_t<16, 64>(a, st);
+		else if (bs == 256) launch_step_t<16, 256>(a, st);
+		else launch_step_t<16, 128>(a, st);
+		break;
+	case 32:
+		if (bs == 64) launch_step_t<32, 64>(a, st);
+		else launch_step_t<32, 128>(a, st);
+		break;
+	case 64: launch_step_t<64, 64>(a, st); break;
+	default: launch_step_t<128, 64>(a, st); break;
+	}
+}
+
+int ceil_div(long long a, long long b) { return (int)((a + b - 1) / b); }
+
 } // namespace
 
 struct umans_ctx {
-	umans_world_desc desc{};
+	std::vector<umans_world_desc> worlds;   // one per world of the batch
 	int device = 0;
 	hipStream_t stream = nullptr;
 	std::string err;
@@ -54,17 +108,22 @@ struct umans_ctx {
 
 	int n = 0, cap = 0;
 	DeviceState X, Y;            // X: state between steps, Y: cell-sorted state of a step
+	DeviceState B;               // backup of X used by the autotuner
 	unsigned* cell_count = nullptr;
 	unsigned* cell_start = nullptr;
 	unsigned* scan_sums = nullptr;
 	unsigned* cell_of = nullptr;
 	unsigned* rank = nullptr;
-	int cell_cap = 0;
-	int scan_cells = 0;          // cells the scan covers (toric: the grid; infinite: cell_cap)
-	GridInfo* dgrid = nullptr;
+	int cell_alloc = 0;          // allocated cell counters (>= total_cells + 1)
+	int total_cells = 0;         // sum of the worlds' cell_cap
+	WorldInfo* dworlds = nullptr;
+	GridInfo* dgrids = nullptr;
 	int* dbbox = nullptr;
+	bool any_infinite = false;
 	unsigned long long* dstats = nullptr;
 	int kcap = 0;
+	float cell_factor = 1.0f;    // cell width = factor x max interaction range (autotuned)
+	int block = 0;               // step-kernel block size (0 = default; autotuned)
 
 	int record = 0;
 	int* rec_count = nullptr;
@@ -74,12 +133,17 @@ struct umans_ctx {
 	int rec_stride = 0;
 	bool have_record = false;
 
+	// hipGraph of one step (kernels only: counters are cleared by a fill kernel)
+	int use_graph = 0;
+	hipGraphExec_t graph_exec = nullptr;
+	hipGraph_t graph = nullptr;
+
 	long long steps = 0;
 	double sim_time = 0.0;
 
 	// per-phase timers: one event triple per profiled step, read back lazily
 	int profiling = 0;
-	std::vector<hipEvent_t> ev;          // 3 per pending step
+	std::vector<hipEvent_t> ev;
 	int ev_used = 0;
 	long long prof_steps = 0;
 	double prof_grid_ms = 0.0, prof_step_ms = 0.0;
@@ -93,16 +157,22 @@ struct umans_ctx {
 		err = std::string(what) + ": " + hipGetErrorString(e);
 		return UMANS_EHIP;
 	}
+	void drop_graph() {
+		if (graph_exec) hipGraphExecDestroy(graph_exec);
+		if (graph) hipGraphDestroy(graph);
+		graph_exec = nullptr;
+		graph = nullptr;
+	}
 	void free_agents() {
-		X.release(); Y.release();
-		hipFree(cell_count); hipFree(cell_start); hipFree(scan_sums); hipFree(cell_of); hipFree(rank);
+		drop_graph();
+		X.release(); Y.release(); B.release();
+		hipFree(cell_of); hipFree(rank);
 		hipFree(rec_count); hipFree(rec_id); hipFree(rec_d2); hipFree(rec_seg);
-		cell_count = cell_start = scan_sums = cell_of = rank = nullptr;
+		cell_of = rank = nullptr;
 		rec_count = rec_id = rec_seg = nullptr;
 		rec_d2 = nullptr;
-		n = cap = cell_cap = 0;
+		n = cap = 0;
 	}
-	// read back and accumulate the pending event triples
 	int drain_profile() {
 		for (int i = 0; i + 2 < ev_used; i += 3) {
 			float g = 0.0f, s = 0.0f;
@@ -129,6 +199,9 @@ struct umans_ctx {
 		for (hipEvent_t e : ev) hipEventDestroy(e);
 		ev.clear();
 		ev_used = 0;
+	}
+	float cell_size_of(const umans_world_desc& d) const {
+		return d.cell_size > 0.0f ? d.cell_size : std::max(cell_factor * max_range, 0.25f);
 	}
 };
 
@@ -194,11 +267,9 @@ int make_dev_policy(const umans_policy& p, DevPolicy& d, std::string& err) {
 	d.s_type = s.s_type; d.s_base = s.s_base; d.s_base_dir = s.s_base_dir; d.s_radius = s.s_radius;
 	d.s_speed_samples = s.s_speed_samples; d.s_angle_samples = s.s_angle_samples;
 	d.s_random_samples = s.s_random_samples; d.s_include_base = s.s_include_base ? 1 : 0;
-	if (d.mode == MODE_SAMPLE) {
-		if (s.s_type == 0 && (s.s_speed_samples < 1 || s.s_angle_samples < 1)) {
-			err = "regular sampling needs SpeedSamples >= 1 and AngleSamples >= 1";
-			return UMANS_EINVAL;
-		}
+	if (d.mode == MODE_SAMPLE && s.s_type == 0 && (s.s_speed_samples < 1 || s.s_angle_samples < 1)) {
+		err = "regular sampling needs SpeedSamples >= 1 and AngleSamples >= 1";
+		return UMANS_EINVAL;
 	}
 	// costFunction.cpp:90 and :122-125, in the reference's float/double mix
 	const float maxAngle = (float)(s.s_angle / 360.0 * 3.1415926535897);
@@ -214,53 +285,73 @@ int make_dev_policy(const umans_policy& p, DevPolicy& d, std::string& err) {
 	return UMANS_OK;
 }
 
-int kmax_bucket(int kcap) {
-	if (kcap <= 16) return 16;
-	if (kcap <= 32) return 32;
-	if (kcap <= 64) return 64;
-	return 128;
+int check_world(const umans_world_desc& d) {
+	if (d.world_type != 0 && d.world_type != 1) return UMANS_EINVAL;
+	if (!(d.delta_time > 0.0f)) return UMANS_EINVAL;   // crowdSimulator.cpp:575-584
+	if (d.world_type == 1 && !(d.width > 0.0f && d.height > 0.0f)) return UMANS_EINVAL;  // :194-204
+	return UMANS_OK;
 }
 
-template <int KMAX, int BS>
-void launch_step_t(const StepArgs& a, hipStream_t st) {
-	const int blocks = (a.n + BS - 1) / BS;
-	if (blocks > 0) hipLaunchKernelGGL((k_step<KMAX, BS>), dim3(blocks), dim3(BS), 0, st, a);
-}
-
-void launch_step(int kmax, const StepArgs& a, hipStream_t st) {
-	switch (kmax) {
-	case 16: launch_step_t<16, 128>(a, st); break;
-	case 32: launch_step_t<32, 128>(a, st); break;
-	case 64: launch_step_t<64, 64>(a, st); break;
-	default: launch_step_t<128, 64>(a, st); break;
+// Per-world cell reservations and WorldInfo records.
+int setup_worlds(umans_ctx* c, const std::vector<int>& per_world) {
+	const int nw = (int)c->worlds.size();
+	std::vector<WorldInfo> wi(nw);
+	long long off = 0;
+	c->any_infinite = false;
+	for (int w = 0; w < nw; ++w) {
+		const umans_world_desc& d = c->worlds[w];
+		WorldInfo& r = wi[w];
+		r.world_type = d.world_type;
+		r.W = d.width;
+		r.H = d.height;
+		r.dt = d.delta_time;
+		r.cell_size = c->cell_size_of(d);
+		r.pad = 0;
+		const long long cap_cells = std::max(4LL * per_world[w], 64LL);
+		long long cells;
+		if (d.world_type == 1) {
+			// the same reduction k_grid_setup applies on the device
+			double nx = std::max(1.0, std::floor((double)d.width / (double)r.cell_size));
+			double ny = std::max(1.0, std::floor((double)d.height / (double)r.cell_size));
+			const double lim = (double)std::max(cap_cells, 4096LL);
+			while (nx * ny > lim) {
+				if (nx >= ny) nx = std::max(1.0, std::floor(nx / 2.0)); else ny = std::max(1.0, std::floor(ny / 2.0));
+			}
+			cells = (long long)(nx * ny);
+			r.cell_cap = (int)std::max(cap_cells, 4096LL);
+		} else {
+			cells = cap_cells;
+			r.cell_cap = (int)cells;
+			c->any_infinite = true;
+		}
+		r.cell_offset = (int)off;
+		off += cells;
+		if (off > (1LL << 30)) return c->fail(UMANS_EINVAL, "grid too large");
 	}
-}
-
-int ceil_div(long long a, long long b) { return (int)((a + b - 1) / b); }
-
-// Toric grid: fixed for the world; cells at least `cs` wide.
-GridInfo toric_grid(float W, float H, float cs, int cap) {
-	GridInfo g{};
-	long long nx = std::max(1LL, (long long)std::floor((double)W / cs));
-	long long ny = std::max(1LL, (long long)std::floor((double)H / cs));
-	while (nx * ny > cap) {
-		if (nx >= ny) nx = std::max(1LL, nx / 2); else ny = std::max(1LL, ny / 2);
+	c->total_cells = (int)off;
+	if (c->total_cells + 1 > c->cell_alloc) {
+		hipFree(c->cell_count); hipFree(c->cell_start); hipFree(c->scan_sums);
+		c->cell_count = c->cell_start = c->scan_sums = nullptr;
+		c->cell_alloc = 0;
+		const int cells = c->total_cells + 1;
+		int rc;
+		if ((rc = c->hip(dalloc(&c->cell_count, cells), "alloc grid"))) return rc;
+		if ((rc = c->hip(dalloc(&c->cell_start, cells), "alloc grid"))) return rc;
+		if ((rc = c->hip(dalloc(&c->scan_sums, ceil_div(cells, kScanTile) + 1), "alloc grid"))) return rc;
+		c->cell_alloc = cells;
 	}
-	g.ox = -0.5f * W; g.oy = -0.5f * H;
-	g.ncx = (int)nx; g.ncy = (int)ny;
-	g.invx = (float)((double)nx / W); g.invy = (float)((double)ny / H);
-	g.ncells = g.ncx * g.ncy;
-	return g;
+	return c->hip(hipMemcpyAsync(c->dworlds, wi.data(), nw * sizeof(WorldInfo), hipMemcpyHostToDevice, c->stream),
+	              "world upload");
 }
 
-int enqueue_step(umans_ctx* c) {
+int enqueue_step(umans_ctx* c, bool events) {
 	const int n = c->n;
 	hipStream_t st = c->stream;
-	const int cells = c->scan_cells;  // counters cleared and scanned: cells + 1
-	const float cs = c->desc.cell_size > 0.0f ? c->desc.cell_size : std::max(c->max_range, 0.5f);
+	const int nw = (int)c->worlds.size();
+	const int scan_n = c->total_cells + 1;
 
 	hipEvent_t e0 = nullptr, e1 = nullptr, e2 = nullptr;
-	if (c->profiling) {
+	if (events) {
 		if (c->ev_used + 3 > 3 * 256) {      // bound the pending events
 			const int rc = c->drain_profile();
 			if (rc) return rc;
@@ -268,16 +359,16 @@ int enqueue_step(umans_ctx* c) {
 		e0 = c->next_event(); e1 = c->next_event(); e2 = c->next_event();
 		hipEventRecord(e0, st);
 	}
-	hipLaunchKernelGGL(k_fill, dim3(ceil_div(cells + 1, 256)), dim3(256), 0, st, c->cell_count, cells + 1, 0u);
+	hipLaunchKernelGGL(k_fill, dim3(ceil_div(scan_n, 256)), dim3(256), 0, st, c->cell_count, scan_n, 0u);
 	hipLaunchKernelGGL(k_fill, dim3(1), dim3(64), 0, st, (unsigned*)c->dstats, 4, 0u);
-	if (c->desc.world_type != 1) {
-		hipLaunchKernelGGL(k_bbox_init, dim3(1), dim3(64), 0, st, c->dbbox);
-		hipLaunchKernelGGL(k_bbox, dim3(std::min(ceil_div(n, 256), 1024)), dim3(256), 0, st, c->X.s.pos, n, c->dbbox);
-		hipLaunchKernelGGL(k_grid_setup, dim3(1), dim3(64), 0, st, c->dbbox, cs, c->cell_cap, c->dgrid);
+	if (c->any_infinite) {
+		hipLaunchKernelGGL(k_bbox_init, dim3(ceil_div(nw, 256)), dim3(256), 0, st, c->dbbox, nw);
+		hipLaunchKernelGGL(k_bbox, dim3(ceil_div(n, 256)), dim3(256), 0, st, c->X.s.pos, c->X.s.meta, c->dworlds, n,
+		                   c->dbbox);
 	}
-	hipLaunchKernelGGL(k_count, dim3(ceil_div(n, 256)), dim3(256), 0, st, c->X.s.pos, n, c->dgrid,
-	                   c->cell_count, c->cell_of, c->rank);
-	const int scan_n = cells + 1;
+	hipLaunchKernelGGL(k_grid_setup, dim3(ceil_div(nw, 64)), dim3(64), 0, st, c->dworlds, c->dbbox, c->dgrids, nw);
+	hipLaunchKernelGGL(k_count, dim3(ceil_div(n, 256)), dim3(256), 0, st, c->X.s.pos, c->X.s.meta, n, c->dworlds,
+	                   c->dgrids, c->cell_count, c->cell_of, c->rank);
 	const int nb = ceil_div(scan_n, kScanTile);
 	hipLaunchKernelGGL(k_scan_local, dim3(nb), dim3(kScanThreads), 0, st, c->cell_count, c->cell_start, c->scan_sums, scan_n);
 	hipLaunchKernelGGL(k_scan_sums, dim3(1), dim3(kScanThreads), 0, st, c->scan_sums, nb);
@@ -289,46 +380,83 @@ int enqueue_step(umans_ctx* c) {
 	a.in = c->Y.s;
 	a.out = c->X.s;
 	a.pol = c->dpol;
-	a.grid = c->dgrid;
+	a.worlds = c->dworlds;
+	a.grids = c->dgrids;
 	a.cell_start = c->cell_start;
 	a.n = n;
-	a.world_type = c->desc.world_type;
-	a.W = c->desc.width;
-	a.H = c->desc.height;
-	a.dt = c->desc.delta_time;
 	a.kcap = c->kcap;
-	if (c->record) {
+	if (c->record && c->rec_count) {
 		a.rec_count = c->rec_count; a.rec_id = c->rec_id; a.rec_d2 = c->rec_d2; a.rec_seg = c->rec_seg;
 		a.rec_stride = c->rec_stride;
 	}
 	a.stats = c->dstats;
 	if (e1) hipEventRecord(e1, st);
-	launch_step(kmax_bucket(c->kcap), a, st);
+	const int kmax = kmax_bucket(c->kcap);
+	launch_step(kmax, c->block > 0 ? c->block : default_block(kmax), a, st);
 	if (e2) hipEventRecord(e2, st);
 	return c->hip(hipGetLastError(), "step launch");
 }
 
-} // namespace
-
-extern "C" {
-
-int umans_create(const umans_world_desc* desc, umans_ctx** out) {
-	if (!desc || !out) return UMANS_EINVAL;
-	*out = nullptr;
-	if (desc->world_type != 0 && desc->world_type != 1) return UMANS_EINVAL;
-	if (!(desc->delta_time > 0.0f)) return UMANS_EINVAL;   // crowdSimulator.cpp:575-584
-	if (desc->world_type == 1 && !(desc->width > 0.0f && desc->height > 0.0f)) return UMANS_EINVAL;
-	umans_ctx* c = new umans_ctx();
-	c->desc = *desc;
-	c->device = desc->device;
-	c->kcap = desc->max_neighbors > 0 ? desc->max_neighbors : kUncappedMax;
-	if (c->kcap > kUncappedMax) {
-		delete c;
-		return UMANS_EINVAL;
+int capture_graph(umans_ctx* c) {
+	c->drop_graph();
+	int rc = c->hip(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal), "graph capture");
+	if (rc) return rc;
+	const int erc = enqueue_step(c, false);
+	hipGraph_t g = nullptr;
+	const hipError_t e = hipStreamEndCapture(c->stream, &g);
+	if (erc) {
+		if (g) hipGraphDestroy(g);
+		return erc;
 	}
+	if ((rc = c->hip(e, "graph capture"))) return rc;
+	c->graph = g;
+	return c->hip(hipGraphInstantiate(&c->graph_exec, g, nullptr, nullptr, 0), "graph instantiate");
+}
+
+int run_steps(umans_ctx* c, int nsteps) {
+	for (int s = 0; s < nsteps; ++s) {
+		if (c->n > 0) {
+			int rc;
+			if (c->use_graph && !c->profiling) {
+				if (!c->graph_exec && (rc = capture_graph(c))) return rc;
+				if ((rc = c->hip(hipGraphLaunch(c->graph_exec, c->stream), "graph launch"))) return rc;
+			} else if ((rc = enqueue_step(c, c->profiling != 0))) {
+				return rc;
+			}
+		}
+		c->steps += 1;
+		c->sim_time += (double)c->worlds[0].delta_time;   // WorldBase::time_ (worldBase.cpp:122)
+	}
+	return UMANS_OK;
+}
+
+int agents_per_world(umans_ctx* c, std::vector<int>& per_world) {
+	per_world.assign(c->worlds.size(), 0);
+	std::vector<int4> meta(c->n);
+	int rc = c->hip(hipMemcpyAsync(meta.data(), c->X.s.meta, c->n * sizeof(int4), hipMemcpyDeviceToHost, c->stream), "meta");
+	if (!rc) rc = c->hip(hipStreamSynchronize(c->stream), "meta");
+	if (rc) return rc;
+	for (const int4& m : meta) per_world[m.y >> kPolicyBits]++;
+	return UMANS_OK;
+}
+
+int create_impl(const umans_world_desc* descs, int nworlds, umans_ctx** out) {
+	if (!descs || !out || nworlds < 1 || nworlds > kMaxWorlds) return UMANS_EINVAL;
+	*out = nullptr;
+	for (int w = 0; w < nworlds; ++w)
+		if (check_world(descs[w])) return UMANS_EINVAL;
+	const int kcap = descs[0].max_neighbors > 0 ? descs[0].max_neighbors : kUncappedMax;
+	if (kcap > kUncappedMax) return UMANS_EINVAL;
+	if (descs[0].block_size > 0 && !block_supported(kmax_bucket(kcap), descs[0].block_size)) return UMANS_EINVAL;
+	umans_ctx* c = new umans_ctx();
+	c->worlds.assign(descs, descs + nworlds);
+	c->device = descs[0].device;
+	c->kcap = kcap;
+	c->block = std::max(descs[0].block_size, 0);
 	if (hipSetDevice(c->device) != hipSuccess || hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess ||
-	    dalloc(&c->dgrid, 1) != hipSuccess || dalloc(&c->dbbox, 4) != hipSuccess ||
-	    dalloc(&c->dstats, 2) != hipSuccess || dalloc(&c->dpol, kMaxPolicies) != hipSuccess) {
+	    dalloc(&c->dgrids, nworlds) != hipSuccess || dalloc(&c->dworlds, nworlds) != hipSuccess ||
+	    dalloc(&c->dbbox, 4 * nworlds) != hipSuccess || dalloc(&c->dstats, 2) != hipSuccess ||
+	    dalloc(&c->dpol, kMaxPolicies) != hipSuccess) {
 		umans_destroy(c);
 		return UMANS_EHIP;
 	}
@@ -336,12 +464,25 @@ int umans_create(const umans_world_desc* desc, umans_ctx** out) {
 	return UMANS_OK;
 }
 
+} // namespace
+
+extern "C" {
+
+int umans_create(const umans_world_desc* desc, umans_ctx** out) { return create_impl(desc, 1, out); }
+
+int umans_create_batch(const umans_world_desc* worlds, int n_worlds, umans_ctx** out) {
+	return create_impl(worlds, n_worlds, out);
+}
+
+int umans_num_worlds(const umans_ctx* c) { return c ? (int)c->worlds.size() : 0; }
+
 void umans_destroy(umans_ctx* c) {
 	if (!c) return;
 	hipSetDevice(c->device);
 	if (c->stream) hipStreamSynchronize(c->stream);
 	c->free_agents();
-	hipFree(c->dgrid); hipFree(c->dbbox); hipFree(c->dstats); hipFree(c->dpol);
+	hipFree(c->dgrids); hipFree(c->dworlds); hipFree(c->dbbox); hipFree(c->dstats); hipFree(c->dpol);
+	hipFree(c->cell_count); hipFree(c->cell_start); hipFree(c->scan_sums);
 	c->free_events();
 	if (c->stream) hipStreamDestroy(c->stream);
 	delete c;
@@ -360,12 +501,13 @@ int umans_set_policies(umans_ctx* c, const umans_policy* p, int n) {
 		if (rc) return c->fail(rc, "policy " + std::to_string(i) + ": " + e);
 		mr = std::max(mr, v[i].query_range);
 	}
+	if (c->n > 0 && mr > c->max_range) return c->fail(UMANS_ESTATE, "a larger interaction range needs a new upload");
 	hipSetDevice(c->device);
 	const int rc = c->hip(hipMemcpyAsync(c->dpol, v.data(), n * sizeof(DevPolicy), hipMemcpyHostToDevice, c->stream),
 	                      "policy upload");
 	if (rc) return rc;
 	c->hpol.swap(v);
-	c->max_range = mr;
+	if (c->n == 0) c->max_range = mr;
 	return c->hip(hipStreamSynchronize(c->stream), "policy upload");
 }
 
@@ -376,19 +518,26 @@ int umans_upload(umans_ctx* c, const umans_agents* a, int n) {
 	if (n > 0 && (!a->id || !a->pos || !a->goal || !a->radius || !a->pref_speed || !a->max_speed || !a->max_accel || !a->policy))
 		return c->fail(UMANS_EINVAL, "missing agent array");
 	const int npol = (int)c->hpol.size();
+	const int nw = (int)c->worlds.size();
 	std::vector<float2> pos(n), vel(n), goal(n), view(n);
 	std::vector<float4> prm(n);
 	std::vector<int4> meta(n);
+	std::vector<int> per_world(nw, 0);
+	c->max_range = 0.0f;
+	for (const DevPolicy& p : c->hpol) c->max_range = std::max(c->max_range, p.query_range);
 	for (int i = 0; i < n; ++i) {
 		if (a->policy[i] < 0 || a->policy[i] >= npol)
 			return c->fail(UMANS_EINVAL, "agent " + std::to_string(i) + " has an unknown policy id");
+		const int w = a->world ? a->world[i] : 0;
+		if (w < 0 || w >= nw) return c->fail(UMANS_EINVAL, "agent " + std::to_string(i) + " has an unknown world");
+		per_world[w]++;
 		pos[i] = make_float2(a->pos[2 * i], a->pos[2 * i + 1]);
 		vel[i] = a->vel ? make_float2(a->vel[2 * i], a->vel[2 * i + 1]) : make_float2(0.0f, 0.0f);
 		goal[i] = make_float2(a->goal[2 * i], a->goal[2 * i + 1]);
 		if (a->viewdir) view[i] = make_float2(a->viewdir[2 * i], a->viewdir[2 * i + 1]);
 		else {
 			// Agent::setGoal looks at the goal (agent.cpp:120-127)
-			float dx = goal[i].x - pos[i].x, dy = goal[i].y - pos[i].y;
+			const float dx = goal[i].x - pos[i].x, dy = goal[i].y - pos[i].y;
 			const float m = std::sqrt(dx * dx + dy * dy);
 			view[i] = m > 0.0f ? make_float2(dx / m, dy / m) : make_float2(0.0f, 0.0f);
 		}
@@ -400,37 +549,24 @@ int umans_upload(umans_ctx* c, const umans_agents* a, int n) {
 			r = (uint32_t)((uint32_t)a->id[i] % 2147483647u);
 			if (r == 0) r = 1;
 		}
-		meta[i] = make_int4(a->id[i], a->policy[i], i, (int)r);
+		meta[i] = make_int4(a->id[i], a->policy[i] | (w << kPolicyBits), i, (int)r);
 	}
 	hipSetDevice(c->device);
+	c->drop_graph();
 	if (n > c->cap) {
 		c->free_agents();
-		const int cap = n;
 		int rc;
-		if ((rc = c->hip(c->X.alloc(cap), "alloc state"))) return rc;
-		if ((rc = c->hip(c->Y.alloc(cap), "alloc state"))) return rc;
-		const int cells = std::max(4 * cap, 4096);
-		if ((rc = c->hip(dalloc(&c->cell_count, cells + 1), "alloc grid"))) return rc;
-		if ((rc = c->hip(dalloc(&c->cell_start, cells + 1), "alloc grid"))) return rc;
-		if ((rc = c->hip(dalloc(&c->scan_sums, ceil_div(cells + 1, kScanTile) + 1), "alloc grid"))) return rc;
-		if ((rc = c->hip(dalloc(&c->cell_of, cap), "alloc grid"))) return rc;
-		if ((rc = c->hip(dalloc(&c->rank, cap), "alloc grid"))) return rc;
-		c->cap = cap;
-		c->cell_cap = cells;
+		if ((rc = c->hip(c->X.alloc(n), "alloc state"))) return rc;
+		if ((rc = c->hip(c->Y.alloc(n), "alloc state"))) return rc;
+		if ((rc = c->hip(dalloc(&c->cell_of, n), "alloc grid"))) return rc;
+		if ((rc = c->hip(dalloc(&c->rank, n), "alloc grid"))) return rc;
+		c->cap = n;
 	}
 	c->n = n;
 	c->have_record = false;
-	if (c->desc.world_type == 1) {
-		const float cs = c->desc.cell_size > 0.0f ? c->desc.cell_size : std::max(c->max_range, 0.5f);
-		const GridInfo g = toric_grid(c->desc.width, c->desc.height, cs, c->cell_cap);
-		int rc = c->hip(hipMemcpyAsync(c->dgrid, &g, sizeof(g), hipMemcpyHostToDevice, c->stream), "grid upload");
-		if (rc) return rc;
-		c->scan_cells = g.ncells;
-	} else {
-		c->scan_cells = c->cell_cap;
-	}
-	const State& s = c->X.s;
 	int rc;
+	if ((rc = setup_worlds(c, per_world))) return rc;
+	const State& s = c->X.s;
 	if ((rc = c->hip(hipMemcpyAsync(s.pos, pos.data(), n * sizeof(float2), hipMemcpyHostToDevice, c->stream), "upload"))) return rc;
 	if ((rc = c->hip(hipMemcpyAsync(s.vel, vel.data(), n * sizeof(float2), hipMemcpyHostToDevice, c->stream), "upload"))) return rc;
 	if ((rc = c->hip(hipMemcpyAsync(s.goal, goal.data(), n * sizeof(float2), hipMemcpyHostToDevice, c->stream), "upload"))) return rc;
@@ -444,7 +580,8 @@ int umans_num_agents(const umans_ctx* c) { return c ? c->n : 0; }
 
 int umans_set_record_neighbors(umans_ctx* c, int enable) {
 	if (!c) return UMANS_EINVAL;
-	if (enable && c->cap > 0 && !c->rec_count) {
+	if (enable && !c->rec_count) {
+		if (c->cap <= 0) return c->fail(UMANS_ESTATE, "upload agents before enabling the neighbour record");
 		hipSetDevice(c->device);
 		const int stride = kmax_bucket(c->kcap);
 		int rc;
@@ -454,9 +591,18 @@ int umans_set_record_neighbors(umans_ctx* c, int enable) {
 		if ((rc = c->hip(dalloc(&c->rec_seg, (size_t)c->cap * stride), "alloc record"))) return rc;
 		c->rec_stride = stride;
 	}
-	if (enable && !c->rec_count) return c->fail(UMANS_ESTATE, "upload agents before enabling the neighbour record");
 	c->record = enable ? 1 : 0;
+	c->drop_graph();
 	return UMANS_OK;
+}
+
+int umans_set_graph(umans_ctx* c, int enable) {
+	if (!c) return UMANS_EINVAL;
+	hipSetDevice(c->device);
+	const int rc = c->hip(hipStreamSynchronize(c->stream), "graph");
+	c->use_graph = enable ? 1 : 0;
+	c->drop_graph();
+	return rc;
 }
 
 int umans_step(umans_ctx* c, int nsteps) {
@@ -464,16 +610,91 @@ int umans_step(umans_ctx* c, int nsteps) {
 	if (nsteps < 0) return c->fail(UMANS_EINVAL, "negative step count");
 	if (c->hpol.empty()) return c->fail(UMANS_ESTATE, "no policies");
 	hipSetDevice(c->device);
-	for (int s = 0; s < nsteps; ++s) {
-		if (c->n > 0) {
-			const int rc = enqueue_step(c);
-			if (rc) return rc;
-		}
-		c->steps += 1;
-		c->sim_time += (double)c->desc.delta_time;   // WorldBase::time_ (worldBase.cpp:122)
-	}
-	if (c->record && nsteps > 0) c->have_record = true;
+	const int rc = run_steps(c, nsteps);
+	if (rc) return rc;
+	if (c->record && nsteps > 0 && c->rec_count) c->have_record = true;
 	return UMANS_OK;
+}
+
+int umans_autotune(umans_ctx* c, int trial_steps, umans_tuning* out) {
+	if (!c) return UMANS_EINVAL;
+	if (c->n == 0) return c->fail(UMANS_ESTATE, "upload agents before autotuning");
+	if (trial_steps < 1) trial_steps = 3;
+	hipSetDevice(c->device);
+	int rc;
+	std::vector<int> per_world;
+	if ((rc = agents_per_world(c, per_world))) return rc;
+	if (!c->B.s.pos && (rc = c->hip(c->B.alloc(c->cap), "alloc backup"))) return rc;
+	if ((rc = c->hip(copy_state(c->B.s, c->X.s, c->n, c->stream), "backup"))) return rc;
+	const int kmax = kmax_bucket(c->kcap);
+	const float factors[] = {0.5f, 0.75f, 1.0f, 1.5f};
+	const int blocks[] = {64, 128, 256};
+	const float f0 = c->cell_factor;
+	const int b0 = c->block;
+	const int rec0 = c->record;
+	c->record = 0;
+	float best_ms = 3.4e38f, best_f = f0;
+	int best_b = b0 > 0 ? b0 : default_block(kmax);
+	hipEvent_t t0 = nullptr, t1 = nullptr;
+	hipEventCreate(&t0);
+	hipEventCreate(&t1);
+	for (float f : factors) {
+		for (int b : blocks) {
+			if (!block_supported(kmax, b)) continue;
+			c->cell_factor = f;
+			c->block = b;
+			c->drop_graph();
+			if ((rc = setup_worlds(c, per_world))) break;
+			if ((rc = c->hip(copy_state(c->X.s, c->B.s, c->n, c->stream), "restore"))) break;
+			if ((rc = enqueue_step(c, false))) break;   // warm-up
+			hipEventRecord(t0, c->stream);
+			for (int s = 0; s < trial_steps && !rc; ++s) rc = enqueue_step(c, false);
+			hipEventRecord(t1, c->stream);
+			if (rc) break;
+			if ((rc = c->hip(hipEventSynchronize(t1), "autotune"))) break;
+			float ms = 0.0f;
+			hipEventElapsedTime(&ms, t0, t1);
+			if (ms < best_ms) { best_ms = ms; best_f = f; best_b = b; }
+		}
+		if (rc) break;
+	}
+	hipEventDestroy(t0);
+	hipEventDestroy(t1);
+	// restore the configuration chosen (or the previous one on failure) and the state
+	c->cell_factor = rc ? f0 : best_f;
+	c->block = rc ? b0 : best_b;
+	c->record = rec0;
+	c->drop_graph();
+	int rc2 = setup_worlds(c, per_world);
+	if (!rc2) rc2 = c->hip(copy_state(c->X.s, c->B.s, c->n, c->stream), "restore");
+	if (!rc2) rc2 = c->hip(hipStreamSynchronize(c->stream), "autotune");
+	if (rc) return rc;
+	if (rc2) return rc2;
+	if (out) {
+		out->cell_factor = c->cell_factor;
+		out->block_size = c->block;
+		out->ms_per_step = best_ms / trial_steps;
+	}
+	return UMANS_OK;
+}
+
+int umans_set_tuning(umans_ctx* c, const umans_tuning* t) {
+	if (!c || !t) return UMANS_EINVAL;
+	if (!(t->cell_factor > 0.0f) || (t->block_size != 0 && !block_supported(kmax_bucket(c->kcap), t->block_size)))
+		return c->fail(UMANS_EINVAL, "unsupported tuning");
+	hipSetDevice(c->device);
+	int rc = c->hip(hipStreamSynchronize(c->stream), "tuning");
+	if (rc) return rc;
+	c->cell_factor = t->cell_factor;
+	c->block = t->block_size;
+	c->drop_graph();
+	if (c->n > 0) {
+		std::vector<int> per_world;
+		if ((rc = agents_per_world(c, per_world))) return rc;
+		if ((rc = setup_worlds(c, per_world))) return rc;
+		rc = c->hip(hipStreamSynchronize(c->stream), "tuning");
+	}
+	return rc;
 }
 
 int umans_set_profiling(umans_ctx* c, int enable) {
@@ -540,7 +761,8 @@ int umans_download(umans_ctx* c, umans_agents* a) {
 		const int i = meta[k].z;   // upload row
 		if (i < 0 || i >= n) return c->fail(UMANS_EHIP, "corrupt agent row");
 		if (a->id) a->id[i] = meta[k].x;
-		if (a->policy) a->policy[i] = meta[k].y;
+		if (a->policy) a->policy[i] = meta[k].y & kPolicyMask;
+		if (a->world) a->world[i] = meta[k].y >> kPolicyBits;
 		if (a->rng) a->rng[i] = (uint32_t)meta[k].w;
 		if (a->pos) { a->pos[2 * i] = pos[k].x; a->pos[2 * i + 1] = pos[k].y; }
 		if (a->vel) { a->vel[2 * i] = vel[k].x; a->vel[2 * i + 1] = vel[k].y; }

@@ -38,52 +38,83 @@ UD int wave_max_i(int v) {
 	return v;
 }
 
-// bbox[0..3] = ordered keys of min x, min y, max x, max y (initialised by k_bbox_init)
-__global__ void k_bbox_init(int* bbox) {
-	if (threadIdx.x == 0) {
-		bbox[0] = 0x7fffffff; bbox[1] = 0x7fffffff;
-		bbox[2] = (int)0x80000000; bbox[3] = (int)0x80000000;
+// Per world w, bbox[4w..4w+3] = ordered keys of min x, min y, max x, max y.
+__global__ void k_bbox_init(int* bbox, int nworlds) {
+	const int i = blockIdx.x * blockDim.x + threadIdx.x;
+	if (i < nworlds) {
+		bbox[4 * i + 0] = 0x7fffffff; bbox[4 * i + 1] = 0x7fffffff;
+		bbox[4 * i + 2] = (int)0x80000000; bbox[4 * i + 3] = (int)0x80000000;
 	}
 }
-__global__ __launch_bounds__(256) void k_bbox(const float2* pos, int n, int* bbox) {
+// Bounding boxes of the infinite worlds. Agents of a batch are grouped by world, so a
+// wave whose lanes share one world reduces first and issues one atomic per bound.
+__global__ __launch_bounds__(256) void k_bbox(const float2* pos, const int4* meta, const WorldInfo* worlds,
+                                             int n, int* bbox) {
+	const int i = blockIdx.x * blockDim.x + threadIdx.x;
+	int w = -1;
 	int mnx = 0x7fffffff, mny = 0x7fffffff, mxx = (int)0x80000000, mxy = (int)0x80000000;
-	for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+	if (i < n) {
+		w = meta[i].y >> kPolicyBits;
 		const float2 p = pos[i];
-		if (isfinite(p.x) && isfinite(p.y)) {
-			const int kx = ordered_key(p.x), ky = ordered_key(p.y);
-			mnx = min(mnx, kx); mny = min(mny, ky); mxx = max(mxx, kx); mxy = max(mxy, ky);
+		if (worlds[w].world_type != 1 && isfinite(p.x) && isfinite(p.y)) {
+			mnx = mxx = ordered_key(p.x);
+			mny = mxy = ordered_key(p.y);
+		} else {
+			w = -1;
 		}
 	}
-	mnx = wave_min_i(mnx); mny = wave_min_i(mny); mxx = wave_max_i(mxx); mxy = wave_max_i(mxy);
-	if ((threadIdx.x & 63) == 0) {
-		atomicMin(&bbox[0], mnx); atomicMin(&bbox[1], mny);
-		atomicMax(&bbox[2], mxx); atomicMax(&bbox[3], mxy);
+	const int w0 = __shfl(w, 0);
+	const bool uniform = __all(w == w0 || w == -1);
+	if (uniform) {
+		mnx = wave_min_i(mnx); mny = wave_min_i(mny); mxx = wave_max_i(mxx); mxy = wave_max_i(mxy);
+		const int wl = wave_max_i(w);
+		if ((threadIdx.x & 63) == 0 && wl >= 0) {
+			atomicMin(&bbox[4 * wl + 0], mnx); atomicMin(&bbox[4 * wl + 1], mny);
+			atomicMax(&bbox[4 * wl + 2], mxx); atomicMax(&bbox[4 * wl + 3], mxy);
+		}
+	} else if (w >= 0) {
+		atomicMin(&bbox[4 * w + 0], mnx); atomicMin(&bbox[4 * w + 1], mny);
+		atomicMax(&bbox[4 * w + 2], mxx); atomicMax(&bbox[4 * w + 3], mxy);
 	}
 }
-// Infinite world: a grid over the bounding box with cells of at least `cell`
-// metres and at most `cap` cells.
-__global__ void k_grid_setup(const int* bbox, float cell, int cap, GridInfo* g) {
-	if (threadIdx.x != 0) return;
+// One thread per world. Infinite worlds: a grid over the bounding box with cells of
+// at least cell_size metres and at most cell_cap cells. Toric worlds: the world
+// rectangle centred on the origin with cells at least cell_size wide.
+__global__ void k_grid_setup(const WorldInfo* worlds, const int* bbox, GridInfo* grids, int nworlds) {
+	const int wi = blockIdx.x * blockDim.x + threadIdx.x;
+	if (wi >= nworlds) return;
+	const WorldInfo W = worlds[wi];
 	GridInfo r;
-	const bool empty = bbox[0] > bbox[2];
-	const float x0 = empty ? 0.0f : from_ordered_key(bbox[0]);
-	const float y0 = empty ? 0.0f : from_ordered_key(bbox[1]);
-	const float x1 = empty ? 0.0f : from_ordered_key(bbox[2]);
-	const float y1 = empty ? 0.0f : from_ordered_key(bbox[3]);
-	double cs = cell > 0.0f ? (double)cell : 1.0;
-	double w = (double)x1 - (double)x0, h = (double)y1 - (double)y0;
-	double nx = floor(w / cs) + 1.0, ny = floor(h / cs) + 1.0;
-	while (nx * ny > (double)cap) {
-		cs *= 1.25;
-		nx = floor(w / cs) + 1.0;
-		ny = floor(h / cs) + 1.0;
-	}
-	r.ox = x0; r.oy = y0;
-	r.invx = (float)(1.0 / cs); r.invy = r.invx;
-	r.ncx = (int)nx; r.ncy = (int)ny;
-	r.ncells = r.ncx * r.ncy;
 	r.pad = 0;
-	*g = r;
+	double cs = W.cell_size > 0.0f ? (double)W.cell_size : 1.0;
+	if (W.world_type == 1) {
+		double nx = fmax(1.0, floor((double)W.W / cs)), ny = fmax(1.0, floor((double)W.H / cs));
+		while (nx * ny > (double)W.cell_cap) {
+			if (nx >= ny) nx = fmax(1.0, floor(nx / 2.0)); else ny = fmax(1.0, floor(ny / 2.0));
+		}
+		r.ox = -0.5f * W.W; r.oy = -0.5f * W.H;
+		r.ncx = (int)nx; r.ncy = (int)ny;
+		r.invx = (float)(nx / (double)W.W); r.invy = (float)(ny / (double)W.H);
+	} else {
+		const int* b = bbox + 4 * wi;
+		const bool empty = b[0] > b[2];
+		const float x0 = empty ? 0.0f : from_ordered_key(b[0]);
+		const float y0 = empty ? 0.0f : from_ordered_key(b[1]);
+		const float x1 = empty ? 0.0f : from_ordered_key(b[2]);
+		const float y1 = empty ? 0.0f : from_ordered_key(b[3]);
+		const double w = (double)x1 - (double)x0, h = (double)y1 - (double)y0;
+		double nx = floor(w / cs) + 1.0, ny = floor(h / cs) + 1.0;
+		while (nx * ny > (double)W.cell_cap) {
+			cs *= 1.25;
+			nx = floor(w / cs) + 1.0;
+			ny = floor(h / cs) + 1.0;
+		}
+		r.ox = x0; r.oy = y0;
+		r.invx = (float)(1.0 / cs); r.invy = r.invx;
+		r.ncx = (int)nx; r.ncy = (int)ny;
+	}
+	r.ncells = r.ncx * r.ncy;
+	grids[wi] = r;
 }
 
 // Cell index along one axis: monotone non-decreasing in x, clamped into the grid
@@ -94,13 +125,15 @@ UD int bin_axis(float x, float o, float inv, int nc) {
 	return (int)t;
 }
 
-__global__ __launch_bounds__(256) void k_count(const float2* pos, int n, const GridInfo* grid,
-                                              unsigned* cell_count, unsigned* cell_of, unsigned* rank) {
+__global__ __launch_bounds__(256) void k_count(const float2* pos, const int4* meta, int n, const WorldInfo* worlds,
+                                              const GridInfo* grids, unsigned* cell_count, unsigned* cell_of,
+                                              unsigned* rank) {
 	const int k = blockIdx.x * blockDim.x + threadIdx.x;
 	if (k >= n) return;
-	const GridInfo g = *grid;
+	const int w = meta[k].y >> kPolicyBits;
+	const GridInfo g = grids[w];
 	const float2 p = pos[k];
-	const int c = bin_axis(p.y, g.oy, g.invy, g.ncy) * g.ncx + bin_axis(p.x, g.ox, g.invx, g.ncx);
+	const int c = worlds[w].cell_offset + bin_axis(p.y, g.oy, g.invy, g.ncy) * g.ncx + bin_axis(p.x, g.ox, g.invx, g.ncx);
 	cell_of[k] = (unsigned)c;
 	rank[k] = atomicAdd(&cell_count[c], 1u);
 }
@@ -198,7 +231,7 @@ __global__ __launch_bounds__(256) void k_reorder(State src, State dst, int n, co
 // order (d2 ascending, ties by agent ID); the list keeps the first kcap entries of the
 // concatenation. Returns the list length; `total` is the uncapped length.
 template <int KMAX, int BS>
-UD int find_neighbors(const StepArgs& A, const GridInfo& g, int self, V2 p, float range,
+UD int find_neighbors(const StepArgs& A, const WorldInfo& wi, const GridInfo& g, int self, V2 p, float range,
                       double* d2s, unsigned* nbs, int& total) {
 	const State& S = A.in;
 	const double r2 = (double)(range * range);
@@ -209,11 +242,11 @@ UD int find_neighbors(const StepArgs& A, const GridInfo& g, int self, V2 p, floa
 	for (int s = 0; s < 5; ++s) {
 		float dx = 0.0f, dy = 0.0f;
 		if (s > 0) {
-			if (A.world_type != 1) break;
-			if (s == 1) { if (!((double)(p.x - range) < -0.5 * (double)A.W)) continue; dx = A.W; }
-			else if (s == 2) { if (!((double)(p.x + range) > 0.5 * (double)A.W)) continue; dx = -A.W; }
-			else if (s == 3) { if (!((double)(p.y - range) < -0.5 * (double)A.H)) continue; dy = A.H; }
-			else { if (!((double)(p.y + range) > 0.5 * (double)A.H)) continue; dy = -A.H; }
+			if (wi.world_type != 1) break;
+			if (s == 1) { if (!((double)(p.x - range) < -0.5 * (double)wi.W)) continue; dx = wi.W; }
+			else if (s == 2) { if (!((double)(p.x + range) > 0.5 * (double)wi.W)) continue; dx = -wi.W; }
+			else if (s == 3) { if (!((double)(p.y - range) < -0.5 * (double)wi.H)) continue; dy = wi.H; }
+			else { if (!((double)(p.y + range) > 0.5 * (double)wi.H)) continue; dy = -wi.H; }
 		}
 		const V2 q{p.x + dx, p.y + dy};
 		// cell range: a margin covers the rounding of r2 and of the float bounds
@@ -225,7 +258,7 @@ UD int find_neighbors(const StepArgs& A, const GridInfo& g, int self, V2 p, floa
 		const int cap = kcap - filled;
 		int seg_cnt = 0;
 		for (int cy = cy0; cy <= cy1; ++cy) {
-			const int row = cy * g.ncx;
+			const int row = wi.cell_offset + cy * g.ncx;
 			const unsigned b = A.cell_start[row + cx0], e = A.cell_start[row + cx1 + 1];
 			for (unsigned j = b; j < e; ++j) {
 				if ((int)j == self) continue;
@@ -363,19 +396,24 @@ __global__ __launch_bounds__(BS) void k_step(StepArgs A) {
 	const bool active = k < A.n;
 	int total = 0, cnt = 0;
 	const State& S = A.in;
-	const GridInfo g = *A.grid;
 	int4 meta = make_int4(0, 0, 0, 0);
+	int pi = 0, wix = 0;
 	Self a;
+	WorldInfo wi{};
 	if (active) {
 		meta = S.meta[k];
+		pi = meta.y & kPolicyMask;
+		wix = meta.y >> kPolicyBits;
+		wi = A.worlds[wix];
+		const GridInfo g = A.grids[wix];
 		const float4 prm = S.prm[k];
 		a.pos = v2(S.pos[k]);
 		a.vel = v2(S.vel[k]);
 		a.pref = v2(S.pref[k]);
 		a.radius = prm.x; a.pref_speed = prm.y; a.max_speed = prm.z; a.max_accel = prm.w;
-		a.dt = A.dt;
-		a.relaxation = A.pol[meta.y].relaxation;
-		cnt = find_neighbors<KMAX, BS>(A, g, k, a.pos, A.pol[meta.y].query_range, s_d2 + tid, s_nb + tid, total);
+		a.dt = wi.dt;
+		a.relaxation = A.pol[pi].relaxation;
+		cnt = find_neighbors<KMAX, BS>(A, wi, g, k, a.pos, A.pol[pi].query_range, s_d2 + tid, s_nb + tid, total);
 	}
 	// stats: one atomic per wave
 	{
@@ -388,7 +426,7 @@ __global__ __launch_bounds__(BS) void k_step(StepArgs A) {
 		}
 	}
 	if (!active) return;
-	const DevPolicy& P = A.pol[meta.y];
+	const DevPolicy& P = A.pol[pi];
 	if (A.rec_count) {
 		A.rec_count[k] = cnt;
 		const int m = cnt < A.rec_stride ? cnt : A.rec_stride;
@@ -400,7 +438,7 @@ __global__ __launch_bounds__(BS) void k_step(StepArgs A) {
 			A.rec_seg[o] = (int)(u >> kSegShift);
 		}
 	}
-	const NbList L{s_nb + tid, BS, cnt, S.pos, S.vel, S.prm, S.pref, A.W, A.H, a.pos};
+	const NbList L{s_nb + tid, BS, cnt, S.pos, S.vel, S.prm, S.pref, wi.W, wi.H, a.pos};
 	uint32_t rng = (uint32_t)meta.w;
 
 	// Policy::ComputeNewVelocity (policy.cpp:43-76)
@@ -449,10 +487,10 @@ __global__ __launch_bounds__(BS) void k_step(StepArgs A) {
 	const V2 vd = (2.0f * vel + a.pref) / 3.0f;
 	if ((double)sqr_mag(vd) > 0.01) view = normalized(vd);
 	V2 pos = a.pos + (vel * a.dt);
-	if (A.world_type == 1) {
-		const float hw = 0.5f * A.W, hh = 0.5f * A.H;
-		if (pos.x > hw) pos.x -= A.W; else if (pos.x < -hw) pos.x += A.W;
-		if (pos.y > hh) pos.y -= A.H; else if (pos.y < -hh) pos.y += A.H;
+	if (wi.world_type == 1) {
+		const float hw = 0.5f * wi.W, hh = 0.5f * wi.H;
+		if (pos.x > hw) pos.x -= wi.W; else if (pos.x < -hw) pos.x += wi.W;
+		if (pos.y > hh) pos.y -= wi.H; else if (pos.y < -hh) pos.y += wi.H;
 	}
 	A.out.pos[k] = f2(pos);
 	A.out.vel[k] = f2(vel);

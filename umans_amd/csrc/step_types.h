@@ -20,6 +20,10 @@ enum {
 	MODE_GLOBAL_FORCE = 3    // best = target velocity of the force model (ForceBasedFunction.cpp:38-41)
 };
 
+// meta.y packs the policy (low bits) and the world of a batch (high bits)
+constexpr int kPolicyBits = 10;
+constexpr int kPolicyMask = (1 << kPolicyBits) - 1;
+
 constexpr unsigned kSegShift = 28;
 constexpr unsigned kIndexMask = (1u << kSegShift) - 1u;
 constexpr int kMaxAngleSamples = 64;
@@ -45,6 +49,18 @@ struct DevPolicy {
 	float ang_sin[kMaxAngleSamples];   // sequence start, start + dA, ... (host libm)
 };
 
+// One world of a batch; a single world is a batch of one (the reference has one world
+// per CrowdSimulator, crowdSimulator.h:37).
+struct WorldInfo {
+	int world_type;          // 0 infinite, 1 toric
+	float W, H;
+	float dt;
+	int cell_offset;         // first global cell of this world's grid
+	int cell_cap;            // cells reserved for this world's grid
+	float cell_size;         // requested cell width (m)
+	int pad;
+};
+
 struct GridInfo {
 	float ox, oy;            // origin of cell (0, 0)
 	float invx, invy;        // cells per metre
@@ -61,19 +77,17 @@ struct State {
 	float2* viewdir;
 	float2* pref;            // preferred velocity of the current step
 	float4* prm;             // radius, preferred speed, maximum speed, maximum acceleration
-	int4* meta;              // agent ID, policy, upload row, RNG state
+	int4* meta;              // agent ID, policy | world << kPolicyBits, upload row, RNG state
 };
 
 struct StepArgs {
 	State in;                // cell-sorted state of this step (read only)
 	State out;               // next state, same order
 	const DevPolicy* pol;
-	const GridInfo* grid;
-	const unsigned* cell_start;   // ncells + 1
+	const WorldInfo* worlds;
+	const GridInfo* grids;
+	const unsigned* cell_start;   // total cells + 1
 	int n;
-	int world_type;
-	float W, H;
-	float dt;
 	int kcap;                // neighbour cap (<= KMAX of the kernel instance)
 	// optional neighbour record (null when not recording)
 	int* rec_count;

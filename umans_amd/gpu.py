@@ -23,7 +23,8 @@ EXPORTS = [
     "umans_download", "umans_num_agents", "umans_step", "umans_synchronize", "umans_get_stats",
     "umans_set_record_neighbors", "umans_dump_neighbors", "umans_cost_id", "umans_cost_name",
     "umans_cost_param_index", "umans_cost_param_name", "umans_cost_defaults", "umans_policy_init",
-    "umans_set_profiling", "umans_get_profile",
+    "umans_set_profiling", "umans_get_profile", "umans_create_batch", "umans_num_worlds",
+    "umans_set_graph", "umans_autotune", "umans_set_tuning",
 ]
 
 
@@ -46,7 +47,7 @@ class PolicyC(C.Structure):
 class AgentsC(C.Structure):
     _fields_ = [("id", _I), ("pos", _F), ("vel", _F), ("goal", _F), ("viewdir", _F), ("radius", _F),
                 ("pref_speed", _F), ("max_speed", _F), ("max_accel", _F), ("policy", _I), ("rng", _U),
-                ("pref_vel", _F)]
+                ("pref_vel", _F), ("world", _I)]
 
 
 class Stats(C.Structure):
@@ -56,6 +57,10 @@ class Stats(C.Structure):
 
 class Profile(C.Structure):
     _fields_ = [("steps", C.c_int64), ("grid_ms", C.c_double), ("step_ms", C.c_double)]
+
+
+class Tuning(C.Structure):
+    _fields_ = [("cell_factor", C.c_float), ("block_size", C.c_int32), ("ms_per_step", C.c_float)]
 
 
 class UmansError(RuntimeError):
@@ -95,6 +100,11 @@ def lib():
         L.umans_cost_param_name.restype = C.c_char_p
         L.umans_cost_defaults.argtypes = [C.c_int, _F, _F]
         L.umans_policy_init.argtypes = [C.POINTER(PolicyC), C.c_int]
+        L.umans_create_batch.argtypes = [C.POINTER(WorldDesc), C.c_int, C.POINTER(vp)]
+        L.umans_num_worlds.argtypes = [vp]
+        L.umans_set_graph.argtypes = [vp, C.c_int]
+        L.umans_autotune.argtypes = [vp, C.c_int, C.POINTER(Tuning)]
+        L.umans_set_tuning.argtypes = [vp, C.POINTER(Tuning)]
         L.umans_set_profiling.argtypes = [vp, C.c_int]
         L.umans_get_profile.argtypes = [vp, C.POINTER(Profile)]
         _lib = L
@@ -124,15 +134,22 @@ def policy_struct(packed):
 class World:
     """One simulated world on one GPU (a context of the C ABI)."""
 
-    def __init__(self, world_type, width, height, dt, policies, max_neighbors=0, cell_size=0.0, device=0):
+    def __init__(self, world_type, width, height, dt, policies, max_neighbors=0, cell_size=0.0, device=0,
+                 block_size=0, batch=None):
+        """batch: optional list of (world_type, width, height, dt) for a batch of worlds
+        (then world_type/width/height/dt are ignored and agents carry a `world` index)."""
         L = lib()
         self._h = C.c_void_p()
-        desc = WorldDesc(int(world_type), float(width), float(height), float(dt), int(max_neighbors),
-                         float(cell_size), 0, int(device))
-        rc = L.umans_create(C.byref(desc), C.byref(self._h))
+        if batch is None:
+            batch = [(world_type, width, height, dt)]
+        descs = (WorldDesc * len(batch))(*[WorldDesc(int(t), float(w), float(h), float(d), int(max_neighbors),
+                                                     float(cell_size), int(block_size), int(device))
+                                           for (t, w, h, d) in batch])
+        rc = L.umans_create_batch(descs, len(batch), C.byref(self._h))
         if rc != 0:
-            raise UmansError(f"umans_create failed ({rc})")
-        self.desc = desc
+            raise UmansError(f"umans_create_batch failed ({rc})")
+        self.desc = descs[0]
+        self.n_worlds = len(batch)
         self._check(L.umans_set_policies(self._h, (PolicyC * len(policies))(*[policy_struct(p) for p in policies]),
                                          len(policies)))
         self.n = 0
@@ -168,7 +185,8 @@ class World:
                     arr("goal", _F, np.float32), arr("viewdir", _F, np.float32, False),
                     arr("radius", _F, np.float32), arr("pref_speed", _F, np.float32),
                     arr("max_speed", _F, np.float32), arr("max_accel", _F, np.float32),
-                    arr("policy", _I, np.int32), arr("rng", _U, np.uint32, False), None)
+                    arr("policy", _I, np.int32), arr("rng", _U, np.uint32, False), None,
+                    arr("world", _I, np.int32, False))
         self._check(lib().umans_upload(self._h, C.byref(a), n))
         self.n = n
 
@@ -189,10 +207,11 @@ class World:
         out["id"] = np.zeros(n, np.int32)
         out["policy"] = np.zeros(n, np.int32)
         out["rng"] = np.zeros(n, np.uint32)
+        out["world"] = np.zeros(n, np.int32)
         a = AgentsC(_p(out["id"], _I), _p(out["pos"], _F), _p(out["vel"], _F), _p(out["goal"], _F),
                     _p(out["viewdir"], _F), _p(out["radius"], _F), _p(out["pref_speed"], _F),
                     _p(out["max_speed"], _F), _p(out["max_accel"], _F), _p(out["policy"], _I),
-                    _p(out["rng"], _U), _p(out["pref_vel"], _F))
+                    _p(out["rng"], _U), _p(out["pref_vel"], _F), _p(out["world"], _I))
         self._check(lib().umans_download(self._h, C.byref(a)))
         return out
 
@@ -204,6 +223,18 @@ class World:
         seg = np.zeros((n, stride), np.int32)
         self._check(lib().umans_dump_neighbors(self._h, stride, _p(cnt, _I), _p(ids, _I), _p(d2, _D), _p(seg, _I)))
         return cnt, ids, d2, seg
+
+    def set_graph(self, enable=True):
+        self._check(lib().umans_set_graph(self._h, 1 if enable else 0))
+
+    def autotune(self, trial_steps=3):
+        t = Tuning()
+        self._check(lib().umans_autotune(self._h, int(trial_steps), C.byref(t)))
+        return {"cell_factor": t.cell_factor, "block_size": t.block_size, "ms_per_step": t.ms_per_step}
+
+    def set_tuning(self, cell_factor, block_size=0):
+        t = Tuning(float(cell_factor), int(block_size), 0.0)
+        self._check(lib().umans_set_tuning(self._h, C.byref(t)))
 
     def set_profiling(self, enable=True):
         self._check(lib().umans_set_profiling(self._h, 1 if enable else 0))

@@ -163,3 +163,65 @@ def random_cloud(n, size, policies, world_type=1, max_neighbors=0, seed=0, dt=0.
     a["policy"][:] = (np.arange(n) % len(policies)).astype(np.int32)
     a["viewdir"] = initial_viewdir(a["pos"], a["goal"])
     return Scenario("cloud%d" % n, world_type, size, size, dt, max_neighbors, list(policies), a, 1)
+
+
+@dataclass
+class Batch:
+    """Independent worlds stepped together (one policy per world here)."""
+    name: str
+    worlds: list                 # (world_type, width, height, dt) per world
+    policies: list
+    agents: dict                 # as Scenario.agents plus "world"
+    max_neighbors: int
+    steps: int
+    sizes: np.ndarray
+
+    @property
+    def n(self):
+        return len(self.agents["id"])
+
+
+def sweep(n_worlds=512, n_min=20, n_max=5000, steps=500, seed=4, max_neighbors=0, dt=0.1):
+    """Batched scenario sweep: agent counts log-uniform in [n_min, n_max]; layout by
+    world index mod 3 (circle of circumference N metres with antipodal goals; two
+    opposing lanes 4 m wide at 0.5 agents/m^2; uniform square at 0.5 agents/m^2 with
+    random goals); per world a {PowerLaw + GoalReachingForce} sampling policy with
+    k ~ U[1,3], tau0 ~ U[1,5] s, 12 angles x 6 speeds; infinite worlds; dt 0.1 s.
+    Agent IDs restart at 0 in every world, as each world is its own simulator."""
+    r = np.random.default_rng(seed)
+    sizes = np.rint(np.exp(r.uniform(np.log(n_min), np.log(n_max), n_worlds))).astype(np.int64)
+    parts, pols, worlds = [], [], []
+    for w, nw in enumerate(sizes):
+        nw = int(nw)
+        a = _agents(nw)
+        lay = w % 3
+        if lay == 0:
+            R = nw * 1.0 / (2.0 * np.pi)
+            th = 2.0 * np.pi * np.arange(nw) / nw
+            a["pos"][:, 0] = (R * np.cos(th)).astype(np.float32)
+            a["pos"][:, 1] = (R * np.sin(th)).astype(np.float32)
+            a["goal"][:] = -a["pos"]
+        elif lay == 1:
+            L = nw / (0.5 * 4.0)
+            half = nw // 2
+            x = r.uniform(0.0, L, nw)
+            y = r.uniform(-2.0, 2.0, nw)
+            left = np.arange(nw) < half
+            a["pos"][:, 0] = np.where(left, -10.0 - x, 10.0 + x).astype(np.float32)
+            a["pos"][:, 1] = y.astype(np.float32)
+            a["goal"][:, 0] = np.where(left, a["pos"][:, 0] + L + 40.0, a["pos"][:, 0] - L - 40.0).astype(np.float32)
+            a["goal"][:, 1] = a["pos"][:, 1]
+        else:
+            side = np.sqrt(nw / 0.5)
+            a["pos"][:] = r.uniform(-side / 2, side / 2, (nw, 2)).astype(np.float32)
+            a["goal"][:] = r.uniform(-side / 2, side / 2, (nw, 2)).astype(np.float32)
+        a["viewdir"] = initial_viewdir(a["pos"], a["goal"])
+        a["policy"][:] = w
+        a["world"] = np.full(nw, w, np.int32)
+        k = float(r.uniform(1.0, 3.0))
+        tau0 = float(r.uniform(1.0, 5.0))
+        pols.append(powerlaw_sampling_policy(k=k, tau0=tau0, rng=5.0, angles=12, speeds=6))
+        worlds.append((0, 0.0, 0.0, dt))
+        parts.append(a)
+    agents = {k: np.concatenate([p[k] for p in parts]) for k in parts[0]}
+    return Batch("sweep%d" % n_worlds, worlds, pols, agents, max_neighbors, steps, sizes)
