@@ -53,7 +53,17 @@ def cpu_baseline(sc, steps, threads):
                                                                                     "range": 4.0, "agentMass": 80.0}),
                                                      ("GoalReachingForce", 1.0, {})], relaxation_time=0.5),
                 to_reference_xml_policy("global", [("ORCA", 1.0, {"timeHorizon": 3.0, "range": 4.0})])]
-        sim = R.RefSim(sc.world_type, sc.width, sc.height, sc.dt, pols, sc.agents, threads=threads)
+        # the reference reports its world on stdout; keep this program's stdout to the JSON line
+        sys.stdout.flush()
+        saved = os.dup(1)
+        devnull = os.open(os.devnull, os.O_WRONLY)
+        os.dup2(devnull, 1)
+        try:
+            sim = R.RefSim(sc.world_type, sc.width, sc.height, sc.dt, pols, sc.agents, threads=threads)
+        finally:
+            os.dup2(saved, 1)
+            os.close(devnull)
+            os.close(saved)
         sim.step(1)   # first step outside the timing, as the warm-up of the GPU leg
         t = sim.time_steps(steps)
         sim.close()
@@ -88,6 +98,20 @@ def copy_probe(mib):
     hip.hipFree(dst)
     print(json.dumps({"copy_probe_bytes": nbytes, "copies": 3}))
     return 0
+
+
+def measured_traffic(kernel="k_step"):
+    """HBM bytes per launch of `kernel` on the mixed crowd from the committed rocprofv3
+    FETCH_SIZE/WRITE_SIZE summary (scaled by the copy probe), or None."""
+    path = os.path.join(ROOT, "profiles", "k_step_mixed1M_hbm_counters.md")
+    try:
+        for line in open(path):
+            cells = [c.strip() for c in line.split("|")]
+            if len(cells) > 4 and cells[1] == kernel:
+                return float(cells[4]) * 1e6
+    except OSError:
+        pass
+    return None
 
 
 def main():
@@ -168,7 +192,8 @@ def main():
                     if sc.name.startswith("mixed") else sc.name, "parallelism": "single GPU"}),
         "tuning": tuning, "graph": bool(args.graph),
         "roofline": {"bound": "hbm", "kernel": "k_step", "achieved": achieved, "peak": RL.HBM_PEAK_GBS,
-                     "unit": "GB/s", "frac": achieved / RL.HBM_PEAK_GBS, "traffic": None,
+                     "unit": "GB/s", "frac": achieved / RL.HBM_PEAK_GBS,
+                     "traffic": measured_traffic() if sc.name.startswith("mixed") else None,
                      "bytes_per_agent": RL.STEP_BYTES, "avg_launch_ms": step_s * 1e3,
                      "valu": {"achieved": valu, "peak": RL.FP32_PEAK_TFLOPS, "unit": "TFLOP/s",
                               "frac": valu / RL.FP32_PEAK_TFLOPS, "flops_per_agent": flops_per_agent}},
