@@ -161,6 +161,28 @@ typedef struct {
 int umans_autotune(umans_ctx* ctx, int trial_steps, umans_tuning* out);
 int umans_set_tuning(umans_ctx* ctx, const umans_tuning* tuning);
 
+/* Differentiable rollout (no counterpart in the reference). Starting from the current
+ * state, umans_adjoint_forward runs nsteps steps, keeping a checkpoint of the state
+ * every checkpoint_every steps, and returns the loss
+ *   L = 1/(nsteps * n) * sum over steps t and agents i of |p_i(t) - target_i(t)|^2
+ * (the minimum-image difference in toric worlds); targets are nsteps x n x 2 floats in
+ * upload order, target t being the position after step t+1. umans_adjoint_backward
+ * then recomputes each segment from its checkpoint and applies the transposed Jacobian
+ * of every step in reverse (the neighbour lists, filters and clamp branches of the
+ * forward step held fixed), and returns dL/d(parameters) as 2 floats per policy —
+ * (V0, sigma) of SocialForcesAvoidance or (k, tau0) of PowerLaw, chosen by param_kind
+ * — and dL/d(initial position) and dL/d(initial velocity), n x 2 floats each in upload
+ * order (either may be null). The state is left at the end of the rollout. Every
+ * policy must use the gradient method with force-model costs (PowerLaw,
+ * SocialForcesAvoidance, GoalReachingForce). */
+typedef struct {
+	int32_t nsteps;
+	int32_t checkpoint_every;
+	int32_t param_kind;        /* UMANS_COST_SOCIAL_FORCES or UMANS_COST_POWER_LAW */
+} umans_adjoint_desc;
+int umans_adjoint_forward(umans_ctx* ctx, const umans_adjoint_desc* desc, const float* targets, double* loss);
+int umans_adjoint_backward(umans_ctx* ctx, float* d_params, float* d_pos0, float* d_vel0);
+
 /* Per-phase GPU timers (hipEvents on the context's stream), accumulated over the
  * steps run while enabled: grid = cell keys + scan + reorder (+ bounding box in
  * infinite worlds), step = the fused per-agent kernel. The reference has only a

@@ -1,7 +1,7 @@
 // libumans_gpu: context management and the C ABI of include/umans_gpu.h.
 // The kernels are in kernels.h; the registry in registry.cpp.
 #include "umans_gpu.h"
-#include "kernels.h"
+#include "adjoint.h"
 
 #include <algorithm>
 #include <cmath>
@@ -141,6 +141,20 @@ struct umans_ctx {
 	long long steps = 0;
 	double sim_time = 0.0;
 
+	// differentiable rollout (umans_adjoint_forward / _backward)
+	int adj_steps = 0, adj_ckpt = 0, adj_param_kind = -1, adj_n = 0;
+	bool adj_ready = false;
+	std::vector<DeviceState> ckpts;      // state before steps 0, ck, 2ck, ...
+	std::vector<DeviceState> seg_states; // sorted input state of each step of a segment
+	float2* targets = nullptr;           // [steps][n], upload-row order
+	unsigned* seg_nb = nullptr;          // [ck][cap][KMAX]
+	int* seg_cnt = nullptr;              // [ck][cap]
+	float2* seg_lg = nullptr;            // [ck][n] loss gradient of each step's output
+	float2* adj_buf = nullptr;           // 5 x n: p, v, p', v', pref
+	float* adj_params = nullptr;         // [policies][2]
+	double* dloss = nullptr;
+	double last_loss = 0.0;
+
 	// per-phase timers: one event triple per profiled step, read back lazily
 	int profiling = 0;
 	std::vector<hipEvent_t> ev;
@@ -163,8 +177,21 @@ struct umans_ctx {
 		graph_exec = nullptr;
 		graph = nullptr;
 	}
+	void free_adjoint() {
+		for (DeviceState& d : ckpts) d.release();
+		for (DeviceState& d : seg_states) d.release();
+		ckpts.clear();
+		seg_states.clear();
+		hipFree(targets); hipFree(seg_nb); hipFree(seg_cnt); hipFree(seg_lg); hipFree(adj_buf);
+		hipFree(adj_params); hipFree(dloss);
+		targets = nullptr; seg_nb = nullptr; seg_cnt = nullptr; seg_lg = nullptr; adj_buf = nullptr;
+		adj_params = nullptr; dloss = nullptr;
+		adj_ready = false;
+		adj_steps = adj_ckpt = adj_n = 0;
+	}
 	void free_agents() {
 		drop_graph();
+		free_adjoint();
 		X.release(); Y.release(); B.release();
 		hipFree(cell_of); hipFree(rank);
 		hipFree(rec_count); hipFree(rec_id); hipFree(rec_d2); hipFree(rec_seg);
@@ -344,8 +371,10 @@ int setup_worlds(umans_ctx* c, const std::vector<int>& per_world) {
 	              "world upload");
 }
 
-int enqueue_step(umans_ctx* c, bool events) {
+int enqueue_step(umans_ctx* c, bool events, const State* ysave = nullptr, unsigned* raw_nb = nullptr,
+                 int* raw_count = nullptr) {
 	const int n = c->n;
+	const State& Ys = ysave ? *ysave : c->Y.s;
 	hipStream_t st = c->stream;
 	const int nw = (int)c->worlds.size();
 	const int scan_n = c->total_cells + 1;
@@ -373,11 +402,11 @@ int enqueue_step(umans_ctx* c, bool events) {
 	hipLaunchKernelGGL(k_scan_local, dim3(nb), dim3(kScanThreads), 0, st, c->cell_count, c->cell_start, c->scan_sums, scan_n);
 	hipLaunchKernelGGL(k_scan_sums, dim3(1), dim3(kScanThreads), 0, st, c->scan_sums, nb);
 	hipLaunchKernelGGL(k_scan_add, dim3(nb), dim3(kScanThreads), 0, st, c->cell_start, c->scan_sums, scan_n);
-	hipLaunchKernelGGL(k_reorder, dim3(ceil_div(n, 256)), dim3(256), 0, st, c->X.s, c->Y.s, n, c->cell_of, c->rank,
+	hipLaunchKernelGGL(k_reorder, dim3(ceil_div(n, 256)), dim3(256), 0, st, c->X.s, Ys, n, c->cell_of, c->rank,
 	                   c->cell_start);
 
 	StepArgs a{};
-	a.in = c->Y.s;
+	a.in = Ys;
 	a.out = c->X.s;
 	a.pol = c->dpol;
 	a.worlds = c->dworlds;
@@ -388,6 +417,11 @@ int enqueue_step(umans_ctx* c, bool events) {
 	if (c->record && c->rec_count) {
 		a.rec_count = c->rec_count; a.rec_id = c->rec_id; a.rec_d2 = c->rec_d2; a.rec_seg = c->rec_seg;
 		a.rec_stride = c->rec_stride;
+	}
+	if (raw_nb) {
+		a.raw_nb = raw_nb;
+		a.raw_count = raw_count;
+		a.raw_stride = kmax_bucket(c->kcap);
 	}
 	a.stats = c->dstats;
 	if (e1) hipEventRecord(e1, st);
@@ -438,6 +472,28 @@ int agents_per_world(umans_ctx* c, std::vector<int>& per_world) {
 	if (rc) return rc;
 	for (const int4& m : meta) per_world[m.y >> kPolicyBits]++;
 	return UMANS_OK;
+}
+
+// ------------------------------------------------------------------ adjoint
+bool adjoint_policy_ok(const DevPolicy& p) {
+	if (p.mode != MODE_GRADIENT) return false;
+	for (int c = 0; c < p.n_cost; ++c)
+		if (p.kind[c] != KIND_GOAL_REACHING && p.kind[c] != KIND_POWER_LAW && p.kind[c] != KIND_SOCIAL_FORCES) return false;
+	return true;
+}
+
+void launch_loss(umans_ctx* c, const float2* target, double* loss, float2* grad, float scale) {
+	hipLaunchKernelGGL(k_loss, dim3(ceil_div(c->n, 256)), dim3(256), 0, c->stream, c->X.s, c->n, c->dworlds, target,
+	                   loss, grad, scale);
+}
+
+void fill_f(umans_ctx* c, void* p, long long nfloats) {
+	hipLaunchKernelGGL(k_fill_f, dim3(ceil_div(nfloats, 256)), dim3(256), 0, c->stream, (float*)p, nfloats, 0.0f);
+}
+
+__global__ void k_axpy2(float2* y, const float2* x, int n) {
+	const int i = blockIdx.x * blockDim.x + threadIdx.x;
+	if (i < n) { y[i].x += x[i].x; y[i].y += x[i].y; }
 }
 
 int create_impl(const umans_world_desc* descs, int nworlds, umans_ctx** out) {
@@ -502,11 +558,14 @@ int umans_set_policies(umans_ctx* c, const umans_policy* p, int n) {
 		mr = std::max(mr, v[i].query_range);
 	}
 	if (c->n > 0 && mr > c->max_range) return c->fail(UMANS_ESTATE, "a larger interaction range needs a new upload");
+	if (c->n > 0 && n < (int)c->hpol.size())
+		return c->fail(UMANS_ESTATE, "uploaded agents may use every policy: the table cannot shrink before a new upload");
 	hipSetDevice(c->device);
 	const int rc = c->hip(hipMemcpyAsync(c->dpol, v.data(), n * sizeof(DevPolicy), hipMemcpyHostToDevice, c->stream),
 	                      "policy upload");
 	if (rc) return rc;
 	c->hpol.swap(v);
+	c->adj_ready = false;
 	if (c->n == 0) c->max_range = mr;
 	return c->hip(hipStreamSynchronize(c->stream), "policy upload");
 }
@@ -802,6 +861,128 @@ int umans_dump_neighbors(umans_ctx* c, int stride, int32_t* count, int32_t* ids,
 			seg[(size_t)i * stride + e] = hseg[(size_t)k * rs + e];
 		}
 	}
+	return UMANS_OK;
+}
+
+int umans_adjoint_forward(umans_ctx* c, const umans_adjoint_desc* d, const float* targets, double* loss) {
+	if (!c || !d || !targets) return UMANS_EINVAL;
+	if (c->n == 0) return c->fail(UMANS_ESTATE, "upload agents first");
+	if (d->nsteps < 1 || d->checkpoint_every < 1) return c->fail(UMANS_EINVAL, "bad step or checkpoint count");
+	if (d->param_kind != UMANS_COST_SOCIAL_FORCES && d->param_kind != UMANS_COST_POWER_LAW)
+		return c->fail(UMANS_EINVAL, "parameters are differentiated for SocialForcesAvoidance or PowerLaw");
+	for (size_t i = 0; i < c->hpol.size(); ++i)
+		if (!adjoint_policy_ok(c->hpol[i]))
+			return c->fail(UMANS_EINVAL, "the adjoint needs gradient-method policies of force models (policy " +
+			                                 std::to_string(i) + ")");
+	hipSetDevice(c->device);
+	c->drop_graph();
+	c->free_adjoint();
+	const int n = c->n, T = d->nsteps, ck = d->checkpoint_every;
+	const int nseg = ceil_div(T, ck);
+	const int kmax = kmax_bucket(c->kcap);
+	int rc;
+	c->ckpts.resize(nseg);
+	for (DeviceState& s : c->ckpts)
+		if ((rc = c->hip(s.alloc(n), "alloc checkpoints"))) return rc;
+	c->seg_states.resize(std::min(ck, T));
+	for (DeviceState& s : c->seg_states)
+		if ((rc = c->hip(s.alloc(n), "alloc segment"))) return rc;
+	const int segn = (int)c->seg_states.size();
+	if ((rc = c->hip(dalloc(&c->targets, (size_t)T * n), "alloc targets"))) return rc;
+	if ((rc = c->hip(dalloc(&c->seg_nb, (size_t)segn * n * kmax), "alloc lists"))) return rc;
+	if ((rc = c->hip(dalloc(&c->seg_cnt, (size_t)segn * n), "alloc lists"))) return rc;
+	if ((rc = c->hip(dalloc(&c->seg_lg, (size_t)segn * n), "alloc loss grad"))) return rc;
+	if ((rc = c->hip(dalloc(&c->adj_buf, (size_t)5 * n), "alloc adjoint"))) return rc;
+	if ((rc = c->hip(dalloc(&c->adj_params, 2 * c->hpol.size()), "alloc adjoint"))) return rc;
+	if ((rc = c->hip(dalloc(&c->dloss, 1), "alloc loss"))) return rc;
+	if ((rc = c->hip(hipMemcpyAsync(c->targets, targets, (size_t)T * n * sizeof(float2), hipMemcpyHostToDevice,
+	                                c->stream), "targets"))) return rc;
+	fill_f(c, c->dloss, 2);
+	for (int t = 0; t < T; ++t) {
+		if (t % ck == 0 && (rc = c->hip(copy_state(c->ckpts[t / ck].s, c->X.s, n, c->stream), "checkpoint"))) return rc;
+		if ((rc = enqueue_step(c, false))) return rc;
+		launch_loss(c, c->targets + (size_t)t * n, c->dloss, nullptr, 0.0f);
+		c->steps += 1;
+		c->sim_time += (double)c->worlds[0].delta_time;
+	}
+	double h = 0.0;
+	if ((rc = c->hip(hipMemcpyAsync(&h, c->dloss, sizeof(double), hipMemcpyDeviceToHost, c->stream), "loss"))) return rc;
+	if ((rc = c->hip(hipStreamSynchronize(c->stream), "loss"))) return rc;
+	c->last_loss = h / ((double)T * (double)n);
+	c->adj_steps = T;
+	c->adj_ckpt = ck;
+	c->adj_n = n;
+	c->adj_param_kind = d->param_kind;
+	c->adj_ready = true;
+	if (loss) *loss = c->last_loss;
+	return UMANS_OK;
+}
+
+int umans_adjoint_backward(umans_ctx* c, float* d_params, float* d_pos0, float* d_vel0) {
+	if (!c) return UMANS_EINVAL;
+	if (!c->adj_ready || c->adj_n != c->n) return c->fail(UMANS_ESTATE, "run umans_adjoint_forward first");
+	hipSetDevice(c->device);
+	const int n = c->n, T = c->adj_steps, ck = c->adj_ckpt;
+	const int nseg = ceil_div(T, ck);
+	const int kmax = kmax_bucket(c->kcap);
+	const int npol = (int)c->hpol.size();
+	const float scale = (float)(1.0 / ((double)T * (double)n));
+	int rc;
+	if (!c->B.s.pos && (rc = c->hip(c->B.alloc(c->cap), "alloc backup"))) return rc;
+	if ((rc = c->hip(copy_state(c->B.s, c->X.s, n, c->stream), "backup"))) return rc;
+	float2* ap = c->adj_buf;
+	float2* av = c->adj_buf + n;
+	float2* ap2 = c->adj_buf + 2 * n;
+	float2* av2 = c->adj_buf + 3 * n;
+	float2* apref = c->adj_buf + 4 * n;
+	fill_f(c, c->adj_buf, 2LL * 5 * n);
+	fill_f(c, c->adj_params, 2LL * npol);
+	for (int seg = nseg - 1; seg >= 0; --seg) {
+		const int t0 = seg * ck, t1 = std::min(t0 + ck, T);
+		if ((rc = c->hip(copy_state(c->X.s, c->ckpts[seg].s, n, c->stream), "restore checkpoint"))) return rc;
+		for (int t = t0; t < t1; ++t) {
+			const int s = t - t0;
+			if ((rc = enqueue_step(c, false, &c->seg_states[s].s, c->seg_nb + (size_t)s * n * kmax,
+			                       c->seg_cnt + (size_t)s * n))) return rc;
+			fill_f(c, c->seg_lg + (size_t)s * n, 2LL * n);
+			launch_loss(c, c->targets + (size_t)t * n, nullptr, c->seg_lg + (size_t)s * n, scale);
+		}
+		for (int t = t1 - 1; t >= t0; --t) {
+			const int s = t - t0;
+			hipLaunchKernelGGL(k_axpy2, dim3(ceil_div(n, 256)), dim3(256), 0, c->stream, ap, c->seg_lg + (size_t)s * n, n);
+			fill_f(c, ap2, 2LL * n);
+			fill_f(c, av2, 2LL * n);
+			fill_f(c, apref, 2LL * n);
+			AdjArgs A{};
+			A.in = c->seg_states[s].s;
+			A.nb_count = c->seg_cnt + (size_t)s * n;
+			A.nb = c->seg_nb + (size_t)s * n * kmax;
+			A.stride = kmax;
+			A.pol = c->dpol;
+			A.worlds = c->dworlds;
+			A.n = n;
+			A.adj_pos_out = ap;
+			A.adj_vel_out = av;
+			A.adj_pos_in = ap2;
+			A.adj_vel_in = av2;
+			A.adj_pref = apref;
+			A.adj_params = c->adj_params;
+			A.param_kind = c->adj_param_kind;
+			hipLaunchKernelGGL(k_step_adjoint, dim3(ceil_div(n, 128)), dim3(128), 0, c->stream, A);
+			hipLaunchKernelGGL(k_pref_adjoint, dim3(ceil_div(n, 256)), dim3(256), 0, c->stream, c->seg_states[s].s, n,
+			                   (const float2*)apref, ap2);
+			std::swap(ap, ap2);
+			std::swap(av, av2);
+		}
+	}
+	if ((rc = c->hip(hipGetLastError(), "adjoint launch"))) return rc;
+	std::vector<float> hp(2 * npol);
+	if ((rc = c->hip(hipMemcpyAsync(hp.data(), c->adj_params, hp.size() * sizeof(float), hipMemcpyDeviceToHost, c->stream), "adjoint"))) return rc;
+	if (d_pos0 && (rc = c->hip(hipMemcpyAsync(d_pos0, ap, (size_t)n * sizeof(float2), hipMemcpyDeviceToHost, c->stream), "adjoint"))) return rc;
+	if (d_vel0 && (rc = c->hip(hipMemcpyAsync(d_vel0, av, (size_t)n * sizeof(float2), hipMemcpyDeviceToHost, c->stream), "adjoint"))) return rc;
+	if ((rc = c->hip(copy_state(c->X.s, c->B.s, n, c->stream), "restore"))) return rc;
+	if ((rc = c->hip(hipStreamSynchronize(c->stream), "adjoint"))) return rc;
+	if (d_params) std::memcpy(d_params, hp.data(), hp.size() * sizeof(float));
 	return UMANS_OK;
 }
 

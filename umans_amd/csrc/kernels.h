@@ -438,6 +438,10 @@ __global__ __launch_bounds__(BS) void k_step(StepArgs A) {
 			A.rec_seg[o] = (int)(u >> kSegShift);
 		}
 	}
+	if (A.raw_nb) {
+		A.raw_count[k] = cnt;
+		for (int e = 0; e < cnt; ++e) A.raw_nb[(size_t)k * A.raw_stride + e] = s_nb[e * BS + tid];
+	}
 	const NbList L{s_nb + tid, BS, cnt, S.pos, S.vel, S.prm, S.pref, wi.W, wi.H, a.pos};
 	uint32_t rng = (uint32_t)meta.w;
 

@@ -95,6 +95,10 @@ struct StepArgs {
 	double* rec_d2;
 	int* rec_seg;
 	int rec_stride;
+	// optional raw lists (sorted index | segment << kSegShift, stride KMAX) for the adjoint
+	unsigned* raw_nb;
+	int* raw_count;
+	int raw_stride;
 	unsigned long long* stats;    // [0] overflow agents, [1] max list length
 };
 

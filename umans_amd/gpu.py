@@ -24,7 +24,7 @@ EXPORTS = [
     "umans_set_record_neighbors", "umans_dump_neighbors", "umans_cost_id", "umans_cost_name",
     "umans_cost_param_index", "umans_cost_param_name", "umans_cost_defaults", "umans_policy_init",
     "umans_set_profiling", "umans_get_profile", "umans_create_batch", "umans_num_worlds",
-    "umans_set_graph", "umans_autotune", "umans_set_tuning",
+    "umans_set_graph", "umans_autotune", "umans_set_tuning", "umans_adjoint_forward", "umans_adjoint_backward",
 ]
 
 
@@ -61,6 +61,10 @@ class Profile(C.Structure):
 
 class Tuning(C.Structure):
     _fields_ = [("cell_factor", C.c_float), ("block_size", C.c_int32), ("ms_per_step", C.c_float)]
+
+
+class AdjointDesc(C.Structure):
+    _fields_ = [("nsteps", C.c_int32), ("checkpoint_every", C.c_int32), ("param_kind", C.c_int32)]
 
 
 class UmansError(RuntimeError):
@@ -105,6 +109,8 @@ def lib():
         L.umans_set_graph.argtypes = [vp, C.c_int]
         L.umans_autotune.argtypes = [vp, C.c_int, C.POINTER(Tuning)]
         L.umans_set_tuning.argtypes = [vp, C.POINTER(Tuning)]
+        L.umans_adjoint_forward.argtypes = [vp, C.POINTER(AdjointDesc), _F, C.POINTER(C.c_double)]
+        L.umans_adjoint_backward.argtypes = [vp, _F, _F, _F]
         L.umans_set_profiling.argtypes = [vp, C.c_int]
         L.umans_get_profile.argtypes = [vp, C.POINTER(Profile)]
         _lib = L
@@ -235,6 +241,29 @@ class World:
     def set_tuning(self, cell_factor, block_size=0):
         t = Tuning(float(cell_factor), int(block_size), 0.0)
         self._check(lib().umans_set_tuning(self._h, C.byref(t)))
+
+    def set_policies(self, policies):
+        self._check(lib().umans_set_policies(self._h, (PolicyC * len(policies))(*[policy_struct(p) for p in policies]),
+                                             len(policies)))
+
+    def adjoint_forward(self, targets, nsteps, checkpoint_every=10, param_kind=2):
+        """Run nsteps from the current state with checkpoints; return the position loss
+        against targets (nsteps x n x 2, upload order)."""
+        t = np.ascontiguousarray(targets, dtype=np.float32)
+        if t.shape != (nsteps, self.n, 2):
+            raise ValueError(f"targets must have shape {(nsteps, self.n, 2)}")
+        self._adj_keep = t
+        loss = C.c_double()
+        d = AdjointDesc(int(nsteps), int(checkpoint_every), int(param_kind))
+        self._check(lib().umans_adjoint_forward(self._h, C.byref(d), _p(t, _F), C.byref(loss)))
+        return loss.value
+
+    def adjoint_backward(self, n_policies):
+        dp = np.zeros((n_policies, 2), np.float32)
+        dpos = np.zeros((self.n, 2), np.float32)
+        dvel = np.zeros((self.n, 2), np.float32)
+        self._check(lib().umans_adjoint_backward(self._h, _p(dp, _F), _p(dpos, _F), _p(dvel, _F)))
+        return dp, dpos, dvel
 
     def set_profiling(self, enable=True):
         self._check(lib().umans_set_profiling(self._h, 1 if enable else 0))

@@ -225,3 +225,36 @@ def sweep(n_worlds=512, n_min=20, n_max=5000, steps=500, seed=4, max_neighbors=0
         parts.append(a)
     agents = {k: np.concatenate([p[k] for p in parts]) for k in parts[0]}
     return Batch("sweep%d" % n_worlds, worlds, pols, agents, max_neighbors, steps, sizes)
+
+
+def calibration(n=10_000, size=100.0, V0=1500.0, sigma=0.12, seed=5, dt=0.1, steps=100, max_accel=50.0,
+                max_neighbors=0):
+    """Adjoint calibration crowd: n agents uniform in a size x size toric world, goals
+    100 m away in a random direction, {SocialForcesAvoidance (A = V0, B = sigma,
+    agentMass 80, range 5 m) + GoalReachingForce} with the gradient method, dt 0.1 s.
+    Initial velocities v0 = v_pref (1 + 0.1 N(0,1)). The maximum acceleration is raised
+    to 50 m/s^2 so that the acceleration clamp does not flatten dL/dA."""
+    r = np.random.default_rng(seed)
+    a = _agents(n)
+    a["pos"][:] = r.uniform(-size / 2, size / 2, (n, 2)).astype(np.float32)
+    phi = r.uniform(0.0, 2.0 * np.pi, n)
+    a["goal"][:, 0] = (a["pos"][:, 0] + 100.0 * np.cos(phi)).astype(np.float32)
+    a["goal"][:, 1] = (a["pos"][:, 1] + 100.0 * np.sin(phi)).astype(np.float32)
+    a["max_accel"][:] = max_accel
+    d = a["goal"] - a["pos"]
+    pref = d / np.linalg.norm(d, axis=1, keepdims=True) * a["pref_speed"][:, None]
+    a["vel"][:] = (pref * (1.0 + 0.1 * r.standard_normal(n))[:, None]).astype(np.float32)
+    a["viewdir"] = initial_viewdir(a["pos"], a["goal"])
+    pol = socialforces_gradient_policy(V0=V0, sigma=sigma, rng=5.0, mass=80.0, relax=0.5)
+    return Scenario("calibration%d" % n, 1, size, size, dt, max_neighbors, [pol], a, steps)
+
+
+def with_sf_params(packed, V0, sigma):
+    """Copy of a packed policy with the SocialForcesAvoidance V0 and sigma replaced."""
+    q = {k: (list(v) if isinstance(v, list) else v) for k, v in packed.items()}
+    q["cost_params"] = [list(x) for x in packed["cost_params"]]
+    for c in range(q["n_cost"]):
+        if q["cost_kind"][c] == 2:
+            q["cost_params"][c][2] = float(V0)
+            q["cost_params"][c][3] = float(sigma)
+    return q
