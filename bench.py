@@ -33,6 +33,8 @@ def workload(name):
         return S.circle(50), RL.flops_orca(10)
     if name == "sweep":
         return S.sweep(), RL.flops_powerlaw_sampling(30, 72)
+    if name == "calibration":
+        return None, None
     raise SystemExit(f"unknown workload {name}")
 
 
@@ -100,6 +102,57 @@ def copy_probe(mib):
     return 0
 
 
+def adjoint_leg(steps=100, ck=10):
+    """Adjoint calibration crowd: 10k agents, 100 x 100 m toric world, SocialForces +
+    GoalReachingForce (gradient method), dt 0.1 s, 100 steps. Targets from (A, B) =
+    (2000, 0.08); gradients at (1500, 0.12) w.r.t. (A, B) and the 10k initial
+    velocities, checkpoints every 10 steps. Reports forward and backward time per step
+    against the plain step, and central differences (h = 1e-3 relative, FP32 GPU
+    forward) next to the adjoint."""
+    import numpy as np
+    from umans_amd import adjoint as ADJ, gpu
+    sc = S.calibration()
+    w = gpu.World(sc.world_type, sc.width, sc.height, sc.dt, sc.policies, sc.max_neighbors)
+    tg = ADJ.rollout_targets(w, sc, 2000.0, 0.08, steps)
+    A0, B0 = 1500.0, 0.12
+
+    def loss_at(A, B):
+        w.set_policies([S.with_sf_params(p, A, B) for p in sc.policies])
+        w.upload(sc.agents)
+        return w.adjoint_forward(tg, steps, ck)
+
+    loss_at(A0, B0)                                   # warm-up
+    w.upload(sc.agents)
+    w.synchronize()
+    t0 = time.perf_counter()
+    w.step(steps)
+    w.synchronize()
+    t_plain = time.perf_counter() - t0
+    w.set_policies([S.with_sf_params(p, A0, B0) for p in sc.policies])
+    w.upload(sc.agents)
+    t0 = time.perf_counter()
+    loss = w.adjoint_forward(tg, steps, ck)
+    t_fwd = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    dp, _, dv = w.adjoint_backward(1)
+    t_bwd = time.perf_counter() - t0
+    fd = []
+    for i, (x, h) in enumerate(((A0, 1e-3 * A0), (B0, 1e-3 * B0))):
+        a = [A0, B0]
+        a[i] = x + h
+        lp = loss_at(*a)
+        a[i] = x - h
+        lm = loss_at(*a)
+        fd.append((lp - lm) / (2 * h))
+    w.close()
+    return {"metric": "backward/forward time", "value": t_bwd / t_plain, "unit": "ratio",
+            "workload": sc.name, "agents": sc.n, "steps": steps, "checkpoint_every": ck,
+            "plain_ms_per_step": t_plain * 1e3 / steps, "adjoint_forward_ms_per_step": t_fwd * 1e3 / steps,
+            "backward_ms_per_step": t_bwd * 1e3 / steps, "loss": loss,
+            "dL_dA": float(dp[0, 0]), "dL_dB": float(dp[0, 1]), "dL_dv0_norm": float(np.linalg.norm(dv)),
+            "fd_h1e-3_dA": fd[0], "fd_h1e-3_dB": fd[1]}
+
+
 def measured_traffic(kernel="k_step"):
     """HBM bytes per launch of `kernel` on the mixed crowd from the committed rocprofv3
     FETCH_SIZE/WRITE_SIZE summary (scaled by the copy probe), or None."""
@@ -124,6 +177,8 @@ def main():
     ap.add_argument("--cpu-threads", type=int, default=min(16, os.cpu_count() or 1))
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-autotune", action="store_true")
+    ap.add_argument("--no-extras", action="store_true",
+                    help="skip the batched-sweep and adjoint legs added to the default line")
     ap.add_argument("--graph", action="store_true", help="replay a captured hipGraph per step "
                     "(no per-kernel timing then)")
     ap.add_argument("--copy-probe-mib", type=int, default=0,
@@ -138,6 +193,9 @@ def main():
         return 2
 
     from umans_amd import gpu
+    if args.workload == "calibration":
+        print(json.dumps(adjoint_leg()))
+        return 0
     sc, flops_per_agent = workload(args.workload)
     batched = isinstance(sc, S.Batch)
     if batched:
@@ -204,6 +262,25 @@ def main():
         print(f"# GPU leg done ({value:.3e} agent-steps/s); timing the CPU baseline", file=sys.stderr, flush=True)
         out["cpu_baseline"] = cpu_baseline(sc, args.cpu_steps, args.cpu_threads)
         out["cpu_baseline"]["gpu_over_cpu"] = value / out["cpu_baseline"]["value"]
+    if not args.no_extras and sc.name.startswith("mixed"):
+        print("# timing the batched sweep and the adjoint legs", file=sys.stderr, flush=True)
+        sw, _ = workload("sweep")
+        wb = gpu.World(0, 0, 0, 0.1, sw.policies, max_neighbors=sw.max_neighbors, batch=sw.worlds)
+        wb.upload(sw.agents)
+        tb = wb.autotune(trial_steps=3)
+        wb.set_graph(True)
+        wb.step(3)
+        wb.synchronize()
+        steps_b = 100
+        t0 = time.perf_counter()
+        wb.step(steps_b)
+        wb.synchronize()
+        tbt = time.perf_counter() - t0
+        wb.close()
+        out["batched"] = {"metric": "batched agent-steps/s", "value": sw.n * steps_b / tbt, "unit": "agent-steps/s",
+                          "worlds": len(sw.worlds), "agents": sw.n, "steps": steps_b,
+                          "ms_per_step": tbt * 1e3 / steps_b, "graph": True, "tuning": tb}
+        out["adjoint"] = adjoint_leg()
     print(json.dumps(out))
     return 0
 

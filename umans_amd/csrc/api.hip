@@ -70,26 +70,42 @@ bool block_supported(int kmax, int bs) {
 }
 int default_block(int kmax) { return kmax <= 32 ? 128 : 64; }
 
-template <int KMAX, int BS>
+template <int KMAX, int BS, int CLS>
 void launch_step_t(const StepArgs& a, hipStream_t st) {
 	const int blocks = (a.n + BS - 1) / BS;
-	if (blocks > 0) hipLaunchKernelGGL((k_step<KMAX, BS>), dim3(blocks), dim3(BS), 0, st, a);
+	if (blocks > 0) hipLaunchKernelGGL((k_step<KMAX, BS, CLS>), dim3(blocks), dim3(BS), 0, st, a);
 }
 
-void launch_step(int kmax, int bs, const StepArgs& a, hipStream_t st) {
+template <int CLS>
+void launch_step_c(int kmax, int bs, const StepArgs& a, hipStream_t st) {
 	switch (kmax) {
 	case 16:
-		if (bs == 64) launch_step_t<16, 64>(a, st);
-		else if (bs == 256) launch_step_t<16, 256>(a, st);
-		else launch_step_t<16, 128>(a, st);
+		if (bs == 64) launch_step_t<16, 64, CLS>(a, st);
+		else if (bs == 256) launch_step_t<16, 256, CLS>(a, st);
+		else launch_step_t<16, 128, CLS>(a, st);
 		break;
 	case 32:
-		if (bs == 64) launch_step_t<32, 64>(a, st);
-		else launch_step_t<32, 128>(a, st);
+		if (bs == 64) launch_step_t<32, 64, CLS>(a, st);
+		else launch_step_t<32, 128, CLS>(a, st);
 		break;
-	case 64: launch_step_t<64, 64>(a, st); break;
-	default: launch_step_t<128, 64>(a, st); break;
+	case 64: launch_step_t<64, 64, CLS>(a, st); break;
+	default: launch_step_t<128, 64, CLS>(a, st); break;
 	}
+}
+
+void launch_step(int cls, int kmax, int bs, const StepArgs& a, hipStream_t st) {
+	switch (cls) {
+	case CLS_FORCE_GRAD: launch_step_c<CLS_FORCE_GRAD>(kmax, bs, a, st); break;
+	case CLS_FORCE_SAMPLE: launch_step_c<CLS_FORCE_SAMPLE>(kmax, bs, a, st); break;
+	case CLS_ORCA: launch_step_c<CLS_ORCA>(kmax, bs, a, st); break;
+	default: launch_step_c<CLS_GENERIC>(kmax, bs, a, st); break;
+	}
+}
+
+// The ORCA class keeps its half-planes in LDS (2 x KMAX x block x 16 B): smaller blocks.
+int class_block(int cls, int kmax, int bs) {
+	if (cls == CLS_ORCA && kmax <= 32) return 64;
+	return bs;
 }
 
 int ceil_div(long long a, long long b) { return (int)((a + b - 1) / b); }
@@ -122,6 +138,13 @@ struct umans_ctx {
 	bool any_infinite = false;
 	unsigned long long* dstats = nullptr;
 	int kcap = 0;
+	// policy classes: one step-kernel instance per class over a bucket of agents
+	std::vector<int> h_policy;           // policy of each uploaded agent
+	int cls_count[NUM_CLS] = {0, 0, 0, 0};
+	int cls_off[NUM_CLS] = {0, 0, 0, 0};
+	int* d_cls_off = nullptr;
+	unsigned* d_cursor = nullptr;
+	unsigned* d_list = nullptr;
 	float cell_factor = 1.0f;    // cell width = factor x max interaction range (autotuned)
 	int block = 0;               // step-kernel block size (0 = default; autotuned)
 
@@ -193,7 +216,8 @@ struct umans_ctx {
 		drop_graph();
 		free_adjoint();
 		X.release(); Y.release(); B.release();
-		hipFree(cell_of); hipFree(rank);
+		hipFree(cell_of); hipFree(rank); hipFree(d_list);
+		d_list = nullptr;
 		hipFree(rec_count); hipFree(rec_id); hipFree(rec_d2); hipFree(rec_seg);
 		cell_of = rank = nullptr;
 		rec_count = rec_id = rec_seg = nullptr;
@@ -298,6 +322,14 @@ int make_dev_policy(const umans_policy& p, DevPolicy& d, std::string& err) {
 		err = "regular sampling needs SpeedSamples >= 1 and AngleSamples >= 1";
 		return UMANS_EINVAL;
 	}
+	bool all_force = true;
+	for (int c = 0; c < d.n_cost; ++c)
+		if (d.kind[c] != KIND_GOAL_REACHING && d.kind[c] != KIND_POWER_LAW && d.kind[c] != KIND_SOCIAL_FORCES)
+			all_force = false;
+	if (all_force && (d.mode == MODE_GRADIENT || d.mode == MODE_GLOBAL_FORCE)) d.cls = CLS_FORCE_GRAD;
+	else if (all_force && d.mode == MODE_SAMPLE) d.cls = CLS_FORCE_SAMPLE;
+	else if (d.mode == MODE_GLOBAL_ORCA) d.cls = CLS_ORCA;
+	else d.cls = CLS_GENERIC;
 	// costFunction.cpp:90 and :122-125, in the reference's float/double mix
 	const float maxAngle = (float)(s.s_angle / 360.0 * 3.1415926535897);
 	d.max_angle = maxAngle;
@@ -371,6 +403,16 @@ int setup_worlds(umans_ctx* c, const std::vector<int>& per_world) {
 	              "world upload");
 }
 
+// Agents per policy class from the uploaded policies (host copy), and the bucket offsets.
+int update_classes(umans_ctx* c) {
+	for (int q = 0; q < NUM_CLS; ++q) c->cls_count[q] = 0;
+	for (int p : c->h_policy) c->cls_count[c->hpol[p].cls]++;
+	int off = 0;
+	for (int q = 0; q < NUM_CLS; ++q) { c->cls_off[q] = off; off += c->cls_count[q]; }
+	return c->hip(hipMemcpyAsync(c->d_cls_off, c->cls_off, sizeof(c->cls_off), hipMemcpyHostToDevice, c->stream),
+	              "class offsets");
+}
+
 int enqueue_step(umans_ctx* c, bool events, const State* ysave = nullptr, unsigned* raw_nb = nullptr,
                  int* raw_count = nullptr) {
 	const int n = c->n;
@@ -424,9 +466,28 @@ int enqueue_step(umans_ctx* c, bool events, const State* ysave = nullptr, unsign
 		a.raw_stride = kmax_bucket(c->kcap);
 	}
 	a.stats = c->dstats;
-	if (e1) hipEventRecord(e1, st);
 	const int kmax = kmax_bucket(c->kcap);
-	launch_step(kmax, c->block > 0 ? c->block : default_block(kmax), a, st);
+	const int bs = c->block > 0 ? c->block : default_block(kmax);
+	int nclasses = 0, only = CLS_GENERIC;
+	for (int q = 0; q < NUM_CLS; ++q)
+		if (c->cls_count[q] > 0) { ++nclasses; only = q; }
+	if (nclasses > 1) {
+		hipLaunchKernelGGL(k_fill, dim3(1), dim3(64), 0, st, c->d_cursor, NUM_CLS, 0u);
+		hipLaunchKernelGGL(k_bucket, dim3(ceil_div(n, 256)), dim3(256), 0, st, Ys.meta, c->dpol, n, c->d_cls_off,
+		                   c->d_cursor, c->d_list);
+	}
+	if (e1) hipEventRecord(e1, st);
+	if (nclasses <= 1) {
+		launch_step(only, kmax, class_block(only, kmax, bs), a, st);
+	} else {
+		for (int q = 0; q < NUM_CLS; ++q) {
+			if (!c->cls_count[q]) continue;
+			StepArgs aq = a;
+			aq.list = c->d_list + c->cls_off[q];
+			aq.n = c->cls_count[q];
+			launch_step(q, kmax, class_block(q, kmax, bs), aq, st);
+		}
+	}
 	if (e2) hipEventRecord(e2, st);
 	return c->hip(hipGetLastError(), "step launch");
 }
@@ -512,7 +573,8 @@ int create_impl(const umans_world_desc* descs, int nworlds, umans_ctx** out) {
 	if (hipSetDevice(c->device) != hipSuccess || hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess ||
 	    dalloc(&c->dgrids, nworlds) != hipSuccess || dalloc(&c->dworlds, nworlds) != hipSuccess ||
 	    dalloc(&c->dbbox, 4 * nworlds) != hipSuccess || dalloc(&c->dstats, 2) != hipSuccess ||
-	    dalloc(&c->dpol, kMaxPolicies) != hipSuccess) {
+	    dalloc(&c->dpol, kMaxPolicies) != hipSuccess || dalloc(&c->d_cls_off, NUM_CLS) != hipSuccess ||
+	    dalloc(&c->d_cursor, NUM_CLS) != hipSuccess) {
 		umans_destroy(c);
 		return UMANS_EHIP;
 	}
@@ -538,6 +600,7 @@ void umans_destroy(umans_ctx* c) {
 	if (c->stream) hipStreamSynchronize(c->stream);
 	c->free_agents();
 	hipFree(c->dgrids); hipFree(c->dworlds); hipFree(c->dbbox); hipFree(c->dstats); hipFree(c->dpol);
+	hipFree(c->d_cls_off); hipFree(c->d_cursor);
 	hipFree(c->cell_count); hipFree(c->cell_start); hipFree(c->scan_sums);
 	c->free_events();
 	if (c->stream) hipStreamDestroy(c->stream);
@@ -566,7 +629,9 @@ int umans_set_policies(umans_ctx* c, const umans_policy* p, int n) {
 	if (rc) return rc;
 	c->hpol.swap(v);
 	c->adj_ready = false;
+	c->drop_graph();
 	if (c->n == 0) c->max_range = mr;
+	else if (int rc2 = update_classes(c)) return rc2;
 	return c->hip(hipStreamSynchronize(c->stream), "policy upload");
 }
 
@@ -619,12 +684,15 @@ int umans_upload(umans_ctx* c, const umans_agents* a, int n) {
 		if ((rc = c->hip(c->Y.alloc(n), "alloc state"))) return rc;
 		if ((rc = c->hip(dalloc(&c->cell_of, n), "alloc grid"))) return rc;
 		if ((rc = c->hip(dalloc(&c->rank, n), "alloc grid"))) return rc;
+		if ((rc = c->hip(dalloc(&c->d_list, n), "alloc buckets"))) return rc;
 		c->cap = n;
 	}
 	c->n = n;
+	c->h_policy.assign(a->policy, a->policy + n);
 	c->have_record = false;
 	int rc;
 	if ((rc = setup_worlds(c, per_world))) return rc;
+	if ((rc = update_classes(c))) return rc;
 	const State& s = c->X.s;
 	if ((rc = c->hip(hipMemcpyAsync(s.pos, pos.data(), n * sizeof(float2), hipMemcpyHostToDevice, c->stream), "upload"))) return rc;
 	if ((rc = c->hip(hipMemcpyAsync(s.vel, vel.data(), n * sizeof(float2), hipMemcpyHostToDevice, c->stream), "upload"))) return rc;

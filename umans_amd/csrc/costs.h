@@ -183,11 +183,18 @@ UD bool is_force_based(int kind) {
 
 // ----------------------------------------------------------------------- ORCA
 struct Line { V2 point, dir; };
+// A lane's half-planes: contiguous in private memory (stride 1) or interleaved with the
+// other lanes of the block in LDS (stride = block size).
+struct Lines {
+	Line* p;
+	int s;
+	UD Line& operator[](int i) const { return p[i * s]; }
+};
 constexpr float kOrcaEps = 0.00001f;   // ORCASolver.h:62
 
 // Solver::createAgentOrcaLines (ORCASolver.cpp:96-171), including the collision branch
 // that divides by the squared length (ORCASolver.cpp:161-165).
-UD int orca_lines(const float* prm, float range, const Self& a, const NbList& L, Line* lines) {
+UD int orca_lines(const float* prm, float range, const Self& a, const NbList& L, Lines lines) {
 	const float tau = prm[0];
 	const float maxDistSq = range * range;
 	int n = 0;
@@ -232,7 +239,7 @@ UD int orca_lines(const float* prm, float range, const Self& a, const NbList& L,
 	return n;
 }
 // linearProgram1 (ORCASolver.cpp:173-244)
-UD bool lp1(const Line* L, int no, float radius, V2 opt, bool dirOpt, V2& res) {
+UD bool lp1(Lines L, int no, float radius, V2 opt, bool dirOpt, V2& res) {
 	const float dp = dot(L[no].point, L[no].dir);
 	const float disc = dp * dp + radius * radius - sqr_mag(L[no].point);
 	if (disc < 0.0f) return false;
@@ -261,7 +268,7 @@ UD bool lp1(const Line* L, int no, float radius, V2 opt, bool dirOpt, V2& res) {
 	return true;
 }
 // linearProgram2 (ORCASolver.cpp:246-277)
-UD int lp2(const Line* L, int n, float radius, V2 opt, bool dirOpt, V2& res) {
+UD int lp2(Lines L, int n, float radius, V2 opt, bool dirOpt, V2& res) {
 	if (dirOpt) res = opt * radius;
 	else if (sqr_mag(opt) > radius * radius) res = normalized(opt) * radius;
 	else res = opt;
@@ -274,7 +281,7 @@ UD int lp2(const Line* L, int n, float radius, V2 opt, bool dirOpt, V2& res) {
 	return n;
 }
 // linearProgram3 (ORCASolver.cpp:279-326) without obstacle lines
-UD void lp3(const Line* L, int n, int begin, float radius, V2& res, Line* proj) {
+UD void lp3(Lines L, int n, int begin, float radius, V2& res, Lines proj) {
 	float distance = 0.0f;
 	for (int i = begin; i < n; ++i) {
 		if (det(L[i].dir, L[i].point - res) > distance) {
@@ -298,14 +305,14 @@ UD void lp3(const Line* L, int n, int begin, float radius, V2& res, Line* proj) 
 	}
 }
 // Solver::solveOrcaProgram (ORCASolver.cpp:61-94)
-UD V2 orca_solve(const Line* L, int n, const Self& a, Line* proj) {
+UD V2 orca_solve(Lines L, int n, const Self& a, Lines proj) {
 	V2 res;
 	const int fail = lp2(L, n, a.max_speed, a.pref, false, res);
 	if (fail != n) lp3(L, n, fail, a.max_speed, res, proj);
 	return res;
 }
 // ORCA::GetCost (ORCA.cpp:49-86)
-UD float orca_cost(const Line* L, int n, const Self& a, V2 v) {
+UD float orca_cost(Lines L, int n, const Self& a, V2 v) {
 	float maxD = -kMaxFloat;
 	for (int i = 0; i < n; ++i) {
 		const V2 pmv = L[i].point - v;

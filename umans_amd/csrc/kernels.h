@@ -299,54 +299,78 @@ UD int find_neighbors(const StepArgs& A, const WorldInfo& wi, const GridInfo& g,
 }
 
 // --------------------------------------------------------------- per-agent state
-template <int KMAX>
+// Policy classes: agents are bucketed by class each step and every class runs its own
+// instance of the step kernel with only its code path compiled in.
+enum { CLS_FORCE_GRAD = 0, CLS_FORCE_SAMPLE = 1, CLS_ORCA = 2, CLS_GENERIC = 3, NUM_CLS = 4 };
+
+// Loop over a policy's cost functions: fully unrolled over the 4 slots for the force
+// classes (so per-cost values stay in registers), a plain loop otherwise.
+template <int CLS, typename F>
+UD void for_costs(const DevPolicy& P, F&& f) {
+	if constexpr (CLS == CLS_GENERIC) {
+		for (int c = 0; c < P.n_cost; ++c) f(c);
+	} else {
+#pragma unroll
+		for (int c = 0; c < 4; ++c)
+			if (c < P.n_cost) f(c);
+	}
+}
+
+template <int KMAX, int CLS>
 struct Prep {
 	V2 target[4];            // force-based costs: target velocity
 	float ttcp[4];           // Karamouzas: TTC at the preferred velocity
-	Line lines[KMAX];        // ORCA half-planes
+	Lines lines;             // ORCA half-planes
 	int n_lines;
 };
 
 // CostFunction::GetCost of cost c at velocity v
-template <int KMAX>
-UD float cost_of(const DevPolicy& P, int c, const Self& a, const NbList& L, const Prep<KMAX>& pr, V2 v) {
+template <int KMAX, int CLS>
+UD float cost_of(const DevPolicy& P, int c, const Self& a, const NbList& L, const Prep<KMAX, CLS>& pr, V2 v) {
 	const int kind = P.kind[c];
-	const float* prm = P.params[c];
-	const float range = P.range[c];
-	switch (kind) {
-	case KIND_GOAL_REACHING: case KIND_POWER_LAW: case KIND_SOCIAL_FORCES:
-		return 0.5f * sqr_mag(v - pr.target[c]) / a.dt;     // ForceBasedFunction.cpp:26-30
-	case KIND_ORCA: return orca_cost(pr.lines, pr.n_lines, a, v);
-	case KIND_KARAMOUZAS: return karamouzas_cost(prm, range, a, L, v, pr.ttcp[c]);
-	case KIND_RVO: return rvo_cost(prm, range, a, L, v);
-	case KIND_MOUSSAID: return moussaid_cost(prm, range, a, L, v);
-	case KIND_TTCA_DCA: return ttcadca_cost(prm, range, a, L, v);
-	case KIND_PARIS: return paris_cost(prm, range, a, L, v);
-	case KIND_VAN_TOLL: return vantoll_cost(prm, range, a, L, v);
+	if constexpr (CLS == CLS_FORCE_GRAD || CLS == CLS_FORCE_SAMPLE) {
+		return 0.5f * sqr_mag(v - pr.target[c]) / a.dt;       // ForceBasedFunction.cpp:26-30
+	} else {
+		const float* prm = P.params[c];
+		const float range = P.range[c];
+		switch (kind) {
+		case KIND_GOAL_REACHING: case KIND_POWER_LAW: case KIND_SOCIAL_FORCES:
+			return 0.5f * sqr_mag(v - pr.target[c]) / a.dt;
+		case KIND_ORCA: return orca_cost(pr.lines, pr.n_lines, a, v);
+		case KIND_KARAMOUZAS: return karamouzas_cost(prm, range, a, L, v, pr.ttcp[c]);
+		case KIND_RVO: return rvo_cost(prm, range, a, L, v);
+		case KIND_MOUSSAID: return moussaid_cost(prm, range, a, L, v);
+		case KIND_TTCA_DCA: return ttcadca_cost(prm, range, a, L, v);
+		case KIND_PARIS: return paris_cost(prm, range, a, L, v);
+		case KIND_VAN_TOLL: return vantoll_cost(prm, range, a, L, v);
+		}
+		return 0.0f;
 	}
-	return 0.0f;
 }
 
 // CostFunction::GetGradient: analytic for force-based costs (ForceBasedFunction.cpp:32-36)
 // and TtcaDca; central differences with delta 0.1 otherwise (costFunction.cpp:36-54).
-template <int KMAX>
-UD V2 gradient_of(const DevPolicy& P, int c, const Self& a, const NbList& L, const Prep<KMAX>& pr, V2 v) {
+template <int KMAX, int CLS>
+UD V2 gradient_of(const DevPolicy& P, int c, const Self& a, const NbList& L, const Prep<KMAX, CLS>& pr, V2 v) {
 	const int kind = P.kind[c];
-	if (is_force_based(kind)) return (v - pr.target[c]) / a.dt;
-	if (kind == KIND_TTCA_DCA) return ttcadca_gradient(P.params[c], P.range[c], a, L, v);
-	const float d = 0.1f;
-	const V2 dx{d, 0.0f}, dy{0.0f, d};
-	const float cl = cost_of<KMAX>(P, c, a, L, pr, v - dx), cr = cost_of<KMAX>(P, c, a, L, pr, v + dx);
-	const float gx = (cr - cl) / (2.0f * d);
-	const float cb = cost_of<KMAX>(P, c, a, L, pr, v - dy), ct = cost_of<KMAX>(P, c, a, L, pr, v + dy);
-	const float gy = (ct - cb) / (2.0f * d);
-	return V2{gx, gy};
+	if (CLS == CLS_FORCE_GRAD || CLS == CLS_FORCE_SAMPLE || is_force_based(kind)) return (v - pr.target[c]) / a.dt;
+	if constexpr (CLS == CLS_GENERIC) {
+		if (kind == KIND_TTCA_DCA) return ttcadca_gradient(P.params[c], P.range[c], a, L, v);
+		const float d = 0.1f;
+		const V2 dx{d, 0.0f}, dy{0.0f, d};
+		const float cl = cost_of<KMAX, CLS>(P, c, a, L, pr, v - dx), cr = cost_of<KMAX, CLS>(P, c, a, L, pr, v + dx);
+		const float gx = (cr - cl) / (2.0f * d);
+		const float cb = cost_of<KMAX, CLS>(P, c, a, L, pr, v - dy), ct = cost_of<KMAX, CLS>(P, c, a, L, pr, v + dy);
+		const float gy = (ct - cb) / (2.0f * d);
+		return V2{gx, gy};
+	}
+	return V2{0.0f, 0.0f};
 }
 
 // CostFunction::ApproximateGlobalMinimumBySampling (costFunction.cpp:66-164): the first
 // strict minimum of sum(coeff * cost) wins; (0,0) if no candidate beats MaxFloat.
-template <int KMAX>
-UD V2 sample_best(const DevPolicy& P, const Self& a, const NbList& L, const Prep<KMAX>& pr, uint32_t& rng) {
+template <int KMAX, int CLS>
+UD V2 sample_best(const DevPolicy& P, const Self& a, const NbList& L, const Prep<KMAX, CLS>& pr, uint32_t& rng) {
 	const V2 base = P.s_base == 1 ? a.vel : V2{0.0f, 0.0f};
 	float radius = a.pref_speed;
 	if (P.s_radius == 1) radius = a.max_speed;
@@ -358,7 +382,7 @@ UD V2 sample_best(const DevPolicy& P, const Self& a, const NbList& L, const Prep
 	float bestCost = kMaxFloat;
 	auto eval = [&](V2 v) {
 		float tc = 0.0f;
-		for (int c = 0; c < P.n_cost; ++c) tc += P.coeff[c] * cost_of<KMAX>(P, c, a, L, pr, v);
+		for_costs<CLS>(P, [&](int c) { tc += P.coeff[c] * cost_of<KMAX, CLS>(P, c, a, L, pr, v); });
 		if (tc < bestCost) { best = v; bestCost = tc; }
 	};
 	if (P.s_type == 1) {
@@ -387,13 +411,21 @@ UD V2 sample_best(const DevPolicy& P, const Self& a, const NbList& L, const Prep
 }
 
 // ------------------------------------------------------------------ the step
-template <int KMAX, int BS>
+// One lane per agent of the class's bucket (A.list) or of the whole population.
+template <int KMAX, int BS, int CLS>
 __global__ __launch_bounds__(BS) void k_step(StepArgs A) {
 	__shared__ double s_d2[KMAX * BS];
 	__shared__ unsigned s_nb[KMAX * BS];
+	// ORCA half-planes in LDS when they fit beside the neighbour lists (2 x 16 KB at
+	// KMAX x block = 1024); larger instances keep them in private memory.
+	constexpr bool kLinesInLds = (CLS == CLS_ORCA) && (KMAX * BS <= 1024);
+	constexpr int kLdsLines = kLinesInLds ? KMAX * BS : 1;
+	__shared__ Line s_lines[kLdsLines];
+	__shared__ Line s_proj[kLdsLines];
 	const int tid = threadIdx.x;
-	const int k = blockIdx.x * BS + tid;
-	const bool active = k < A.n;
+	const int t = blockIdx.x * BS + tid;
+	const bool active = t < A.n;
+	const int k = active ? (A.list ? (int)A.list[t] : t) : 0;
 	int total = 0, cnt = 0;
 	const State& S = A.in;
 	int4 meta = make_int4(0, 0, 0, 0);
@@ -448,30 +480,48 @@ __global__ __launch_bounds__(BS) void k_step(StepArgs A) {
 	// Policy::ComputeNewVelocity (policy.cpp:43-76)
 	V2 next{0.0f, 0.0f};
 	if (!is_zero(a.pref)) {
-		Prep<KMAX> pr;
+		Prep<KMAX, CLS> pr;
 		pr.n_lines = 0;
-		Line proj[KMAX];
-		for (int c = 0; c < P.n_cost; ++c) {
-			const int kind = P.kind[c];
-			if (is_force_based(kind)) pr.target[c] = target_velocity(P, c, a, L);
-			else if (kind == KIND_ORCA) pr.n_lines = orca_lines(P.params[c], P.range[c], a, L, pr.lines);
-			else if (kind == KIND_KARAMOUZAS)
-				pr.ttcp[c] = time_to_first_collision(L, a.pos, a.pref, a.radius, P.range[c], true);
-		}
-		V2 acc;
-		if (P.mode == MODE_GRADIENT) {
-			V2 tg{0.0f, 0.0f};
-			for (int c = 0; c < P.n_cost; ++c) tg = tg + P.coeff[c] * gradient_of<KMAX>(P, c, a, L, pr, a.vel);
-			acc = -1.0f * tg;
+		if constexpr (CLS == CLS_ORCA) {
+			Line lines_priv[kLinesInLds ? 1 : KMAX];
+			Line proj_priv[kLinesInLds ? 1 : KMAX];
+			pr.lines = kLinesInLds ? Lines{s_lines + tid, BS} : Lines{lines_priv, 1};
+			const Lines proj = kLinesInLds ? Lines{s_proj + tid, BS} : Lines{proj_priv, 1};
+			pr.n_lines = orca_lines(P.params[0], P.range[0], a, L, pr.lines);
+			const V2 best = orca_solve(pr.lines, pr.n_lines, a, proj);
+			const V2 acc = clamp_vec((best - a.vel) / smax(a.relaxation, a.dt), a.max_accel);
+			next = clamp_vec(a.vel + acc * a.dt, a.max_speed);
 		} else {
-			V2 best;
-			if (P.mode == MODE_GLOBAL_ORCA) best = orca_solve(pr.lines, pr.n_lines, a, proj);
-			else if (P.mode == MODE_GLOBAL_FORCE) best = pr.target[0];
-			else best = sample_best<KMAX>(P, a, L, pr, rng);
-			acc = (best - a.vel) / smax(a.relaxation, a.dt);
+			Line lines_priv[CLS == CLS_GENERIC ? KMAX : 1];
+			Line proj_priv[CLS == CLS_GENERIC ? KMAX : 1];
+			pr.lines = Lines{lines_priv, 1};
+			for_costs<CLS>(P, [&](int c) {
+				const int kind = P.kind[c];
+				if (CLS != CLS_GENERIC || is_force_based(kind)) pr.target[c] = target_velocity(P, c, a, L);
+				else if (kind == KIND_ORCA) pr.n_lines = orca_lines(P.params[c], P.range[c], a, L, pr.lines);
+				else if (kind == KIND_KARAMOUZAS)
+					pr.ttcp[c] = time_to_first_collision(L, a.pos, a.pref, a.radius, P.range[c], true);
+			});
+			V2 acc;
+			if (CLS == CLS_FORCE_GRAD || (CLS == CLS_GENERIC && P.mode == MODE_GRADIENT)) {
+				if (CLS == CLS_FORCE_GRAD && P.mode == MODE_GLOBAL_FORCE) {
+					acc = (pr.target[0] - a.vel) / smax(a.relaxation, a.dt);
+				} else {
+					V2 tg{0.0f, 0.0f};
+					for_costs<CLS>(P, [&](int c) { tg = tg + P.coeff[c] * gradient_of<KMAX, CLS>(P, c, a, L, pr, a.vel); });
+					acc = -1.0f * tg;
+				}
+			} else {
+				V2 best;
+				if (CLS == CLS_GENERIC && P.mode == MODE_GLOBAL_ORCA)
+					best = orca_solve(pr.lines, pr.n_lines, a, Lines{proj_priv, 1});
+				else if (CLS == CLS_GENERIC && P.mode == MODE_GLOBAL_FORCE) best = pr.target[0];
+				else best = sample_best<KMAX, CLS>(P, a, L, pr, rng);
+				acc = (best - a.vel) / smax(a.relaxation, a.dt);
+			}
+			acc = clamp_vec(acc, a.max_accel);
+			next = clamp_vec(a.vel + acc * a.dt, a.max_speed);
 		}
-		acc = clamp_vec(acc, a.max_accel);
-		next = clamp_vec(a.vel + acc * a.dt, a.max_speed);
 	}
 
 	// Policy::ComputeContactForces (policy.cpp:107-141), agents only
@@ -504,6 +554,27 @@ __global__ __launch_bounds__(BS) void k_step(StepArgs A) {
 	A.out.pref[k] = f2(a.pref);
 	meta.w = (int)rng;
 	A.out.meta[k] = meta;
+}
+
+// Bucket the sorted agents by policy class: list[offset[c] + i] = sorted index, in
+// wave order (one atomic per wave and class).
+__global__ __launch_bounds__(256) void k_bucket(const int4* meta, const DevPolicy* pol, int n,
+                                               const int* cls_offset, unsigned* cursor, unsigned* list) {
+	const int k = blockIdx.x * blockDim.x + threadIdx.x;
+	const int c = (k < n) ? pol[meta[k].y & kPolicyMask].cls : -1;
+	const int lane = threadIdx.x & 63;
+	for (int q = 0; q < NUM_CLS; ++q) {
+		const unsigned long long m = __ballot(c == q);
+		if (!m) continue;
+		const int leader = __ffsll((long long)m) - 1;
+		unsigned base = 0;
+		if (lane == leader) base = atomicAdd(&cursor[q], (unsigned)__popcll(m));
+		base = __shfl(base, leader);
+		if (c == q) {
+			const unsigned before = (unsigned)__popcll(m & ((1ull << lane) - 1ull));
+			list[cls_offset[q] + base + before] = (unsigned)k;
+		}
+	}
 }
 
 } // namespace umans

@@ -41,6 +41,7 @@ struct DevPolicy {
 	float query_range;       // Policy::getInteractionRange, policy.cpp:34-41
 	int n_cost;
 	int orca_cost;           // index of the ORCA cost or -1
+	int cls;                 // kernel class (CLS_* in kernels.h)
 	int kind[4];
 	float coeff[4];          // effective coefficients (1 for a single-cost global fallback)
 	float range[4];
@@ -87,7 +88,8 @@ struct StepArgs {
 	const WorldInfo* worlds;
 	const GridInfo* grids;
 	const unsigned* cell_start;   // total cells + 1
-	int n;
+	const unsigned* list;         // sorted indices of this launch's agents (null: 0..n-1)
+	int n;                        // agents of this launch
 	int kcap;                // neighbour cap (<= KMAX of the kernel instance)
 	// optional neighbour record (null when not recording)
 	int* rec_count;
