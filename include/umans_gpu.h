@@ -211,8 +211,31 @@ int umans_cost_id(const char* name);                           /* -1 if unknown 
 const char* umans_cost_name(int kind);
 int umans_cost_param_index(int kind, const char* param);       /* -1 if unknown */
 const char* umans_cost_param_name(int kind, int index);        /* null past the end */
+const char* umans_cost_param_read_name(int kind, int q);      /* q-th attribute in read order */
 int umans_cost_defaults(int kind, float* range, float params[UMANS_MAX_PARAMS]);
 int umans_policy_init(umans_policy* p, int method);            /* reference defaults */
+
+/* Scenario files (host only; no GPU needed). Reads the reference's XML schema
+ * (CrowdSimulator::FromConfigFile, crowdSimulator.cpp:516-647: configPath redirection,
+ * World, delta_time, end_time, Policies / Agents / Obstacles inline or via file=) into
+ * packed policies (index = order in the file; umans_scenario_get also returns each
+ * policy's XML id) and agent arrays whose policy field indexes the packed policies.
+ * Agent IDs are assigned as WorldBase::AddAgent assigns them. */
+typedef struct {
+	int32_t world_type;
+	float width, height;
+	float delta_time;
+	float end_time;            /* -1 when absent */
+	int32_t n_agents, n_policies, n_obstacles, n_warnings;
+} umans_scenario_info;
+typedef struct umans_scenario umans_scenario;
+int umans_scenario_load(const char* path, umans_scenario** out, char* err, int err_len);
+void umans_scenario_free(umans_scenario* sc);
+int umans_scenario_get_info(const umans_scenario* sc, umans_scenario_info* info);
+const char* umans_scenario_warning(const umans_scenario* sc, int i);
+int umans_scenario_get(const umans_scenario* sc, umans_policy* policies, int32_t* policy_xml_ids,
+                       umans_agents* agents, float* start_time, int32_t* remove_at_goal);
+int umans_scenario_obstacles(const umans_scenario* sc, int32_t* counts, float* points);
 
 #ifdef __cplusplus
 }

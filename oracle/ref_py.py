@@ -115,6 +115,18 @@ class RefSim:
             lib().ref_set_velocities(self.h, n, _p(vel, _F), _p(vd, _F))
         self.n = n
 
+    @classmethod
+    def from_file(cls, path, threads=1):
+        """A reference simulator loaded by CrowdSimulator::FromConfigFile from `path`."""
+        self = cls.__new__(cls)
+        self.h = lib().ref_load(str(path).encode())
+        if not self.h:
+            raise RuntimeError(f"reference failed to load {path}")
+        lib().ref_set_threads(self.h, int(threads))
+        self._keep = []
+        self.n = lib().ref_num_agents(self.h)
+        return self
+
     def close(self):
         if self.h:
             lib().ref_destroy(self.h)

@@ -87,7 +87,37 @@ def main():
     data["world"] = np.array([0, 0, 0, sc.dt], np.float64)
     np.savez_compressed(os.path.join(HERE, "circle50_orca.npz"), **data)
     written += 1
+    written += examples()
     print(f"wrote {written} fixtures to {HERE}")
+
+
+EXAMPLE_FILES = ["Circle25-Dutra.xml", "Circle25-Karamouzas.xml", "Circle25-Moussaid.xml", "Circle25-ORCA.xml",
+                 "Circle25-Paris.xml", "Circle25-PowerLaw.xml", "Circle25-RVO.xml", "Circle25-SocialForces.xml",
+                 "Circle25-VanToll.xml", "Toric.xml"]
+EXAMPLE_STEPS = 10
+
+
+def examples():
+    """Copy a few of the reference's example scenarios (XML inputs, with the policy and
+    agent files they name) under tests/golden/examples/ and record the reference's state
+    and neighbour lists over their first steps, loaded by its own FromConfigFile."""
+    import re
+    import shutil
+    src = "/root/reference/examples"
+    dst = os.path.join(HERE, "examples")
+    os.makedirs(os.path.join(dst, "policies"), exist_ok=True)
+    os.makedirs(os.path.join(dst, "agents"), exist_ok=True)
+    n = 0
+    for f in EXAMPLE_FILES:
+        shutil.copy(os.path.join(src, f), os.path.join(dst, f))
+        for ref in re.findall(r'file="([^"]+)"', open(os.path.join(src, f)).read()):
+            shutil.copy(os.path.join(src, ref), os.path.join(dst, ref))
+        sim = R.RefSim.from_file(os.path.join(dst, f))
+        data = record(sim, EXAMPLE_STEPS)
+        sim.close()
+        np.savez_compressed(os.path.join(HERE, "example_" + f[:-4] + ".npz"), **data)
+        n += 1
+    return n
 
 
 if __name__ == "__main__":

@@ -11,7 +11,21 @@ GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
 
 
 def fixture_names():
-    return sorted(os.path.basename(p)[:-4] for p in glob.glob(os.path.join(GOLDEN, "*.npz")))
+    return sorted(os.path.basename(p)[:-4] for p in glob.glob(os.path.join(GOLDEN, "*.npz"))
+                  if not os.path.basename(p).startswith("example_"))
+
+
+def example_names():
+    return sorted(os.path.basename(p)[len("example_"):-4] for p in glob.glob(os.path.join(GOLDEN, "example_*.npz")))
+
+
+def load_example(name):
+    """(reference record, Scenario loaded from the XML by this library's loader)."""
+    from umans_amd import gpu
+    with np.load(os.path.join(GOLDEN, "example_" + name + ".npz"), allow_pickle=False) as z:
+        d = {k: z[k] for k in z.files}
+    sc, meta = gpu.load_scenario(os.path.join(GOLDEN, "examples", name + ".xml"))
+    return d, sc
 
 
 def load(name):

@@ -14,7 +14,7 @@ import numpy as np
 import pytest
 
 import oracle_py as O
-from goldens import fixture_names, load, neighbor_lists, state_at
+from goldens import example_names, fixture_names, load, load_example, neighbor_lists, state_at
 from gpu_util import REL_TOL, gpu_step, rel_err
 from umans_amd import scenarios as S
 
@@ -101,3 +101,25 @@ def test_mixed_crowd_100k_one_step():
     ok = (e64 <= REL_TOL) | (e32 <= 1e-6)
     assert ok.mean() == 1.0, f"{(~ok).sum()} agents off: e64 {np.sort(e64)[-5:]}"
     np.testing.assert_array_equal(out["id"], a["id"])
+
+
+@pytest.mark.parametrize("name", example_names())
+def test_gpu_on_reference_examples(name):
+    """Example scenarios of the reference read by the XML loader and stepped on the GPU
+    from the reference's state each step: neighbour sets exact (order may differ only
+    among equal distances), velocities to the parity bar."""
+    from test_oracle_golden import tie_equal
+    d, sc = load_example(name)
+    a = dict(sc.agents)
+    for s in range(d["nb_count"].shape[0]):
+        for k in ("pos", "vel", "viewdir"):
+            a[k] = d["state_" + k][s]
+        out, (cnt, ids, d2, seg), st = gpu_step(sc.world_type, sc.width, sc.height, sc.dt, sc.policies, a)
+        a["rng"] = out["rng"]
+        for i, (rid, _, _) in enumerate(neighbor_lists(d, s)):
+            assert tie_equal(rid, ids[i, :cnt[i]], d2[i]), (name, s, i)
+        o64 = O.step(O.make_world(sc.world_type, sc.width, sc.height, sc.dt, 0), sc.policies, a, precision=1)
+        e64 = rel_err(out["vel"], o64["vel64"])
+        e32 = rel_err(out["vel"], d["state_vel"][s + 1])
+        ok = (e64 <= REL_TOL) | (e32 <= 1e-6)
+        assert ok.all(), (name, s, np.where(~ok)[0][:5])

@@ -5,7 +5,7 @@ import numpy as np
 import pytest
 
 import oracle_py as O
-from goldens import fixture_names, load, neighbor_lists, state_at
+from goldens import example_names, fixture_names, load, load_example, neighbor_lists, state_at
 
 NAMES = fixture_names()
 
@@ -46,3 +46,31 @@ def test_oracle_circle_trajectory():
             np.testing.assert_array_equal(a["pos"], d["traj_pos"][k])
             k += 1
     assert k == len(d["traj_steps"])
+
+
+def tie_equal(ref_ids, ids, d2):
+    """The same neighbours in an order that differs at most among equal distances."""
+    if np.array_equal(ref_ids, ids):
+        return True
+    if sorted(ref_ids.tolist()) != sorted(ids.tolist()):
+        return False
+    at = {j: k for k, j in enumerate(ids)}
+    return all(d2[at[j]] == d2[k] for k, j in enumerate(ref_ids))
+
+
+@pytest.mark.parametrize("name", example_names())
+def test_oracle_on_reference_examples(name):
+    """The reference's example scenarios, loaded by this library's XML loader: the oracle
+    reproduces the reference's steps (started from the reference's state each step)."""
+    d, sc = load_example(name)
+    wd = O.make_world(sc.world_type, sc.width, sc.height, sc.dt, 0)
+    a = dict(sc.agents)
+    for s in range(d["nb_count"].shape[0]):
+        for k in ("pos", "vel", "viewdir"):
+            a[k] = d["state_" + k][s]
+        out = O.step(wd, sc.policies, a, precision=0, list_stride=128)
+        a["rng"] = out["rng"]
+        for i, (rid, _, _) in enumerate(neighbor_lists(d, s)):
+            assert tie_equal(rid, out["nb_id"][i, :out["nb_count"][i]], out["nb_d2"][i])
+        np.testing.assert_allclose(out["vel"], d["state_vel"][s + 1], rtol=1e-6, atol=1e-6)
+        np.testing.assert_allclose(out["pos"], d["state_pos"][s + 1], rtol=1e-6, atol=1e-6)

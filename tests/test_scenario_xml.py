@@ -1,0 +1,133 @@
+"""The host scenario loader against the reference's own loader.
+
+For every obstacle-free scenario of the reference's examples/ (read from the reference
+checkout where it exists; skipped elsewhere): the agents our loader produces (IDs,
+positions, goals, viewing directions) equal the reference's after
+CrowdSimulator::FromConfigFile, and two DoSteps of the reference equal two oracle steps
+run with our loaded policies, bit for bit — which pins names, parameters, defaults and
+read order of the registry end to end. Error paths follow crowdSimulator.cpp."""
+import glob
+import os
+
+import numpy as np
+import pytest
+
+import oracle_py as O
+import ref_py as R
+from umans_amd import gpu
+
+EXAMPLES = "/root/reference/examples"
+FILES = sorted(glob.glob(os.path.join(EXAMPLES, "*.xml")))
+needs_ref = pytest.mark.skipif(not (FILES and R.available()), reason="reference checkout or build not present")
+
+
+def _quiet(fn):
+    """Run fn with the C-level stdout/stderr silenced (the reference logs its world)."""
+    import sys
+    sys.stdout.flush(); sys.stderr.flush()
+    saved = [os.dup(1), os.dup(2)]
+    dn = os.open(os.devnull, os.O_WRONLY)
+    os.dup2(dn, 1); os.dup2(dn, 2)
+    try:
+        return fn()
+    finally:
+        os.dup2(saved[0], 1); os.dup2(saved[1], 2)
+        for f in saved + [dn]:
+            os.close(f)
+
+
+@needs_ref
+@pytest.mark.parametrize("path", FILES, ids=[os.path.basename(f) for f in FILES])
+def test_loader_matches_reference(path):
+    try:
+        sc, meta = gpu.load_scenario(path)
+    except gpu.UmansError as e:
+        if "not supported by this backend" in str(e):
+            pytest.skip(str(e))
+        raise
+    if len(meta["obstacle_counts"]):
+        pytest.skip("scenario has obstacles (not on the GPU path)")
+    ref = _quiet(lambda: R.RefSim.from_file(path))
+    st = ref.state()
+    np.testing.assert_array_equal(sc.agents["id"], st["id"])
+    np.testing.assert_array_equal(sc.agents["pos"], st["pos"])
+    np.testing.assert_array_equal(sc.agents["goal"], st["goal"])
+    np.testing.assert_array_equal(sc.agents["viewdir"], st["viewdir"])
+    a = dict(sc.agents)
+    wd = O.make_world(sc.world_type, sc.width, sc.height, sc.dt, 0)
+    for _ in range(2):
+        s1 = ref.state()               # each step starts from the reference's state
+        a["pos"], a["vel"], a["viewdir"] = s1["pos"], s1["vel"], s1["viewdir"]
+        ref.step(1)
+        out = O.step(wd, sc.policies, a, precision=0, list_stride=128)
+        # neighbour sets are equal; the order may differ only among equal distances,
+        # which nanoflann leaves in traversal order and this build orders by agent ID
+        for i, (rid, _, rdsq) in enumerate(ref.neighbors()):
+            oid = out["nb_id"][i, :out["nb_count"][i]]
+            assert sorted(rid.tolist()) == sorted(oid.tolist())
+            if not np.array_equal(rid, oid):
+                d2 = out["nb_d2"][i, :len(oid)]
+                order = {j: k for k, j in enumerate(oid)}
+                assert all(d2[order[j]] == d2[k] for k, j in enumerate(rid)), "order differs beyond ties"
+        a["rng"] = out["rng"]
+        s2 = ref.state()
+        # bit-exact except for sums over tied neighbours taken in a different order
+        np.testing.assert_allclose(out["vel"], s2["vel"], rtol=1e-6, atol=1e-6)
+        np.testing.assert_allclose(out["pos"], s2["pos"], rtol=1e-6, atol=1e-6)
+    ref.close()
+
+
+def _write(tmp_path, text, name="s.xml"):
+    p = tmp_path / name
+    p.write_text(text)
+    return str(p)
+
+
+POL = '<Policies><Policy id="3" OptimizationMethod="gradient"><costfunction name="GoalReachingForce"/></Policy></Policies>'
+
+
+@pytest.mark.parametrize("body,msg", [
+    ("<Foo/>", "No main Simulation"),
+    ('<Simulation delta_time="0.1"></Simulation>', "No World"),
+    ('<Simulation delta_time="0.1"><World type="Toric" width="-1" height="5"/>' + POL + "</Simulation>", "toric"),
+    ('<Simulation><World type="Infinite"/>' + POL + "</Simulation>", "delta_time"),
+    ('<Simulation delta_time="0.1"><World type="Infinite"/></Simulation>', "at least one valid Policy"),
+    ('<Simulation delta_time="0.1"><World/><Policies><Policy id="0" OptimizationMethod="newton">'
+     '<costfunction name="ORCA"/></Policy></Policies></Simulation>', "optimization method"),
+    ('<Simulation delta_time="0.1"><World/><Policies><Policy id="0" OptimizationMethod="global"/></Policies>'
+     "</Simulation>", "at least one cost function"),
+    ('<Simulation delta_time="0.1"><World/>' + POL.replace("</Policies>", POL[10:-11] + "</Policies>")
+     + "</Simulation>", "already taken"),
+    ('<Simulation delta_time="0.1"><World/>' + POL + '<Agents><Agent><Policy id="3"/></Agent></Agents></Simulation>',
+     "position"),
+    ('<Simulation delta_time="0.1"><World/>' + POL + '<Agents><Agent><pos x="0" y="0"/><Policy id="9"/></Agent>'
+     "</Agents></Simulation>", "doesn't exist"),
+])
+def test_loader_errors(tmp_path, body, msg):
+    with pytest.raises(gpu.UmansError, match=msg):
+        gpu.load_scenario(_write(tmp_path, body))
+
+
+def test_loader_ids_files_and_unknown_cost(tmp_path):
+    (tmp_path / "p.xml").write_text('<Policies><Policy id="7" OptimizationMethod="sampling" SpeedSamples="3" '
+                                    'AngleSamples="5" SamplingType="bogus"><costfunction name="Nope"/>'
+                                    '<costfunction name="RVO" w="2.5" coeff="0.5" range="3"/></Policy></Policies>')
+    (tmp_path / "a.xml").write_text('<Agents><Agent id="5"><pos x="1" y="2"/><goal x="1" y="2"/><Policy id="7"/></Agent>'
+                                    '<Agent id="5"><pos x="3" y="4"/><Policy id="7"/></Agent>'
+                                    '<Agent><pos x="5" y="6"/><goal x="8" y="10"/><Policy id="7"/></Agent></Agents>')
+    path = _write(tmp_path, '<Simulation delta_time="0.05" end_time="3"><World type="Toric" width="30" height="20"/>'
+                  '<Policies file="p.xml"/><Agents file="a.xml"/></Simulation>')
+    (tmp_path / "redir.xml").write_text('<configPath path="s.xml"/>')
+    sc, meta = gpu.load_scenario(str(tmp_path / "redir.xml"))
+    assert sc.world_type == 1 and sc.width == 30.0 and sc.height == 20.0
+    assert sc.dt == pytest.approx(0.05) and meta["end_time"] == pytest.approx(3.0)
+    assert meta["policy_xml_ids"] == [7]
+    p = sc.policies[0]
+    assert p["n_cost"] == 1 and p["cost_kind"][0] == 5 and p["cost_params"][0][0] == 2.5
+    assert p["cost_coeff"][0] == 0.5 and p["cost_range"][0] == 3.0
+    assert p["s_speed_samples"] == 3 and p["s_angle_samples"] == 5 and p["s_type"] == 0
+    # WorldBase::AddAgent: a taken ID falls back to the next unused one
+    np.testing.assert_array_equal(sc.agents["id"], [5, 6, 7])
+    np.testing.assert_array_equal(sc.agents["goal"][1], [3, 4])          # no goal element: stays put
+    np.testing.assert_allclose(sc.agents["viewdir"][2], [0.6, 0.8], rtol=1e-6)
+    assert any("Nope" in w for w in meta["warnings"]) and any("sampling type" in w for w in meta["warnings"])

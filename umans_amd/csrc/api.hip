@@ -141,10 +141,10 @@ struct umans_ctx {
 	// policy classes: one step-kernel instance per class over a bucket of agents
 	std::vector<int> h_policy;           // policy of each uploaded agent
 	int cls_count[NUM_CLS] = {0, 0, 0, 0};
-	int cls_off[NUM_CLS] = {0, 0, 0, 0};
-	int* d_cls_off = nullptr;
-	unsigned* d_cursor = nullptr;
-	unsigned* d_list = nullptr;
+	int cls_off[NUM_CLS] = {0, 0, 0, 0};    // first sorted index of each class
+	int cls_slot[NUM_CLS] = {0, 0, 0, 0};   // slot of each present class in the sort key
+	int nslots = 1;
+	int* d_cls_slot = nullptr;
 	float cell_factor = 1.0f;    // cell width = factor x max interaction range (autotuned)
 	int block = 0;               // step-kernel block size (0 = default; autotuned)
 
@@ -216,8 +216,7 @@ struct umans_ctx {
 		drop_graph();
 		free_adjoint();
 		X.release(); Y.release(); B.release();
-		hipFree(cell_of); hipFree(rank); hipFree(d_list);
-		d_list = nullptr;
+		hipFree(cell_of); hipFree(rank);
 		hipFree(rec_count); hipFree(rec_id); hipFree(rec_d2); hipFree(rec_seg);
 		cell_of = rank = nullptr;
 		rec_count = rec_id = rec_seg = nullptr;
@@ -388,11 +387,12 @@ int setup_worlds(umans_ctx* c, const std::vector<int>& per_world) {
 		if (off > (1LL << 30)) return c->fail(UMANS_EINVAL, "grid too large");
 	}
 	c->total_cells = (int)off;
-	if (c->total_cells + 1 > c->cell_alloc) {
+	if ((long long)NUM_CLS * off + 1 > (1LL << 31) - 1) return c->fail(UMANS_EINVAL, "grid too large");
+	if (NUM_CLS * c->total_cells + 1 > c->cell_alloc) {
 		hipFree(c->cell_count); hipFree(c->cell_start); hipFree(c->scan_sums);
 		c->cell_count = c->cell_start = c->scan_sums = nullptr;
 		c->cell_alloc = 0;
-		const int cells = c->total_cells + 1;
+		const int cells = NUM_CLS * c->total_cells + 1;
 		int rc;
 		if ((rc = c->hip(dalloc(&c->cell_count, cells), "alloc grid"))) return rc;
 		if ((rc = c->hip(dalloc(&c->cell_start, cells), "alloc grid"))) return rc;
@@ -403,14 +403,21 @@ int setup_worlds(umans_ctx* c, const std::vector<int>& per_world) {
 	              "world upload");
 }
 
-// Agents per policy class from the uploaded policies (host copy), and the bucket offsets.
+// Agents per policy class from the uploaded policies (host copy); the classes present
+// get consecutive slots of the sort key, so class q occupies sorted indices
+// [cls_off[q], cls_off[q] + cls_count[q]).
 int update_classes(umans_ctx* c) {
 	for (int q = 0; q < NUM_CLS; ++q) c->cls_count[q] = 0;
 	for (int p : c->h_policy) c->cls_count[c->hpol[p].cls]++;
-	int off = 0;
-	for (int q = 0; q < NUM_CLS; ++q) { c->cls_off[q] = off; off += c->cls_count[q]; }
-	return c->hip(hipMemcpyAsync(c->d_cls_off, c->cls_off, sizeof(c->cls_off), hipMemcpyHostToDevice, c->stream),
-	              "class offsets");
+	int off = 0, slot = 0;
+	for (int q = 0; q < NUM_CLS; ++q) {
+		c->cls_off[q] = off;
+		off += c->cls_count[q];
+		c->cls_slot[q] = c->cls_count[q] > 0 ? slot++ : 0;
+	}
+	c->nslots = std::max(slot, 1);
+	return c->hip(hipMemcpyAsync(c->d_cls_slot, c->cls_slot, sizeof(c->cls_slot), hipMemcpyHostToDevice, c->stream),
+	              "class slots");
 }
 
 int enqueue_step(umans_ctx* c, bool events, const State* ysave = nullptr, unsigned* raw_nb = nullptr,
@@ -419,7 +426,7 @@ int enqueue_step(umans_ctx* c, bool events, const State* ysave = nullptr, unsign
 	const State& Ys = ysave ? *ysave : c->Y.s;
 	hipStream_t st = c->stream;
 	const int nw = (int)c->worlds.size();
-	const int scan_n = c->total_cells + 1;
+	const int scan_n = c->nslots * c->total_cells + 1;
 
 	hipEvent_t e0 = nullptr, e1 = nullptr, e2 = nullptr;
 	if (events) {
@@ -439,7 +446,7 @@ int enqueue_step(umans_ctx* c, bool events, const State* ysave = nullptr, unsign
 	}
 	hipLaunchKernelGGL(k_grid_setup, dim3(ceil_div(nw, 64)), dim3(64), 0, st, c->dworlds, c->dbbox, c->dgrids, nw);
 	hipLaunchKernelGGL(k_count, dim3(ceil_div(n, 256)), dim3(256), 0, st, c->X.s.pos, c->X.s.meta, n, c->dworlds,
-	                   c->dgrids, c->cell_count, c->cell_of, c->rank);
+	                   c->dgrids, c->dpol, c->d_cls_slot, c->total_cells, c->cell_count, c->cell_of, c->rank);
 	const int nb = ceil_div(scan_n, kScanTile);
 	hipLaunchKernelGGL(k_scan_local, dim3(nb), dim3(kScanThreads), 0, st, c->cell_count, c->cell_start, c->scan_sums, scan_n);
 	hipLaunchKernelGGL(k_scan_sums, dim3(1), dim3(kScanThreads), 0, st, c->scan_sums, nb);
@@ -454,6 +461,9 @@ int enqueue_step(umans_ctx* c, bool events, const State* ysave = nullptr, unsign
 	a.worlds = c->dworlds;
 	a.grids = c->dgrids;
 	a.cell_start = c->cell_start;
+	a.nslots = c->nslots;
+	a.slot_cells = c->total_cells;
+	a.base = 0;
 	a.n = n;
 	a.kcap = c->kcap;
 	if (c->record && c->rec_count) {
@@ -468,25 +478,13 @@ int enqueue_step(umans_ctx* c, bool events, const State* ysave = nullptr, unsign
 	a.stats = c->dstats;
 	const int kmax = kmax_bucket(c->kcap);
 	const int bs = c->block > 0 ? c->block : default_block(kmax);
-	int nclasses = 0, only = CLS_GENERIC;
-	for (int q = 0; q < NUM_CLS; ++q)
-		if (c->cls_count[q] > 0) { ++nclasses; only = q; }
-	if (nclasses > 1) {
-		hipLaunchKernelGGL(k_fill, dim3(1), dim3(64), 0, st, c->d_cursor, NUM_CLS, 0u);
-		hipLaunchKernelGGL(k_bucket, dim3(ceil_div(n, 256)), dim3(256), 0, st, Ys.meta, c->dpol, n, c->d_cls_off,
-		                   c->d_cursor, c->d_list);
-	}
 	if (e1) hipEventRecord(e1, st);
-	if (nclasses <= 1) {
-		launch_step(only, kmax, class_block(only, kmax, bs), a, st);
-	} else {
-		for (int q = 0; q < NUM_CLS; ++q) {
-			if (!c->cls_count[q]) continue;
-			StepArgs aq = a;
-			aq.list = c->d_list + c->cls_off[q];
-			aq.n = c->cls_count[q];
-			launch_step(q, kmax, class_block(q, kmax, bs), aq, st);
-		}
+	for (int q = 0; q < NUM_CLS; ++q) {
+		if (!c->cls_count[q]) continue;
+		StepArgs aq = a;
+		aq.base = c->cls_off[q];
+		aq.n = c->cls_count[q];
+		launch_step(q, kmax, class_block(q, kmax, bs), aq, st);
 	}
 	if (e2) hipEventRecord(e2, st);
 	return c->hip(hipGetLastError(), "step launch");
@@ -573,8 +571,7 @@ int create_impl(const umans_world_desc* descs, int nworlds, umans_ctx** out) {
 	if (hipSetDevice(c->device) != hipSuccess || hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess ||
 	    dalloc(&c->dgrids, nworlds) != hipSuccess || dalloc(&c->dworlds, nworlds) != hipSuccess ||
 	    dalloc(&c->dbbox, 4 * nworlds) != hipSuccess || dalloc(&c->dstats, 2) != hipSuccess ||
-	    dalloc(&c->dpol, kMaxPolicies) != hipSuccess || dalloc(&c->d_cls_off, NUM_CLS) != hipSuccess ||
-	    dalloc(&c->d_cursor, NUM_CLS) != hipSuccess) {
+	    dalloc(&c->dpol, kMaxPolicies) != hipSuccess || dalloc(&c->d_cls_slot, NUM_CLS) != hipSuccess) {
 		umans_destroy(c);
 		return UMANS_EHIP;
 	}
@@ -600,7 +597,7 @@ void umans_destroy(umans_ctx* c) {
 	if (c->stream) hipStreamSynchronize(c->stream);
 	c->free_agents();
 	hipFree(c->dgrids); hipFree(c->dworlds); hipFree(c->dbbox); hipFree(c->dstats); hipFree(c->dpol);
-	hipFree(c->d_cls_off); hipFree(c->d_cursor);
+	hipFree(c->d_cls_slot);
 	hipFree(c->cell_count); hipFree(c->cell_start); hipFree(c->scan_sums);
 	c->free_events();
 	if (c->stream) hipStreamDestroy(c->stream);
@@ -684,7 +681,6 @@ int umans_upload(umans_ctx* c, const umans_agents* a, int n) {
 		if ((rc = c->hip(c->Y.alloc(n), "alloc state"))) return rc;
 		if ((rc = c->hip(dalloc(&c->cell_of, n), "alloc grid"))) return rc;
 		if ((rc = c->hip(dalloc(&c->rank, n), "alloc grid"))) return rc;
-		if ((rc = c->hip(dalloc(&c->d_list, n), "alloc buckets"))) return rc;
 		c->cap = n;
 	}
 	c->n = n;

@@ -125,15 +125,21 @@ UD int bin_axis(float x, float o, float inv, int nc) {
 	return (int)t;
 }
 
+// Sort key of an agent: (class slot, world, cell). Agents of one policy class end up
+// contiguous, each class in cell order, so every class kernel reads a dense range.
 __global__ __launch_bounds__(256) void k_count(const float2* pos, const int4* meta, int n, const WorldInfo* worlds,
-                                              const GridInfo* grids, unsigned* cell_count, unsigned* cell_of,
+                                              const GridInfo* grids, const DevPolicy* pol, const int* cls_slot,
+                                              int slot_cells, unsigned* cell_count, unsigned* cell_of,
                                               unsigned* rank) {
 	const int k = blockIdx.x * blockDim.x + threadIdx.x;
 	if (k >= n) return;
-	const int w = meta[k].y >> kPolicyBits;
+	const int4 m = meta[k];
+	const int w = m.y >> kPolicyBits;
 	const GridInfo g = grids[w];
 	const float2 p = pos[k];
-	const int c = worlds[w].cell_offset + bin_axis(p.y, g.oy, g.invy, g.ncy) * g.ncx + bin_axis(p.x, g.ox, g.invx, g.ncx);
+	const int slot = cls_slot[pol[m.y & kPolicyMask].cls];
+	const int c = slot * slot_cells + worlds[w].cell_offset + bin_axis(p.y, g.oy, g.invy, g.ncy) * g.ncx +
+	              bin_axis(p.x, g.ox, g.invx, g.ncx);
 	cell_of[k] = (unsigned)c;
 	rank[k] = atomicAdd(&cell_count[c], 1u);
 }
@@ -257,8 +263,9 @@ UD int find_neighbors(const StepArgs& A, const WorldInfo& wi, const GridInfo& g,
 		const int seg_base = filled;
 		const int cap = kcap - filled;
 		int seg_cnt = 0;
+		for (int q = 0; q < A.nslots * A.slot_cells; q += A.slot_cells)
 		for (int cy = cy0; cy <= cy1; ++cy) {
-			const int row = wi.cell_offset + cy * g.ncx;
+			const int row = q + wi.cell_offset + cy * g.ncx;
 			const unsigned b = A.cell_start[row + cx0], e = A.cell_start[row + cx1 + 1];
 			for (unsigned j = b; j < e; ++j) {
 				if ((int)j == self) continue;
@@ -411,21 +418,21 @@ UD V2 sample_best(const DevPolicy& P, const Self& a, const NbList& L, const Prep
 }
 
 // ------------------------------------------------------------------ the step
-// One lane per agent of the class's bucket (A.list) or of the whole population.
+// One lane per agent of the class's sorted range [A.base, A.base + A.n).
 template <int KMAX, int BS, int CLS>
 __global__ __launch_bounds__(BS) void k_step(StepArgs A) {
 	__shared__ double s_d2[KMAX * BS];
 	__shared__ unsigned s_nb[KMAX * BS];
-	// ORCA half-planes in LDS when they fit beside the neighbour lists (2 x 16 KB at
-	// KMAX x block = 1024); larger instances keep them in private memory.
+	// ORCA half-planes in LDS when they fit beside the neighbour lists (16 KB at
+	// KMAX x block = 1024); larger instances keep them in private memory. The projected
+	// lines of linearProgram3 (only for infeasible programs) stay in private memory.
 	constexpr bool kLinesInLds = (CLS == CLS_ORCA) && (KMAX * BS <= 1024);
 	constexpr int kLdsLines = kLinesInLds ? KMAX * BS : 1;
 	__shared__ Line s_lines[kLdsLines];
-	__shared__ Line s_proj[kLdsLines];
 	const int tid = threadIdx.x;
 	const int t = blockIdx.x * BS + tid;
 	const bool active = t < A.n;
-	const int k = active ? (A.list ? (int)A.list[t] : t) : 0;
+	const int k = active ? A.base + t : 0;
 	int total = 0, cnt = 0;
 	const State& S = A.in;
 	int4 meta = make_int4(0, 0, 0, 0);
@@ -484,9 +491,9 @@ __global__ __launch_bounds__(BS) void k_step(StepArgs A) {
 		pr.n_lines = 0;
 		if constexpr (CLS == CLS_ORCA) {
 			Line lines_priv[kLinesInLds ? 1 : KMAX];
-			Line proj_priv[kLinesInLds ? 1 : KMAX];
+			Line proj_priv[KMAX];
 			pr.lines = kLinesInLds ? Lines{s_lines + tid, BS} : Lines{lines_priv, 1};
-			const Lines proj = kLinesInLds ? Lines{s_proj + tid, BS} : Lines{proj_priv, 1};
+			const Lines proj{proj_priv, 1};
 			pr.n_lines = orca_lines(P.params[0], P.range[0], a, L, pr.lines);
 			const V2 best = orca_solve(pr.lines, pr.n_lines, a, proj);
 			const V2 acc = clamp_vec((best - a.vel) / smax(a.relaxation, a.dt), a.max_accel);
@@ -554,27 +561,6 @@ __global__ __launch_bounds__(BS) void k_step(StepArgs A) {
 	A.out.pref[k] = f2(a.pref);
 	meta.w = (int)rng;
 	A.out.meta[k] = meta;
-}
-
-// Bucket the sorted agents by policy class: list[offset[c] + i] = sorted index, in
-// wave order (one atomic per wave and class).
-__global__ __launch_bounds__(256) void k_bucket(const int4* meta, const DevPolicy* pol, int n,
-                                               const int* cls_offset, unsigned* cursor, unsigned* list) {
-	const int k = blockIdx.x * blockDim.x + threadIdx.x;
-	const int c = (k < n) ? pol[meta[k].y & kPolicyMask].cls : -1;
-	const int lane = threadIdx.x & 63;
-	for (int q = 0; q < NUM_CLS; ++q) {
-		const unsigned long long m = __ballot(c == q);
-		if (!m) continue;
-		const int leader = __ffsll((long long)m) - 1;
-		unsigned base = 0;
-		if (lane == leader) base = atomicAdd(&cursor[q], (unsigned)__popcll(m));
-		base = __shfl(base, leader);
-		if (c == q) {
-			const unsigned before = (unsigned)__popcll(m & ((1ull << lane) - 1ull));
-			list[cls_offset[q] + base + before] = (unsigned)k;
-		}
-	}
 }
 
 } // namespace umans

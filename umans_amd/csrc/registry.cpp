@@ -72,6 +72,13 @@ extern "C" const char* umans_cost_param_name(int kind, int index) {
 	return nullptr;
 }
 
+// Attribute names in the order the cost function's parseParameters reads them.
+extern "C" const char* umans_cost_param_read_name(int kind, int q) {
+	const CostDef* c = by_kind(kind);
+	if (!c || q < 0 || q >= c->nparam) return nullptr;
+	return c->params[q].name;
+}
+
 extern "C" int umans_cost_defaults(int kind, float* range, float params[UMANS_MAX_PARAMS]) {
 	const CostDef* c = by_kind(kind);
 	if (!c) return UMANS_EINVAL;

@@ -87,8 +87,10 @@ struct StepArgs {
 	const DevPolicy* pol;
 	const WorldInfo* worlds;
 	const GridInfo* grids;
-	const unsigned* cell_start;   // total cells + 1
-	const unsigned* list;         // sorted indices of this launch's agents (null: 0..n-1)
+	const unsigned* cell_start;   // nslots * slot_cells + 1
+	int nslots;                   // policy classes present; agents are sorted by (class, cell)
+	int slot_cells;               // cells per class slot (all worlds)
+	int base;                     // first sorted index of this launch's agents
 	int n;                        // agents of this launch
 	int kcap;                // neighbour cap (<= KMAX of the kernel instance)
 	// optional neighbour record (null when not recording)

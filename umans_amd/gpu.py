@@ -25,6 +25,8 @@ EXPORTS = [
     "umans_cost_param_index", "umans_cost_param_name", "umans_cost_defaults", "umans_policy_init",
     "umans_set_profiling", "umans_get_profile", "umans_create_batch", "umans_num_worlds",
     "umans_set_graph", "umans_autotune", "umans_set_tuning", "umans_adjoint_forward", "umans_adjoint_backward",
+    "umans_cost_param_read_name", "umans_scenario_load", "umans_scenario_free", "umans_scenario_get_info",
+    "umans_scenario_warning", "umans_scenario_get", "umans_scenario_obstacles",
 ]
 
 
@@ -65,6 +67,12 @@ class Tuning(C.Structure):
 
 class AdjointDesc(C.Structure):
     _fields_ = [("nsteps", C.c_int32), ("checkpoint_every", C.c_int32), ("param_kind", C.c_int32)]
+
+
+class ScenarioInfo(C.Structure):
+    _fields_ = [("world_type", C.c_int32), ("width", C.c_float), ("height", C.c_float), ("delta_time", C.c_float),
+                ("end_time", C.c_float), ("n_agents", C.c_int32), ("n_policies", C.c_int32),
+                ("n_obstacles", C.c_int32), ("n_warnings", C.c_int32)]
 
 
 class UmansError(RuntimeError):
@@ -111,6 +119,16 @@ def lib():
         L.umans_set_tuning.argtypes = [vp, C.POINTER(Tuning)]
         L.umans_adjoint_forward.argtypes = [vp, C.POINTER(AdjointDesc), _F, C.POINTER(C.c_double)]
         L.umans_adjoint_backward.argtypes = [vp, _F, _F, _F]
+        L.umans_cost_param_read_name.argtypes = [C.c_int, C.c_int]
+        L.umans_cost_param_read_name.restype = C.c_char_p
+        L.umans_scenario_load.argtypes = [C.c_char_p, C.POINTER(vp), C.c_char_p, C.c_int]
+        L.umans_scenario_free.argtypes = [vp]
+        L.umans_scenario_free.restype = None
+        L.umans_scenario_get_info.argtypes = [vp, C.POINTER(ScenarioInfo)]
+        L.umans_scenario_warning.argtypes = [vp, C.c_int]
+        L.umans_scenario_warning.restype = C.c_char_p
+        L.umans_scenario_get.argtypes = [vp, C.POINTER(PolicyC), _I, C.POINTER(AgentsC), _F, _I]
+        L.umans_scenario_obstacles.argtypes = [vp, _I, _F]
         L.umans_set_profiling.argtypes = [vp, C.c_int]
         L.umans_get_profile.argtypes = [vp, C.POINTER(Profile)]
         _lib = L
@@ -284,3 +302,59 @@ def world_from_scenario(sc, cell_size=0.0, device=0):
     w = World(sc.world_type, sc.width, sc.height, sc.dt, sc.policies, sc.max_neighbors, cell_size, device)
     w.upload(sc.agents)
     return w
+
+
+def packed_from_struct(q):
+    """Packed-policy dict (umans_amd.registry layout) from a umans_policy struct."""
+    d = {k: getattr(q, k) for k in ("method", "s_type", "s_base", "s_base_dir", "s_radius", "s_angle",
+                                    "s_speed_samples", "s_angle_samples", "s_random_samples", "s_include_base",
+                                    "relaxation_time", "contact_force_scale", "n_cost")}
+    d["cost_kind"] = list(q.cost_kind)
+    d["cost_coeff"] = list(q.cost_coeff)
+    d["cost_range"] = list(q.cost_range)
+    d["cost_params"] = [list(q.cost_params[c]) for c in range(4)]
+    return d
+
+
+def load_scenario(path):
+    """Read a scenario file of the reference's XML schema (CrowdSimulator::FromConfigFile)
+    with the library's host loader. Returns (Scenario, info dict with end_time, policy XML
+    ids, obstacles, warnings, start_time and remove_at_goal per agent)."""
+    from .scenarios import Scenario
+    L = lib()
+    h = C.c_void_p()
+    err = C.create_string_buffer(512)
+    rc = L.umans_scenario_load(str(path).encode(), C.byref(h), err, 512)
+    if rc != 0:
+        raise UmansError(f"cannot load {path}: {err.value.decode()}")
+    try:
+        info = ScenarioInfo()
+        L.umans_scenario_get_info(h, C.byref(info))
+        n, npol = info.n_agents, info.n_policies
+        pols = (PolicyC * max(npol, 1))()
+        xml_ids = np.zeros(max(npol, 1), np.int32)
+        a = {"id": np.zeros(n, np.int32), "pos": np.zeros((n, 2), np.float32), "vel": np.zeros((n, 2), np.float32),
+             "goal": np.zeros((n, 2), np.float32), "viewdir": np.zeros((n, 2), np.float32),
+             "radius": np.zeros(n, np.float32), "pref_speed": np.zeros(n, np.float32),
+             "max_speed": np.zeros(n, np.float32), "max_accel": np.zeros(n, np.float32),
+             "policy": np.zeros(n, np.int32), "rng": np.zeros(n, np.uint32)}
+        st = np.zeros(n, np.float32)
+        rag = np.zeros(n, np.int32)
+        ac = AgentsC(_p(a["id"], _I), _p(a["pos"], _F), _p(a["vel"], _F), _p(a["goal"], _F), _p(a["viewdir"], _F),
+                     _p(a["radius"], _F), _p(a["pref_speed"], _F), _p(a["max_speed"], _F), _p(a["max_accel"], _F),
+                     _p(a["policy"], _I), _p(a["rng"], _U), None, None)
+        L.umans_scenario_get(h, pols, _p(xml_ids, _I), C.byref(ac), _p(st, _F), _p(rag, _I))
+        counts = np.zeros(max(info.n_obstacles, 1), np.int32)
+        L.umans_scenario_obstacles(h, _p(counts, _I), None)
+        pts = np.zeros((int(counts[:info.n_obstacles].sum()), 2), np.float32)
+        L.umans_scenario_obstacles(h, None, _p(pts, _F))
+        warnings = [L.umans_scenario_warning(h, i).decode() for i in range(info.n_warnings)]
+    finally:
+        L.umans_scenario_free(h)
+    packed = [packed_from_struct(pols[i]) for i in range(npol)]
+    sc = Scenario(str(path), info.world_type, info.width, info.height, info.delta_time, 0, packed, a)
+    meta = {"end_time": info.end_time, "policy_xml_ids": xml_ids[:npol].tolist(), "start_time": st,
+            "remove_at_goal": rag, "obstacle_counts": counts[:info.n_obstacles], "obstacle_points": pts,
+            "warnings": warnings}
+    sc.meta.update(meta)
+    return sc, meta
