@@ -59,7 +59,7 @@ struct SFPair {
 	V2 R, V, R2, Eb;
 	float magR, magR2, vb, q, S, P, b, E, dB2, dm, g, pref_mag;
 };
-UD bool sf_forward(const float* prm, const Self& a, const Nb& o, V2 o_pref, SFPair& s) {
+UD bool sf_forward(const float* prm, const Self& a, const Nb& o, V2 o_pref, double view_cos, SFPair& s) {
 	const float sdt = prm[1], V0 = prm[2], sigma = prm[3];
 	s.R = a.pos - o.pos;
 	s.magR = mag(s.R);
@@ -80,8 +80,7 @@ UD bool sf_forward(const float* prm, const Self& a, const Nb& o, V2 o_pref, SFPa
 	s.dm = (s.magR - s.P / s.magR) / s.magR2;
 	s.dB2 = 2.0f * s.S * (1.0f + s.dm);
 	s.E = expf(-s.b / sigma);
-	const float half = (float)(100.0 * kPI / 180.0);
-	const float scale = (angle_between(a.vel, -s.R) < half ? 1.0f : 0.5f);
+	const float scale = in_view(a.vel, -s.R, view_cos) ? 1.0f : 0.5f;
 	s.g = V0 / sigma * s.E * (s.dB2 / (8.0f * s.b)) * scale;   // F = g R
 	return true;
 }
@@ -244,7 +243,7 @@ __global__ __launch_bounds__(128) void k_step_adjoint(AdjArgs A) {
 					if (!(o.dsq <= rangeSq)) continue;
 					if (kind == KIND_SOCIAL_FORCES) {
 						SFPair sp;
-						if (sf_forward(P.params[c], a, o, L.pref_of(o.j), sp)) Fs = Fs + sp.g * sp.R;
+						if (sf_forward(P.params[c], a, o, L.pref_of(o.j), P.view_cos, sp)) Fs = Fs + sp.g * sp.R;
 					} else if (kind == KIND_POWER_LAW) {
 						PLPair pp;
 						if (pl_forward(P.params[c], a, o, pp)) Fs = Fs + pp.comp1 * pp.comp2;
@@ -277,7 +276,7 @@ __global__ __launch_bounds__(128) void k_step_adjoint(AdjArgs A) {
 				const unsigned rj = row_of(S, o.j);
 				if (kind == KIND_SOCIAL_FORCES) {
 					SFPair sp;
-					if (!sf_forward(P.params[c], a, o, L.pref_of(o.j), sp)) continue;
+					if (!sf_forward(P.params[c], a, o, L.pref_of(o.j), P.view_cos, sp)) continue;
 					V2 Rbar, vjbar, prefjbar;
 					float V0b, sgb;
 					V2 vj = o.vel;

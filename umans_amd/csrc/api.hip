@@ -257,6 +257,19 @@ struct umans_ctx {
 
 namespace {
 
+// The largest double c whose angle, as vector2D.h:177-190 computes it ((float)acos(c)),
+// is not below (float)(100 deg): (float)acos(x) < 100 deg  <=>  x > view_cos.
+double view_cos_threshold() {
+	const float half = (float)(100.0 * 3.1415926535897 / 180.0);
+	double lo = -1.0, hi = 1.0;            // (float)acos(lo) >= half > (float)acos(hi)
+	while (std::nextafter(lo, hi) < hi) {
+		const double mid = 0.5 * (lo + hi);
+		const double m = (mid <= lo || mid >= hi) ? std::nextafter(lo, hi) : mid;
+		if ((float)std::acos(m) >= half) lo = m; else hi = m;
+	}
+	return lo;
+}
+
 // Host-side resolution of a packed policy into what the kernels read
 // (Policy::ComputeNewVelocity and getBestVelocityGlobal, policy.cpp:43-100).
 int make_dev_policy(const umans_policy& p, DevPolicy& d, std::string& err) {
@@ -271,6 +284,8 @@ int make_dev_policy(const umans_policy& p, DevPolicy& d, std::string& err) {
 	}
 	d.n_cost = p.n_cost;
 	d.orca_cost = -1;
+	static const double kViewCos = view_cos_threshold();
+	d.view_cos = kViewCos;
 	float qr = 0.0f;
 	for (int c = 0; c < p.n_cost; ++c) {
 		if (p.cost_kind[c] < 0 || p.cost_kind[c] >= UMANS_COST_KIND_COUNT) {

@@ -125,8 +125,20 @@ UD V2 power_law_pair(const float* prm, const Self& a, const Nb& o) {
 	return comp1 * comp2;
 }
 
+// angle(a, b) < (float)(100 deg) as the reference evaluates it (vector2D.h:177-190, a
+// double acos cast to float), decided by comparing the cosine with view_cos, the
+// largest double whose float-rounded acos is still >= 100 degrees (computed on the host
+// with the same libm): no transcendental on the device, and the reference's rounding.
+UD bool in_view(V2 a, V2 b, double view_cos) {
+	const double lengths = (double)(mag(a) * mag(b));
+	if (lengths == 0.0) return true;
+	const double frac = (double)dot(a, b) / lengths;
+	if (frac < -1.0 || frac > 1.0) return true;
+	return frac > view_cos;
+}
+
 // SocialForcesAvoidance::ComputeAgentInteractionForce (SocialForcesAvoidance.cpp:26-110)
-UD V2 social_force_pair(const float* prm, const Self& a, const Nb& o, V2 o_pref) {
+UD V2 social_force_pair(const float* prm, const Self& a, const Nb& o, V2 o_pref, double view_cos) {
 	const float sdt = prm[1], V0 = prm[2], sigma = prm[3];
 	const V2 R = a.pos - o.pos;
 	const float magR = mag(R);
@@ -147,8 +159,7 @@ UD V2 social_force_pair(const float* prm, const Self& a, const Nb& o, V2 o_pref)
 	const float d_b = (1.0f / (8.0f * b)) * d_bSq2;
 	const float d_V = -V0 / sigma * expf(-b / sigma) * d_b;
 	const V2 F = (-R) * d_V;
-	const float half = (float)(100.0 * kPI / 180.0);
-	const float scale = (angle_between(a.vel, -R) < half ? 1.0f : 0.5f);
+	const float scale = in_view(a.vel, -R, view_cos) ? 1.0f : 0.5f;
 	return scale * F;
 }
 
@@ -165,7 +176,7 @@ UD V2 force_of(const DevPolicy& P, int c, const Self& a, const NbList& L) {
 		const Nb n = L.get(e);
 		if (n.dsq <= rangeSq) {
 			const V2 f = (kind == KIND_POWER_LAW) ? power_law_pair(P.params[c], a, n)
-			                                      : social_force_pair(P.params[c], a, n, L.pref_of(n.j));
+			                                      : social_force_pair(P.params[c], a, n, L.pref_of(n.j), P.view_cos);
 			F = F + f;
 		}
 	}

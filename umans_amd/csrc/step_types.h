@@ -42,6 +42,7 @@ struct DevPolicy {
 	int n_cost;
 	int orca_cost;           // index of the ORCA cost or -1
 	int cls;                 // kernel class (CLS_* in kernels.h)
+	double view_cos;         // SocialForces view test: (float)acos(c) < 100 deg  <=>  c > view_cos
 	int kind[4];
 	float coeff[4];          // effective coefficients (1 for a single-cost global fallback)
 	float range[4];

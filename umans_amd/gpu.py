@@ -194,6 +194,23 @@ class World:
         except Exception:
             pass
 
+    @classmethod
+    def from_scenario_file(cls, path, max_neighbors=0, device=0):
+        """A world loaded from a scenario file of the reference's XML schema, agents
+        uploaded. Obstacles, delayed insertion (start_time > 0) and remove_at_goal are
+        not on the GPU path: such scenarios are refused."""
+        sc, meta = load_scenario(path)
+        if len(meta["obstacle_counts"]):
+            raise UmansError(f"{path}: obstacles are not supported on the GPU path")
+        if (meta["start_time"] > 0).any():
+            raise UmansError(f"{path}: agents with a start_time are not supported on the GPU path")
+        if meta["remove_at_goal"].any():
+            raise UmansError(f"{path}: remove_at_goal is not supported on the GPU path")
+        w = cls(sc.world_type, sc.width, sc.height, sc.dt, sc.policies, max_neighbors, device=device)
+        w.upload(sc.agents)
+        w.scenario = sc
+        return w
+
     def upload(self, agents):
         n = len(agents["id"])
         keep = {}
