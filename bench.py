@@ -177,6 +177,8 @@ def main():
     ap.add_argument("--cpu-threads", type=int, default=min(16, os.cpu_count() or 1))
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-autotune", action="store_true")
+    ap.add_argument("--tuning", default="", help="CELL_FACTOR,BLOCK: fix the launch tuning instead of "
+                    "autotuning (profiling runs use the tuning the timed run chose)")
     ap.add_argument("--no-extras", action="store_true",
                     help="skip the batched-sweep and adjoint legs added to the default line")
     ap.add_argument("--graph", action="store_true", help="replay a captured hipGraph per step "
@@ -203,7 +205,12 @@ def main():
     else:
         w = gpu.World(sc.world_type, sc.width, sc.height, sc.dt, sc.policies, sc.max_neighbors)
     w.upload(sc.agents)
-    tuning = None if args.no_autotune else w.autotune(trial_steps=3)
+    if args.tuning:
+        cf, bs = args.tuning.split(",")
+        w.set_tuning(float(cf), int(bs))
+        tuning = {"cell_factor": float(cf), "block_size": int(bs), "fixed": True}
+    else:
+        tuning = None if args.no_autotune else w.autotune(trial_steps=3)
     if args.graph:
         w.set_graph(True)
     w.step(args.warmup)

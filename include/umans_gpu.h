@@ -161,6 +161,17 @@ typedef struct {
 int umans_autotune(umans_ctx* ctx, int trial_steps, umans_tuning* out);
 int umans_set_tuning(umans_ctx* ctx, const umans_tuning* tuning);
 
+/* CSV trajectory output: CrowdSimulator::StartCSVOutput (crowdSimulator.cpp:37-51) and
+ * CSVWriter (csvwriter.cpp:37-87). After every step the positions are copied
+ * asynchronously to pinned host memory and logged per agent; a flush appends rows
+ * "t,x,y" (default stream formatting) to <dir>/output_<id>.csv in ID order. With
+ * flush_immediately every step is flushed. A new upload flushes and stops the output.
+ * Single-world contexts only (UMANS_EINVAL for a batch). The directory is created,
+ * parents included. */
+int umans_csv_start(umans_ctx* ctx, const char* dirname, int flush_immediately);
+int umans_csv_flush(umans_ctx* ctx);
+int umans_csv_stop(umans_ctx* ctx);
+
 /* Differentiable rollout (no counterpart in the reference). Starting from the current
  * state, umans_adjoint_forward runs nsteps steps, keeping a checkpoint of the state
  * every checkpoint_every steps, and returns the loss

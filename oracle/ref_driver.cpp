@@ -82,6 +82,12 @@ void ref_set_velocities(void* sim, int n, const float* vel, const float* viewdir
 			Vector2D(viewdir[2 * i], viewdir[2 * i + 1]));
 }
 
+// CrowdSimulator::StartCSVOutput (crowdSimulator.cpp:37-51); flushed when destroyed.
+void ref_start_csv(void* sim, const char* dir, int flush_immediately)
+{
+	static_cast<CrowdSimulator*>(sim)->StartCSVOutput(dir, flush_immediately != 0);
+}
+
 void ref_step(void* sim, int nsteps)
 {
 	static_cast<CrowdSimulator*>(sim)->RunSimulationSteps(nsteps);

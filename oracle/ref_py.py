@@ -39,6 +39,7 @@ def lib():
         L.ref_add_agents.restype = C.c_int
         L.ref_set_velocities.argtypes = [C.c_void_p, C.c_int, _F, _F]
         L.ref_step.argtypes = [C.c_void_p, C.c_int]
+        L.ref_start_csv.argtypes = [C.c_void_p, C.c_char_p, C.c_int]
         L.ref_time_steps.argtypes = [C.c_void_p, C.c_int]; L.ref_time_steps.restype = C.c_double
         L.ref_get_state.argtypes = [C.c_void_p, _I, _F, _F, _F, _F, _F]
         L.ref_total_neighbors.argtypes = [C.c_void_p]; L.ref_total_neighbors.restype = C.c_longlong
@@ -133,6 +134,9 @@ class RefSim:
             self.h = None
 
     __del__ = close
+
+    def start_csv(self, dirname, flush_immediately=False):
+        lib().ref_start_csv(self.h, str(dirname).encode(), 1 if flush_immediately else 0)
 
     def step(self, nsteps=1):
         lib().ref_step(self.h, int(nsteps))

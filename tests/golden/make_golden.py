@@ -88,6 +88,7 @@ def main():
     np.savez_compressed(os.path.join(HERE, "circle50_orca.npz"), **data)
     written += 1
     written += examples()
+    written += csv_output()
     print(f"wrote {written} fixtures to {HERE}")
 
 
@@ -118,6 +119,34 @@ def examples():
         np.savez_compressed(os.path.join(HERE, "example_" + f[:-4] + ".npz"), **data)
         n += 1
     return n
+
+
+CSV_EXAMPLE = "Circle25-ORCA.xml"
+CSV_STEPS = 10
+
+
+def csv_output():
+    """Run the reference's CSV writer (CrowdSimulator::StartCSVOutput, flushed when the
+    simulator is destroyed) over the first steps of an example and keep every agent's
+    output_<id>.csv text, with the states after each step."""
+    import tempfile
+    path = os.path.join(HERE, "examples", CSV_EXAMPLE)
+    with tempfile.TemporaryDirectory() as tmp:
+        out = os.path.join(tmp, "csv")
+        os.makedirs(out)      # CreateDirectoryIfNonExistent cannot create absolute paths
+        sim = R.RefSim.from_file(path)
+        sim.start_csv(out, False)
+        states = [sim.state()]
+        for _ in range(CSV_STEPS):
+            sim.step(1)
+            states.append(sim.state())
+        sim.close()
+        ids = states[0]["id"]
+        text = np.array([open(os.path.join(out, f"output_{i}.csv")).read() for i in ids])
+    np.savez_compressed(os.path.join(HERE, "csv_" + CSV_EXAMPLE[:-4] + ".npz"), id=ids, text=text,
+                        state_pos=np.stack([s["pos"] for s in states]).astype(np.float32),
+                        state_id=np.stack([s["id"] for s in states]).astype(np.int32))
+    return 1
 
 
 if __name__ == "__main__":

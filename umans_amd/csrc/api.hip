@@ -3,10 +3,13 @@
 #include "umans_gpu.h"
 #include "adjoint.h"
 
+#include <sys/stat.h>
+
 #include <algorithm>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <fstream>
 #include <string>
 #include <vector>
 
@@ -53,6 +56,7 @@ hipError_t copy_state(const State& dst, const State& src, int n, hipStream_t st)
 }
 
 int kmax_bucket(int kcap) {
+	if (kcap <= 12) return 12;
 	if (kcap <= 16) return 16;
 	if (kcap <= 32) return 32;
 	if (kcap <= 64) return 64;
@@ -63,7 +67,7 @@ int kmax_bucket(int kcap) {
 // entries of 12 B).
 bool block_supported(int kmax, int bs) {
 	switch (kmax) {
-	case 16: return bs == 64 || bs == 128 || bs == 256;
+	case 12: case 16: return bs == 64 || bs == 128 || bs == 256;
 	case 32: return bs == 64 || bs == 128;
 	default: return bs == 64;
 	}
@@ -79,6 +83,11 @@ void launch_step_t(const StepArgs& a, hipStream_t st) {
 template <int CLS>
 void launch_step_c(int kmax, int bs, const StepArgs& a, hipStream_t st) {
 	switch (kmax) {
+	case 12:
+		if (bs == 64) launch_step_t<12, 64, CLS>(a, st);
+		else if (bs == 256) launch_step_t<12, 256, CLS>(a, st);
+		else launch_step_t<12, 128, CLS>(a, st);
+		break;
 	case 16:
 		if (bs == 64) launch_step_t<16, 64, CLS>(a, st);
 		else if (bs == 256) launch_step_t<16, 256, CLS>(a, st);
@@ -116,6 +125,12 @@ struct umans_ctx {
 	std::vector<umans_world_desc> worlds;   // one per world of the batch
 	int device = 0;
 	hipStream_t stream = nullptr;
+	// the step kernels of the policy classes are independent (disjoint agent ranges, read-only
+	// neighbour state): each class runs on a stream of its own, forked from and joined back
+	// into `stream`, so the classes share the CUs instead of running one after another
+	hipStream_t cls_stream[NUM_CLS] = {};
+	hipEvent_t fork_ev = nullptr;
+	hipEvent_t join_ev[NUM_CLS] = {};
 	std::string err;
 
 	std::vector<DevPolicy> hpol;
@@ -178,6 +193,20 @@ struct umans_ctx {
 	double* dloss = nullptr;
 	double last_loss = 0.0;
 
+	// CSV trajectory output (CrowdSimulator::StartCSVOutput + CSVWriter): positions are
+	// copied after each step into one of two pinned slots and logged per agent row
+	struct CsvEntry { double t; float x, y; };
+	bool csv_on = false, csv_flush_now = false;
+	std::string csv_dir;
+	std::vector<std::vector<CsvEntry>> csv_log;   // per upload row
+	std::vector<int32_t> csv_ids;                 // agent ID per upload row
+	float2* csv_pos[2] = {nullptr, nullptr};
+	int4* csv_meta[2] = {nullptr, nullptr};
+	hipEvent_t csv_ev[2] = {nullptr, nullptr};
+	double csv_t[2] = {0.0, 0.0};
+	bool csv_pending[2] = {false, false};
+	int csv_slot = 0, csv_cap = 0;
+
 	// per-phase timers: one event triple per profiled step, read back lazily
 	int profiling = 0;
 	std::vector<hipEvent_t> ev;
@@ -211,6 +240,16 @@ struct umans_ctx {
 		adj_params = nullptr; dloss = nullptr;
 		adj_ready = false;
 		adj_steps = adj_ckpt = adj_n = 0;
+	}
+	void free_csv_buffers() {
+		for (int i = 0; i < 2; ++i) {
+			if (csv_pos[i]) hipHostFree(csv_pos[i]);
+			if (csv_meta[i]) hipHostFree(csv_meta[i]);
+			if (csv_ev[i]) hipEventDestroy(csv_ev[i]);
+			csv_pos[i] = nullptr; csv_meta[i] = nullptr; csv_ev[i] = nullptr;
+			csv_pending[i] = false;
+		}
+		csv_cap = 0;
 	}
 	void free_agents() {
 		drop_graph();
@@ -435,6 +474,64 @@ int update_classes(umans_ctx* c) {
 	              "class slots");
 }
 
+// ------------------------------------------------------------------ CSV output
+int csv_take(umans_ctx* c, int slot) {
+	if (!c->csv_pending[slot]) return UMANS_OK;
+	int rc = c->hip(hipEventSynchronize(c->csv_ev[slot]), "csv");
+	if (rc) return rc;
+	const float2* p = c->csv_pos[slot];
+	const int4* m = c->csv_meta[slot];
+	for (int k = 0; k < c->n; ++k) {
+		const int row = m[k].z;
+		c->csv_log[row].push_back({c->csv_t[slot], p[k].x, p[k].y});
+	}
+	c->csv_pending[slot] = false;
+	return UMANS_OK;
+}
+
+// After a step the older pending copy is in csv_slot (the slot written next), the newer
+// one in csv_slot ^ 1: log them in that order.
+int csv_drain(umans_ctx* c) {
+	int rc = csv_take(c, c->csv_slot);
+	if (!rc) rc = csv_take(c, c->csv_slot ^ 1);
+	return rc;
+}
+
+// CSVWriter::Flush (csvwriter.cpp:50-72): one file output_<id>.csv per agent, in ID
+// order, rows "t,x,y" in the default stream format, appended.
+int csv_flush(umans_ctx* c) {
+	int rc = csv_drain(c);
+	if (rc) return rc;
+	std::vector<int> order(c->csv_ids.size());
+	for (size_t i = 0; i < order.size(); ++i) order[i] = (int)i;
+	std::sort(order.begin(), order.end(), [&](int a, int b) { return c->csv_ids[a] < c->csv_ids[b]; });
+	for (int row : order) {
+		auto& log = c->csv_log[row];
+		if (log.empty()) continue;
+		std::ofstream f(c->csv_dir + "output_" + std::to_string(c->csv_ids[row]) + ".csv", std::ios::app);
+		for (const auto& e : log) f << e.t << "," << e.x << "," << e.y << "\n";
+		log.clear();
+	}
+	return UMANS_OK;
+}
+
+// After a step: copy the positions (and rows) into the next pinned slot.
+int csv_after_step(umans_ctx* c) {
+	const int slot = c->csv_slot;
+	int rc = csv_take(c, slot);              // the copy two steps ago must be logged first
+	if (rc) return rc;
+	if ((rc = c->hip(hipMemcpyAsync(c->csv_pos[slot], c->X.s.pos, c->n * sizeof(float2), hipMemcpyDeviceToHost,
+	                                c->stream), "csv copy"))) return rc;
+	if ((rc = c->hip(hipMemcpyAsync(c->csv_meta[slot], c->X.s.meta, c->n * sizeof(int4), hipMemcpyDeviceToHost,
+	                                c->stream), "csv copy"))) return rc;
+	if ((rc = c->hip(hipEventRecord(c->csv_ev[slot], c->stream), "csv event"))) return rc;
+	c->csv_t[slot] = c->sim_time;
+	c->csv_pending[slot] = true;
+	c->csv_slot ^= 1;
+	if (c->csv_flush_now) return csv_flush(c);
+	return UMANS_OK;
+}
+
 int enqueue_step(umans_ctx* c, bool events, const State* ysave = nullptr, unsigned* raw_nb = nullptr,
                  int* raw_count = nullptr) {
 	const int n = c->n;
@@ -494,12 +591,31 @@ int enqueue_step(umans_ctx* c, bool events, const State* ysave = nullptr, unsign
 	const int kmax = kmax_bucket(c->kcap);
 	const int bs = c->block > 0 ? c->block : default_block(kmax);
 	if (e1) hipEventRecord(e1, st);
+	int active = 0;
+	for (int q = 0; q < NUM_CLS; ++q) active += c->cls_count[q] > 0;
+	if (active > 1) hipEventRecord(c->fork_ev, st);
+	bool first = true;
 	for (int q = 0; q < NUM_CLS; ++q) {
 		if (!c->cls_count[q]) continue;
 		StepArgs aq = a;
 		aq.base = c->cls_off[q];
 		aq.n = c->cls_count[q];
-		launch_step(q, kmax, class_block(q, kmax, bs), aq, st);
+		hipStream_t sq = st;
+		if (!first) {
+			sq = c->cls_stream[q];
+			hipStreamWaitEvent(sq, c->fork_ev, 0);
+		}
+		launch_step(q, kmax, class_block(q, kmax, bs), aq, sq);
+		if (!first) hipEventRecord(c->join_ev[q], sq);
+		first = false;
+	}
+	if (active > 1) {
+		first = true;
+		for (int q = 0; q < NUM_CLS; ++q) {
+			if (!c->cls_count[q]) continue;
+			if (!first) hipStreamWaitEvent(st, c->join_ev[q], 0);
+			first = false;
+		}
 	}
 	if (e2) hipEventRecord(e2, st);
 	return c->hip(hipGetLastError(), "step launch");
@@ -534,6 +650,10 @@ int run_steps(umans_ctx* c, int nsteps) {
 		}
 		c->steps += 1;
 		c->sim_time += (double)c->worlds[0].delta_time;   // WorldBase::time_ (worldBase.cpp:122)
+		if (c->csv_on && c->n > 0) {
+			const int rc = csv_after_step(c);
+			if (rc) return rc;
+		}
 	}
 	return UMANS_OK;
 }
@@ -583,7 +703,12 @@ int create_impl(const umans_world_desc* descs, int nworlds, umans_ctx** out) {
 	c->device = descs[0].device;
 	c->kcap = kcap;
 	c->block = std::max(descs[0].block_size, 0);
-	if (hipSetDevice(c->device) != hipSuccess || hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess ||
+	bool streams_ok = hipSetDevice(c->device) == hipSuccess &&
+	                  hipEventCreateWithFlags(&c->fork_ev, hipEventDisableTiming) == hipSuccess;
+	for (int q = 0; q < NUM_CLS && streams_ok; ++q)
+		streams_ok = hipStreamCreateWithFlags(&c->cls_stream[q], hipStreamNonBlocking) == hipSuccess &&
+		             hipEventCreateWithFlags(&c->join_ev[q], hipEventDisableTiming) == hipSuccess;
+	if (!streams_ok || hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess ||
 	    dalloc(&c->dgrids, nworlds) != hipSuccess || dalloc(&c->dworlds, nworlds) != hipSuccess ||
 	    dalloc(&c->dbbox, 4 * nworlds) != hipSuccess || dalloc(&c->dstats, 2) != hipSuccess ||
 	    dalloc(&c->dpol, kMaxPolicies) != hipSuccess || dalloc(&c->d_cls_slot, NUM_CLS) != hipSuccess) {
@@ -610,11 +735,18 @@ void umans_destroy(umans_ctx* c) {
 	if (!c) return;
 	hipSetDevice(c->device);
 	if (c->stream) hipStreamSynchronize(c->stream);
+	if (c->csv_on) csv_flush(c);
+	c->free_csv_buffers();
 	c->free_agents();
 	hipFree(c->dgrids); hipFree(c->dworlds); hipFree(c->dbbox); hipFree(c->dstats); hipFree(c->dpol);
 	hipFree(c->d_cls_slot);
 	hipFree(c->cell_count); hipFree(c->cell_start); hipFree(c->scan_sums);
 	c->free_events();
+	for (int q = 0; q < NUM_CLS; ++q) {
+		if (c->cls_stream[q]) hipStreamDestroy(c->cls_stream[q]);
+		if (c->join_ev[q]) hipEventDestroy(c->join_ev[q]);
+	}
+	if (c->fork_ev) hipEventDestroy(c->fork_ev);
 	if (c->stream) hipStreamDestroy(c->stream);
 	delete c;
 }
@@ -688,6 +820,12 @@ int umans_upload(umans_ctx* c, const umans_agents* a, int n) {
 		meta[i] = make_int4(a->id[i], a->policy[i] | (w << kPolicyBits), i, (int)r);
 	}
 	hipSetDevice(c->device);
+	if (c->csv_on) {                                  // the logged rows belong to the old agent set
+		const int rc0 = csv_flush(c);
+		c->csv_on = false;
+		c->csv_log.clear();
+		if (rc0) return rc0;
+	}
 	c->drop_graph();
 	if (n > c->cap) {
 		c->free_agents();
@@ -832,6 +970,62 @@ int umans_set_tuning(umans_ctx* c, const umans_tuning* t) {
 		if ((rc = setup_worlds(c, per_world))) return rc;
 		rc = c->hip(hipStreamSynchronize(c->stream), "tuning");
 	}
+	return rc;
+}
+
+int umans_csv_start(umans_ctx* c, const char* dirname, int flush_immediately) {
+	if (!c || !dirname || !*dirname) return UMANS_EINVAL;
+	if (c->n == 0) return c->fail(UMANS_ESTATE, "upload agents before starting CSV output");
+	if (c->worlds.size() != 1) return c->fail(UMANS_EINVAL, "CSV output is per world: use a single-world context");
+	if (c->csv_on) {
+		hipSetDevice(c->device);
+		const int rc0 = csv_flush(c);
+		if (rc0) return rc0;
+	}
+	std::string dir = dirname;
+	if (dir.back() != '/') dir += '/';                // CSVWriter::SetOutputDirectory (csvwriter.cpp:37-48)
+	for (size_t i = 1; i <= dir.size(); ++i)
+		if (i == dir.size() || dir[i] == '/') mkdir(dir.substr(0, i).c_str(), 0755);
+	struct stat sb;
+	if (stat(dir.c_str(), &sb) != 0 || !S_ISDIR(sb.st_mode))
+		return c->fail(UMANS_EINVAL, "could not set CSV output directory to " + dir);
+	hipSetDevice(c->device);
+	int rc = c->hip(hipStreamSynchronize(c->stream), "csv");
+	if (rc) return rc;
+	if (c->csv_cap < c->n) {
+		c->free_csv_buffers();
+		for (int i = 0; i < 2; ++i) {
+			if ((rc = c->hip(hipHostMalloc((void**)&c->csv_pos[i], c->n * sizeof(float2)), "csv pinned"))) return rc;
+			if ((rc = c->hip(hipHostMalloc((void**)&c->csv_meta[i], c->n * sizeof(int4)), "csv pinned"))) return rc;
+			if ((rc = c->hip(hipEventCreate(&c->csv_ev[i]), "csv event"))) return rc;
+		}
+		c->csv_cap = c->n;
+	}
+	std::vector<int4> meta(c->n);
+	if ((rc = c->hip(hipMemcpy(meta.data(), c->X.s.meta, c->n * sizeof(int4), hipMemcpyDeviceToHost), "csv ids"))) return rc;
+	c->csv_ids.assign(c->n, 0);
+	for (const int4& m : meta) c->csv_ids[m.z] = m.x;
+	c->csv_log.assign(c->n, {});
+	c->csv_dir = dir;
+	c->csv_flush_now = flush_immediately != 0;
+	c->csv_on = true;
+	return UMANS_OK;
+}
+
+int umans_csv_flush(umans_ctx* c) {
+	if (!c) return UMANS_EINVAL;
+	if (!c->csv_on) return UMANS_OK;
+	hipSetDevice(c->device);
+	return csv_flush(c);
+}
+
+int umans_csv_stop(umans_ctx* c) {
+	if (!c) return UMANS_EINVAL;
+	if (!c->csv_on) return UMANS_OK;
+	hipSetDevice(c->device);
+	const int rc = csv_flush(c);
+	c->csv_on = false;
+	c->csv_log.clear();
 	return rc;
 }
 

@@ -240,7 +240,12 @@ template <int KMAX, int BS>
 UD int find_neighbors(const StepArgs& A, const WorldInfo& wi, const GridInfo& g, int self, V2 p, float range,
                       double* d2s, unsigned* nbs, int& total) {
 	const State& S = A.in;
-	const double r2 = (double)(range * range);
+	const float r2f = range * range;
+	const double r2 = (double)r2f;
+	// float pre-filter: the float d2 of a candidate is within 5 ulp of its double d2, so
+	// one beyond r2 * (1 + 1e-5) is certainly outside; the exact double test decides
+	// every other candidate
+	const float r2_reject = r2f * 1.00001f;
 	const float rs = range * 1.0001f + 1e-3f;
 	const int kcap = A.kcap;
 	int filled = 0;
@@ -263,13 +268,15 @@ UD int find_neighbors(const StepArgs& A, const WorldInfo& wi, const GridInfo& g,
 		const int seg_base = filled;
 		const int cap = kcap - filled;
 		int seg_cnt = 0;
-		for (int q = 0; q < A.nslots * A.slot_cells; q += A.slot_cells)
+		for (int slot0 = 0; slot0 < A.nslots * A.slot_cells; slot0 += A.slot_cells)
 		for (int cy = cy0; cy <= cy1; ++cy) {
-			const int row = q + wi.cell_offset + cy * g.ncx;
+			const int row = slot0 + wi.cell_offset + cy * g.ncx;
 			const unsigned b = A.cell_start[row + cx0], e = A.cell_start[row + cx1 + 1];
 			for (unsigned j = b; j < e; ++j) {
 				if ((int)j == self) continue;
 				const float2 pj = S.pos[j];
+				const float fx = q.x - pj.x, fy = q.y - pj.y;
+				if (fx * fx + fy * fy > r2_reject) continue;
 				const double ddx = qx - (double)pj.x;
 				const double ddy = qy - (double)pj.y;
 				const double d2 = ddx * ddx + ddy * ddy;

@@ -11,7 +11,9 @@ import os
 import numpy as np
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(HERE, "libumans_gpu.so")
+# UMANS_GPU_LIB names another build of the same library (for example one compiled with
+# different launch constants); the default is the in-tree build.
+LIB_PATH = os.environ.get("UMANS_GPU_LIB") or os.path.join(HERE, "libumans_gpu.so")
 
 _F = C.POINTER(C.c_float)
 _I = C.POINTER(C.c_int32)
@@ -26,7 +28,8 @@ EXPORTS = [
     "umans_set_profiling", "umans_get_profile", "umans_create_batch", "umans_num_worlds",
     "umans_set_graph", "umans_autotune", "umans_set_tuning", "umans_adjoint_forward", "umans_adjoint_backward",
     "umans_cost_param_read_name", "umans_scenario_load", "umans_scenario_free", "umans_scenario_get_info",
-    "umans_scenario_warning", "umans_scenario_get", "umans_scenario_obstacles",
+    "umans_scenario_warning", "umans_scenario_get", "umans_scenario_obstacles", "umans_csv_start",
+    "umans_csv_flush", "umans_csv_stop",
 ]
 
 
@@ -129,6 +132,9 @@ def lib():
         L.umans_scenario_warning.restype = C.c_char_p
         L.umans_scenario_get.argtypes = [vp, C.POINTER(PolicyC), _I, C.POINTER(AgentsC), _F, _I]
         L.umans_scenario_obstacles.argtypes = [vp, _I, _F]
+        L.umans_csv_start.argtypes = [vp, C.c_char_p, C.c_int]
+        L.umans_csv_flush.argtypes = [vp]
+        L.umans_csv_stop.argtypes = [vp]
         L.umans_set_profiling.argtypes = [vp, C.c_int]
         L.umans_get_profile.argtypes = [vp, C.POINTER(Profile)]
         _lib = L
@@ -264,6 +270,15 @@ class World:
         seg = np.zeros((n, stride), np.int32)
         self._check(lib().umans_dump_neighbors(self._h, stride, _p(cnt, _I), _p(ids, _I), _p(d2, _D), _p(seg, _I)))
         return cnt, ids, d2, seg
+
+    def csv_start(self, dirname, flush_immediately=False):
+        self._check(lib().umans_csv_start(self._h, str(dirname).encode(), 1 if flush_immediately else 0))
+
+    def csv_flush(self):
+        self._check(lib().umans_csv_flush(self._h))
+
+    def csv_stop(self):
+        self._check(lib().umans_csv_stop(self._h))
 
     def set_graph(self, enable=True):
         self._check(lib().umans_set_graph(self._h, 1 if enable else 0))
