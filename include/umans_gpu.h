@@ -31,6 +31,7 @@ extern "C" {
 
 enum {
 	UMANS_OK = 0,
+	UMANS_ENOTFOUND = -2,  /* no agent with that ID */
 	UMANS_EINVAL = -22,    /* bad argument */
 	UMANS_ENOMEM = -12,    /* device allocation failed */
 	UMANS_EHIP = -1000,    /* HIP runtime error (message in umans_last_error) */
@@ -160,6 +161,47 @@ typedef struct {
 } umans_tuning;
 int umans_autotune(umans_ctx* ctx, int trial_steps, umans_tuning* out);
 int umans_set_tuning(umans_ctx* ctx, const umans_tuning* tuning);
+
+/* Population changes of a single-world context (UMANS_EINVAL for a batch), applied on
+ * the device at the next step or read-back. Rows of umans_download are the list order
+ * of WorldBase::agents_: appended on insertion, the last row moves into a removed one.
+ *
+ * umans_add_agents: WorldBase::AddAgent (worldBase.cpp:161-187). a->id holds desired
+ *   IDs (NULL or < 0: automatic); an ID already in the list is replaced by the next
+ *   unused one; the IDs given are written to out_ids (may be NULL). a->goal NULL = the
+ *   position (APIFunctions.cpp:163-164); a->viewdir NULL = looking at the goal;
+ *   a->vel NULL = zero. start_time (NULL = now): later agents are scheduled and join
+ *   the list at the first step whose start time has come (worldBase.cpp:81-86).
+ *   remove_at_goal (NULL = none): the agent leaves the list after the step in which it
+ *   reaches its goal (worldBase.cpp:124-127). flags: UMANS_ADD_KEEP_IDS takes a->id as
+ *   final, for agents whose IDs the scenario loader assigned at load time.
+ * umans_remove_agent: WorldBase::RemoveAgent (worldBase.cpp:196-235); UMANS_ENOTFOUND.
+ * umans_set_agent_goal: Agent::setGoal, which also turns the agent towards the goal
+ *   (agent.cpp:120-127); UMANS_ENOTFOUND.
+ * umans_set_agent_states: SetAgentPositions (APIFunctions.cpp:117-137): position,
+ *   velocity and viewing direction by ID (n x 2 floats each); unknown IDs are skipped.
+ * umans_set_remove_at_goal: the remove_at_goal flag of listed agents. */
+enum { UMANS_ADD_KEEP_IDS = 1 };   /* IDs already assigned (umans_scenario_get): used as given */
+int umans_add_agents(umans_ctx* ctx, const umans_agents* agents, int n, const float* start_time,
+                     const uint8_t* remove_at_goal, int flags, int32_t* out_ids);
+int umans_remove_agent(umans_ctx* ctx, int32_t id);
+int umans_set_agent_goal(umans_ctx* ctx, int32_t id, float x, float y);
+int umans_set_agent_states(umans_ctx* ctx, int n, const int32_t* ids, const float* pos, const float* vel,
+                           const float* viewdir);
+int umans_set_remove_at_goal(umans_ctx* ctx, int n, const int32_t* ids, const uint8_t* flags);
+int umans_num_scheduled(const umans_ctx* ctx);   /* agents waiting for their start time */
+
+/* The list in row order as packed records with the layout of the reference's AgentData
+ * (APIFunctions.h:33-49; GetAgentPositions, APIFunctions.cpp:86-115): packed on the
+ * device and copied once. `out` holds umans_num_agents() records after pending
+ * population changes are applied (pinned host memory copies fastest). */
+typedef struct {
+	int32_t id;
+	float position_x, position_y;
+	float velocity_x, velocity_y;
+	float viewdir_x, viewdir_y;
+} umans_agent_record;
+int umans_download_records(umans_ctx* ctx, umans_agent_record* out, int capacity);
 
 /* CSV trajectory output: CrowdSimulator::StartCSVOutput (crowdSimulator.cpp:37-51) and
  * CSVWriter (csvwriter.cpp:37-87). After every step the positions are copied

@@ -119,6 +119,26 @@ void ref_get_state(void* sim, int* ids, float* pos, float* vel, float* viewdir, 
 	}
 }
 
+// Per-agent settings in world list order: radius, preferred / maximum speed, maximum
+// acceleration, and the XML id of the agent's policy (searched in 0..max_policy_id).
+void ref_get_settings(void* sim, float* radius, float* pref_speed, float* max_speed, float* max_accel,
+	int* policy_xml_id, int max_policy_id)
+{
+	auto* cs = static_cast<CrowdSimulator*>(sim);
+	const auto& agents = cs->GetWorld()->GetAgents();
+	for (size_t i = 0; i < agents.size(); ++i)
+	{
+		const Agent* a = agents[i];
+		radius[i] = a->getRadius();
+		pref_speed[i] = a->getPreferredSpeed();
+		max_speed[i] = a->getMaximumSpeed();
+		max_accel[i] = a->getMaximumAcceleration();
+		policy_xml_id[i] = -1;
+		for (int p = 0; p <= max_policy_id; ++p)
+			if (cs->GetPolicy(p) == a->getPolicy()) { policy_xml_id[i] = p; break; }
+	}
+}
+
 // Total number of neighbour entries over all agents from the last step.
 long long ref_total_neighbors(void* sim)
 {

@@ -15,6 +15,7 @@ import numpy as np
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "_ref", "libumans_ref.so")
+API_LIB_PATH = os.path.join(HERE, "_ref", "libumans_ref_api.so")
 
 _F = C.POINTER(C.c_float)
 _I = C.POINTER(C.c_int32)
@@ -40,6 +41,7 @@ def lib():
         L.ref_set_velocities.argtypes = [C.c_void_p, C.c_int, _F, _F]
         L.ref_step.argtypes = [C.c_void_p, C.c_int]
         L.ref_start_csv.argtypes = [C.c_void_p, C.c_char_p, C.c_int]
+        L.ref_get_settings.argtypes = [C.c_void_p, _F, _F, _F, _F, _I, C.c_int]
         L.ref_time_steps.argtypes = [C.c_void_p, C.c_int]; L.ref_time_steps.restype = C.c_double
         L.ref_get_state.argtypes = [C.c_void_p, _I, _F, _F, _F, _F, _F]
         L.ref_total_neighbors.argtypes = [C.c_void_p]; L.ref_total_neighbors.restype = C.c_longlong
@@ -153,6 +155,14 @@ class RefSim:
         out["id"] = ids
         return out
 
+    def settings(self, max_policy_id=64):
+        n = lib().ref_num_agents(self.h)
+        out = {k: np.zeros(n, np.float32) for k in ("radius", "pref_speed", "max_speed", "max_accel")}
+        out["policy_xml_id"] = np.zeros(n, np.int32)
+        lib().ref_get_settings(self.h, _p(out["radius"], _F), _p(out["pref_speed"], _F), _p(out["max_speed"], _F),
+                               _p(out["max_accel"], _F), _p(out["policy_xml_id"], _I), int(max_policy_id))
+        return out
+
     def neighbors(self):
         n = lib().ref_num_agents(self.h)
         tot = lib().ref_total_neighbors(self.h)
@@ -163,3 +173,4 @@ class RefSim:
         lib().ref_get_neighbors(self.h, _p(cnt, _I), _p(ids, _I), _p(pos, _F), _p(dsq, _F))
         off = np.concatenate([[0], np.cumsum(cnt)])
         return [(ids[off[i]:off[i + 1]], pos[off[i]:off[i + 1]], dsq[off[i]:off[i + 1]]) for i in range(n)]
+

@@ -89,6 +89,7 @@ def main():
     written += 1
     written += examples()
     written += csv_output()
+    written += dynamic_population()
     print(f"wrote {written} fixtures to {HERE}")
 
 
@@ -147,6 +148,101 @@ def csv_output():
                         state_pos=np.stack([s["pos"] for s in states]).astype(np.float32),
                         state_id=np.stack([s["id"] for s in states]).astype(np.int32))
     return 1
+
+
+DYNAMIC_XML = "Dynamic-Mixed.xml"
+DYNAMIC_STEPS = 60
+
+
+def dynamic_scenario_xml():
+    """Two groups crossing under two policies (XML ids 0 and 7), some agents removed at
+    their goal, some joining later (equal start times included), some with explicit IDs
+    or an ID already in the list. No two agents share an ID: the reference resolves
+    neighbours by ID (WorldBase::GetAgent_const, worldBase.cpp:51-60), so agents sharing
+    one see the same neighbour twice there."""
+    rng = np.random.default_rng(5)
+    pol = ("  <Policies>\n"
+           "    <Policy id=\"0\" OptimizationMethod=\"gradient\" RelaxationTime=\"0.5\">\n"
+           "      <costfunction range=\"5\" name=\"SocialForcesAvoidance\" />\n"
+           "      <costfunction name=\"GoalReachingForce\" />\n"
+           "    </Policy>\n"
+           "    <Policy id=\"7\" OptimizationMethod=\"global\" RelaxationTime=\"0\">\n"
+           "      <costfunction range=\"5\" name=\"ORCA\" />\n"
+           "    </Policy>\n"
+           "  </Policies>\n")
+    agents = []
+
+    def agent(x, y, gx, gy, policy, attrs=""):
+        agents.append(f'    <Agent rad="0.3" pref_speed="1.3" max_speed="1.6"{attrs}>\n'
+                      f'      <pos x="{x:.3f}" y="{y:.3f}"/>\n      <goal x="{gx:.3f}" y="{gy:.3f}"/>\n'
+                      f'      <Policy id="{policy}"/>\n    </Agent>\n')
+    for i in range(16):
+        x, y = -6.0 + (i % 4) * 0.8 + rng.uniform(-0.1, 0.1), -1.5 + (i // 4) * 0.9 + rng.uniform(-0.1, 0.1)
+        gx = 5.0 if i % 2 else -2.5
+        agent(x, y, gx, y + rng.uniform(-0.5, 0.5), 0, ' remove_at_goal="true"' if i % 2 == 0 else "")
+    for i in range(16):
+        x, y = 6.0 - (i % 4) * 0.8 + rng.uniform(-0.1, 0.1), -1.4 + (i // 4) * 0.9 + rng.uniform(-0.1, 0.1)
+        gx = -5.0 if i % 3 else 2.5
+        agent(x, y, gx, y + rng.uniform(-0.5, 0.5), 7, ' remove_at_goal="true"' if i % 3 == 0 else "")
+    late = [(0.5, ""), (0.5, ' id="3"'), (1.0, ' id="500"'), (1.05, ""), (2.0, ' remove_at_goal="true"'),
+            (2.0, ""), (2.0, ' id="40"'), (3.3, ""), (0.5, ' id="700"'), (4.0, ' remove_at_goal="true"')]
+    for j, (t, extra) in enumerate(late):
+        side = -1 if j % 2 == 0 else 1
+        x, y = side * 7.0, -2.0 + 0.45 * j
+        agent(x, y, -side * 4.0, y, 0 if side < 0 else 7, f' start_time="{t}"' + extra)
+    return ('<?xml version="1.0" encoding="utf-8"?>\n<Simulation delta_time="0.1" end_time="10">\n'
+            '  <World type="Infinite" />\n' + pol + "  <Agents>\n" + "".join(agents) + "  </Agents>\n</Simulation>\n")
+
+
+def ragged(states, keys=("id", "pos", "vel", "viewdir")):
+    out = {"count": np.array([len(s["id"]) for s in states], np.int32)}
+    for k in keys:
+        out[k] = np.concatenate([s[k] for s in states])
+    return out
+
+
+def dynamic_population():
+    """The reference over a scenario with scheduled insertions and remove_at_goal: the
+    agent list (IDs in list order) and states after every step; and the reference's
+    embedding API (src/Library/APIFunctions.cpp) driven through a fixed call sequence."""
+    sys.path.insert(0, os.path.dirname(HERE))
+    import embedding_api as E
+    path = os.path.join(HERE, "examples", DYNAMIC_XML)
+    with open(path, "w") as f:
+        f.write(dynamic_scenario_xml())
+    # IDs only: a scheduled agent may be given the ID of another scheduled agent
+    # (GetAgent looks at the list only, worldBase.cpp:165), so two agents share an ID
+    ids_path = os.path.join(HERE, "examples", "Dynamic-Ids.xml")
+    with open(ids_path, "w") as f:
+        f.write('<?xml version="1.0" encoding="utf-8"?>\n<Simulation delta_time="0.1">\n  <World type="Infinite" />\n'
+                '  <Policies file="policies/SocialForces.xml" />\n  <Agents>\n'
+                + "".join(f'    <Agent{a}>\n      <pos x="{x}" y="0"/>\n      <goal x="{x}" y="5"/>\n'
+                          f'      <Policy id="0"/>\n    </Agent>\n'
+                          for x, a in ((0, ""), (3, ' start_time="1"'), (6, ' start_time="2" id="1"'),
+                                       (9, ' id="1"'), (12, ' start_time="0.5"')))
+                + "  </Agents>\n</Simulation>\n")
+    sim = R.RefSim.from_file(ids_path)
+    sim.step(25)
+    ids_list = sim.state()["id"]
+    sim.close()
+    import tempfile
+    with tempfile.TemporaryDirectory() as tmp:
+        sim = R.RefSim.from_file(path)
+        sim.start_csv(tmp, False)
+        states = [sim.state()]
+        for _ in range(DYNAMIC_STEPS):
+            sim.step(1)
+            states.append(sim.state())
+        sim.close()
+        files = sorted(os.listdir(tmp), key=lambda f: int(f[len("output_"):-4]))
+        csv_ids = np.array([int(f[len("output_"):-4]) for f in files], np.int32)
+        csv_text = np.array([open(os.path.join(tmp, f)).read() for f in files])
+    np.savez_compressed(os.path.join(HERE, "dynamic_Mixed.npz"), csv_id=csv_ids, csv_text=csv_text, ids_list=ids_list,
+                        **{"state_" + k: v for k, v in ragged(states).items()})
+    L = E.bind_embedding_api(R.API_LIB_PATH)
+    rec = E.api_sequence(L, path, 7)
+    np.savez_compressed(os.path.join(HERE, "api_Mixed.npz"), **E.flatten(rec))
+    return 2
 
 
 if __name__ == "__main__":

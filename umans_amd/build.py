@@ -17,7 +17,7 @@ LIB = os.path.join(HERE, "libumans_gpu.so")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = os.environ.get("UMANS_OFFLOAD_ARCH", "gfx950")
 
-SOURCES = ["api.hip", "registry.cpp", "scenario.cpp"]
+SOURCES = ["api.hip", "registry.cpp", "scenario.cpp", "extapi.cpp"]
 FLAGS = ["-O3", "-std=c++17", "-fPIC", "-shared", f"--offload-arch={ARCH}", "-ffp-contract=off",
          "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
          "-Wno-unused-command-line-argument", "-I" + os.path.join(ROOT, "include"), "-I" + CSRC]

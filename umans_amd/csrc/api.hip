@@ -2,6 +2,7 @@
 // The kernels are in kernels.h; the registry in registry.cpp.
 #include "umans_gpu.h"
 #include "adjoint.h"
+#include "population.h"
 
 #include <sys/stat.h>
 
@@ -10,7 +11,9 @@
 #include <cstdio>
 #include <cstring>
 #include <fstream>
+#include <queue>
 #include <string>
+#include <unordered_map>
 #include <vector>
 
 using namespace umans;
@@ -194,18 +197,52 @@ struct umans_ctx {
 	double last_loss = 0.0;
 
 	// CSV trajectory output (CrowdSimulator::StartCSVOutput + CSVWriter): positions are
-	// copied after each step into one of two pinned slots and logged per agent row
+	// copied after each step into one of two pinned slots and logged per agent
 	struct CsvEntry { double t; float x, y; };
 	bool csv_on = false, csv_flush_now = false;
 	std::string csv_dir;
-	std::vector<std::vector<CsvEntry>> csv_log;   // per upload row
-	std::vector<int32_t> csv_ids;                 // agent ID per upload row
+	std::vector<std::vector<CsvEntry>> csv_log;   // per log slot
+	std::vector<int32_t> csv_slot_id;             // agent ID of each log slot
+	std::unordered_map<int32_t, int> csv_slot_of_id;
+	std::vector<int> csv_row_slot;                // log slot of each upload row
 	float2* csv_pos[2] = {nullptr, nullptr};
 	int4* csv_meta[2] = {nullptr, nullptr};
 	hipEvent_t csv_ev[2] = {nullptr, nullptr};
 	double csv_t[2] = {0.0, 0.0};
 	bool csv_pending[2] = {false, false};
 	int csv_slot = 0, csv_cap = 0;
+
+	// The agent list (WorldBase::agents_, worldBase.h:117-121): upload row = list index, so
+	// downloads come back in the reference's GetAgents order. Population changes between
+	// steps (AddAgent, RemoveAgent, remove_at_goal, start_time, setters) are collected on
+	// the host and applied on the device at the next step or read-back.
+	std::vector<int32_t> row_id;                  // agent ID of each row
+	std::unordered_map<int32_t, int> row_of;      // ID -> row (built on first use)
+	bool row_index_built = false;
+	long long next_id = 0;                        // WorldBase::nextUnusedAgentID
+	std::vector<unsigned char> h_remove;          // remove_at_goal of each row
+	int n_remove = 0;
+	unsigned char* d_remove = nullptr;            // per device row
+	bool remove_dirty = false;
+	int* d_rows = nullptr;                        // rows that reached their goal / scratch
+	int* d_inv = nullptr;                         // device row -> sorted index / remap
+	unsigned* d_pop_count = nullptr;
+	unsigned* h_pop_count = nullptr;              // pinned
+	int pop_alloc = 0;                            // capacity of d_remove / d_rows / d_inv
+	struct AgentRec { float2 pos, vel, goal, view; float4 prm; int4 meta; };
+	bool pop_open = false;                        // edits collected since the last sync
+	std::vector<int> src;                         // per row: device row, or -1 - fresh index
+	std::vector<AgentRec> fresh;                  // agents added since the last sync
+	std::unordered_map<int, PopEdit> edits;       // per device row, merged in call order
+	// agents scheduled for insertion (WorldBase::agentsToAdd, worldBase.h:93-104)
+	struct PendingRec { AgentRec a; unsigned char remove; };
+	struct PendingLater {
+		bool operator()(const std::pair<int, float>& a, const std::pair<int, float>& b) const {
+			return a.second > b.second;
+		}
+	};
+	std::vector<PendingRec> pending_recs;
+	std::priority_queue<std::pair<int, float>, std::vector<std::pair<int, float>>, PendingLater> pending;
 
 	// per-phase timers: one event triple per profiled step, read back lazily
 	int profiling = 0;
@@ -250,6 +287,12 @@ struct umans_ctx {
 			csv_pending[i] = false;
 		}
 		csv_cap = 0;
+	}
+	void free_pop_buffers() {
+		hipFree(d_remove); hipFree(d_rows); hipFree(d_inv); hipFree(d_pop_count);
+		if (h_pop_count) hipHostFree(h_pop_count);
+		d_remove = nullptr; d_rows = nullptr; d_inv = nullptr; d_pop_count = nullptr; h_pop_count = nullptr;
+		pop_alloc = 0;
 	}
 	void free_agents() {
 		drop_graph();
@@ -482,11 +525,33 @@ int csv_take(umans_ctx* c, int slot) {
 	const float2* p = c->csv_pos[slot];
 	const int4* m = c->csv_meta[slot];
 	for (int k = 0; k < c->n; ++k) {
-		const int row = m[k].z;
-		c->csv_log[row].push_back({c->csv_t[slot], p[k].x, p[k].y});
+		const int sl = c->csv_row_slot[m[k].z];
+		if (sl >= 0) c->csv_log[sl].push_back({c->csv_t[slot], p[k].x, p[k].y});
 	}
 	c->csv_pending[slot] = false;
 	return UMANS_OK;
+}
+
+// The log slot of every row of the current agent list (slots are per agent ID, so an
+// agent keeps its file when rows move after removals). Agents that share an ID log one
+// position per step, the last one in list order's (the std::map of
+// CrowdSimulator::RunSimulationSteps, crowdSimulator.cpp:83-93); the others get -1.
+void csv_rows_changed(umans_ctx* c) {
+	c->csv_row_slot.resize(c->row_id.size());
+	std::unordered_map<int, int> last_row;
+	for (size_t r = 0; r < c->row_id.size(); ++r) {
+		const int32_t id = c->row_id[r];
+		auto it = c->csv_slot_of_id.find(id);
+		if (it == c->csv_slot_of_id.end()) {
+			it = c->csv_slot_of_id.emplace(id, (int)c->csv_slot_id.size()).first;
+			c->csv_slot_id.push_back(id);
+			c->csv_log.emplace_back();
+		}
+		c->csv_row_slot[r] = it->second;
+		auto lr = last_row.find(it->second);
+		if (lr != last_row.end()) c->csv_row_slot[lr->second] = -1;
+		last_row[it->second] = (int)r;
+	}
 }
 
 // After a step the older pending copy is in csv_slot (the slot written next), the newer
@@ -502,23 +567,40 @@ int csv_drain(umans_ctx* c) {
 int csv_flush(umans_ctx* c) {
 	int rc = csv_drain(c);
 	if (rc) return rc;
-	std::vector<int> order(c->csv_ids.size());
+	std::vector<int> order(c->csv_slot_id.size());
 	for (size_t i = 0; i < order.size(); ++i) order[i] = (int)i;
-	std::sort(order.begin(), order.end(), [&](int a, int b) { return c->csv_ids[a] < c->csv_ids[b]; });
-	for (int row : order) {
-		auto& log = c->csv_log[row];
+	std::sort(order.begin(), order.end(), [&](int a, int b) { return c->csv_slot_id[a] < c->csv_slot_id[b]; });
+	for (int sl : order) {
+		auto& log = c->csv_log[sl];
 		if (log.empty()) continue;
-		std::ofstream f(c->csv_dir + "output_" + std::to_string(c->csv_ids[row]) + ".csv", std::ios::app);
+		std::ofstream f(c->csv_dir + "output_" + std::to_string(c->csv_slot_id[sl]) + ".csv", std::ios::app);
 		for (const auto& e : log) f << e.t << "," << e.x << "," << e.y << "\n";
 		log.clear();
 	}
 	return UMANS_OK;
 }
 
+int csv_alloc(umans_ctx* c, int n) {
+	c->free_csv_buffers();
+	int rc;
+	for (int i = 0; i < 2; ++i) {
+		if ((rc = c->hip(hipHostMalloc((void**)&c->csv_pos[i], std::max(n, 1) * sizeof(float2)), "csv pinned"))) return rc;
+		if ((rc = c->hip(hipHostMalloc((void**)&c->csv_meta[i], std::max(n, 1) * sizeof(int4)), "csv pinned"))) return rc;
+		if ((rc = c->hip(hipEventCreate(&c->csv_ev[i]), "csv event"))) return rc;
+	}
+	c->csv_cap = n;
+	return UMANS_OK;
+}
+
 // After a step: copy the positions (and rows) into the next pinned slot.
 int csv_after_step(umans_ctx* c) {
+	int rc;
+	if (c->n > c->csv_cap) {                 // the population grew: wider pinned slots
+		if ((rc = csv_drain(c))) return rc;
+		if ((rc = csv_alloc(c, c->n + c->n / 2))) return rc;
+	}
 	const int slot = c->csv_slot;
-	int rc = csv_take(c, slot);              // the copy two steps ago must be logged first
+	rc = csv_take(c, slot);                  // the copy two steps ago must be logged first
 	if (rc) return rc;
 	if ((rc = c->hip(hipMemcpyAsync(c->csv_pos[slot], c->X.s.pos, c->n * sizeof(float2), hipMemcpyDeviceToHost,
 	                                c->stream), "csv copy"))) return rc;
@@ -637,8 +719,253 @@ int capture_graph(umans_ctx* c) {
 	return c->hip(hipGraphInstantiate(&c->graph_exec, g, nullptr, nullptr, 0), "graph instantiate");
 }
 
+// ------------------------------------------------------------------ population
+// The agent list of a single-world context changes between steps through WorldBase's
+// AddAgent / RemoveAgent / remove_at_goal / scheduled insertion (worldBase.cpp:79-235)
+// and the agent setters (agent.cpp:108-127). The host keeps the list (row -> ID) and
+// collects the changes; sync_population applies them on the device in one pass.
+
+void build_row_index(umans_ctx* c) {
+	if (c->row_index_built) return;
+	c->row_of.clear();
+	c->row_of.reserve(c->row_id.size() * 2);
+	for (size_t r = 0; r < c->row_id.size(); ++r) c->row_of[c->row_id[r]] = (int)r;
+	c->row_index_built = true;
+}
+
+void open_edits(umans_ctx* c) {
+	if (c->pop_open) return;
+	c->src.resize(c->row_id.size());
+	for (size_t r = 0; r < c->src.size(); ++r) c->src[r] = (int)r;
+	c->pop_open = true;
+}
+
+int ensure_pop_buffers(umans_ctx* c, int n) {
+	if (c->pop_alloc >= n && c->d_pop_count) return UMANS_OK;
+	const int cap = std::max(n, 64);
+	hipFree(c->d_remove); hipFree(c->d_rows); hipFree(c->d_inv);
+	c->d_remove = nullptr; c->d_rows = nullptr; c->d_inv = nullptr;
+	c->pop_alloc = 0;
+	int rc;
+	if ((rc = c->hip(dalloc(&c->d_remove, cap), "alloc population"))) return rc;
+	if ((rc = c->hip(dalloc(&c->d_rows, cap), "alloc population"))) return rc;
+	if ((rc = c->hip(dalloc(&c->d_inv, cap), "alloc population"))) return rc;
+	if (!c->d_pop_count && (rc = c->hip(dalloc(&c->d_pop_count, 1), "alloc population"))) return rc;
+	if (!c->h_pop_count && (rc = c->hip(hipHostMalloc((void**)&c->h_pop_count, sizeof(unsigned)), "pinned"))) return rc;
+	c->pop_alloc = cap;
+	c->remove_dirty = true;
+	return UMANS_OK;
+}
+
+// Grow the agent buffers to hold n agents, keeping the first `keep` agents of X.
+int grow_agents(umans_ctx* c, int n, int keep) {
+	const int cap = std::max(n, c->cap + c->cap / 2);
+	DeviceState nx;
+	int rc;
+	if ((rc = c->hip(nx.alloc(cap), "alloc state"))) return rc;
+	if (keep > 0 && (rc = c->hip(copy_state(nx.s, c->X.s, keep, c->stream), "grow"))) { nx.release(); return rc; }
+	if ((rc = c->hip(hipStreamSynchronize(c->stream), "grow"))) { nx.release(); return rc; }
+	const bool rec = c->rec_count != nullptr;
+	c->free_agents();                         // also drops the graph, the adjoint and the record
+	c->X = nx;
+	if ((rc = c->hip(c->Y.alloc(cap), "alloc state"))) return rc;
+	if ((rc = c->hip(dalloc(&c->cell_of, cap), "alloc grid"))) return rc;
+	if ((rc = c->hip(dalloc(&c->rank, cap), "alloc grid"))) return rc;
+	c->cap = cap;
+	if (rec) {
+		const int stride = kmax_bucket(c->kcap);
+		if ((rc = c->hip(dalloc(&c->rec_count, cap), "alloc record"))) return rc;
+		if ((rc = c->hip(dalloc(&c->rec_id, (size_t)cap * stride), "alloc record"))) return rc;
+		if ((rc = c->hip(dalloc(&c->rec_d2, (size_t)cap * stride), "alloc record"))) return rc;
+		if ((rc = c->hip(dalloc(&c->rec_seg, (size_t)cap * stride), "alloc record"))) return rc;
+		c->rec_stride = stride;
+	}
+	return UMANS_OK;
+}
+
+int upload_remove_flags(umans_ctx* c) {
+	c->n_remove = 0;
+	for (unsigned char f : c->h_remove) c->n_remove += f != 0;
+	c->remove_dirty = false;
+	if (c->n_remove == 0 || c->h_remove.empty()) return UMANS_OK;
+	int rc = ensure_pop_buffers(c, (int)c->h_remove.size());
+	if (rc) return rc;
+	c->remove_dirty = false;
+	if ((rc = c->hip(hipMemcpyAsync(c->d_remove, c->h_remove.data(), c->h_remove.size(), hipMemcpyHostToDevice,
+	                                c->stream), "remove flags"))) return rc;
+	return c->hip(hipStreamSynchronize(c->stream), "remove flags");
+}
+
+// Setters collected for agents on the device (keyed by device row), applied in place.
+int apply_device_edits(umans_ctx* c) {
+	if (c->edits.empty()) return UMANS_OK;
+	const int n_dev = c->n;
+	int rc = ensure_pop_buffers(c, n_dev);
+	if (rc) return rc;
+	std::vector<PopEdit> ed;
+	ed.reserve(c->edits.size());
+	for (const auto& kv : c->edits) ed.push_back(kv.second);
+	c->edits.clear();
+	PopEdit* d_ed = nullptr;
+	if ((rc = c->hip(dalloc(&d_ed, ed.size()), "alloc edits"))) return rc;
+	hipLaunchKernelGGL(k_row_index, dim3(ceil_div(n_dev, 256)), dim3(256), 0, c->stream, c->X.s.meta, n_dev, c->d_inv);
+	rc = c->hip(hipMemcpyAsync(d_ed, ed.data(), ed.size() * sizeof(PopEdit), hipMemcpyHostToDevice, c->stream), "edits");
+	if (!rc) {
+		hipLaunchKernelGGL(k_apply_edits, dim3(ceil_div((int)ed.size(), 256)), dim3(256), 0, c->stream, c->X.s,
+		                   c->d_inv, d_ed, (int)ed.size());
+		rc = c->hip(hipStreamSynchronize(c->stream), "edits");
+	}
+	hipFree(d_ed);
+	return rc;
+}
+
+// Apply the collected population changes: edits of device agents, removals (compaction),
+// added agents (appended), then the per-row tables.
+int sync_population(umans_ctx* c) {
+	if (!c->pop_open) return c->remove_dirty ? upload_remove_flags(c) : UMANS_OK;
+	int rc;
+	if (c->csv_on && (rc = csv_drain(c))) return rc;   // logged copies refer to the old rows
+	const int n_dev = c->n;
+	const int n_new = (int)c->row_id.size();
+	bool moved = false;
+	int n_keep = 0;
+	for (int r = 0; r < n_new; ++r) {
+		if (c->src[r] >= 0) { ++n_keep; moved |= c->src[r] != r; }
+	}
+	moved |= n_keep != n_dev;
+	if ((!c->edits.empty() || moved) && (rc = ensure_pop_buffers(c, std::max(n_dev, n_new)))) return rc;
+	// 1. setters on device agents, addressed by their device row
+	if ((rc = apply_device_edits(c))) return rc;
+	// 2. removed and moved device agents: compact into Y with their new rows, swap
+	if (moved && n_dev > 0) {
+		std::vector<int> remap(n_dev, -1);
+		for (int r = 0; r < n_new; ++r)
+			if (c->src[r] >= 0) remap[c->src[r]] = r;
+		if ((rc = c->hip(hipMemcpyAsync(c->d_inv, remap.data(), n_dev * sizeof(int), hipMemcpyHostToDevice, c->stream),
+		                 "remap"))) return rc;
+		hipLaunchKernelGGL(k_fill, dim3(1), dim3(64), 0, c->stream, c->d_pop_count, 1, 0u);
+		hipLaunchKernelGGL(k_compact, dim3(ceil_div(n_dev, 256)), dim3(256), 0, c->stream, c->X.s, c->Y.s, n_dev,
+		                   c->d_inv, c->d_pop_count);
+		if ((rc = c->hip(hipStreamSynchronize(c->stream), "compact"))) return rc;
+		std::swap(c->X, c->Y);
+	}
+	// 3. added agents go after the kept ones
+	if (n_new > c->cap && (rc = grow_agents(c, n_new, n_keep))) return rc;
+	const int n_add = n_new - n_keep;
+	if (n_add > 0) {
+		std::vector<float2> pos(n_add), vel(n_add), goal(n_add), view(n_add);
+		std::vector<float4> prm(n_add);
+		std::vector<int4> meta(n_add);
+		int j = 0;
+		for (int r = 0; r < n_new; ++r) {
+			if (c->src[r] >= 0) continue;
+			const umans_ctx::AgentRec& a = c->fresh[-1 - c->src[r]];
+			pos[j] = a.pos; vel[j] = a.vel; goal[j] = a.goal; view[j] = a.view; prm[j] = a.prm;
+			meta[j] = a.meta;
+			meta[j].z = r;
+			++j;
+		}
+		const State& s = c->X.s;
+		const int o = n_keep;
+		if ((rc = c->hip(hipMemcpyAsync(s.pos + o, pos.data(), n_add * sizeof(float2), hipMemcpyHostToDevice, c->stream), "add"))) return rc;
+		if ((rc = c->hip(hipMemcpyAsync(s.vel + o, vel.data(), n_add * sizeof(float2), hipMemcpyHostToDevice, c->stream), "add"))) return rc;
+		if ((rc = c->hip(hipMemcpyAsync(s.goal + o, goal.data(), n_add * sizeof(float2), hipMemcpyHostToDevice, c->stream), "add"))) return rc;
+		if ((rc = c->hip(hipMemcpyAsync(s.viewdir + o, view.data(), n_add * sizeof(float2), hipMemcpyHostToDevice, c->stream), "add"))) return rc;
+		if ((rc = c->hip(hipMemcpyAsync(s.prm + o, prm.data(), n_add * sizeof(float4), hipMemcpyHostToDevice, c->stream), "add"))) return rc;
+		if ((rc = c->hip(hipMemcpyAsync(s.meta + o, meta.data(), n_add * sizeof(int4), hipMemcpyHostToDevice, c->stream), "add"))) return rc;
+		if ((rc = c->hip(hipStreamSynchronize(c->stream), "add"))) return rc;
+	}
+	c->pop_open = false;
+	c->src.clear();
+	c->fresh.clear();
+	c->edits.clear();
+	// 4. the per-row tables of the new list
+	if (moved || n_add > 0) {
+		c->n = n_new;
+		c->drop_graph();
+		c->free_adjoint();
+		c->have_record = false;
+		if ((rc = setup_worlds(c, std::vector<int>(1, n_new)))) return rc;
+		if ((rc = update_classes(c))) return rc;
+		c->remove_dirty = true;
+		if (c->csv_on) csv_rows_changed(c);
+	}
+	return c->remove_dirty ? upload_remove_flags(c) : UMANS_OK;
+}
+
+// Append an agent to the list (WorldBase::addAgentToList, worldBase.cpp:189-194).
+void list_append(umans_ctx* c, const umans_ctx::AgentRec& a, unsigned char remove) {
+	open_edits(c);
+	build_row_index(c);
+	const int r = (int)c->row_id.size();
+	c->row_id.push_back(a.meta.x);
+	c->row_of[a.meta.x] = r;
+	c->src.push_back(-1 - (int)c->fresh.size());
+	c->fresh.push_back(a);
+	c->h_policy.push_back(a.meta.y & kPolicyMask);
+	c->h_remove.push_back(remove);
+	if (remove) c->remove_dirty = true;
+}
+
+// WorldBase::removeAgentAtListIndex (worldBase.cpp:207-235): the last agent takes the row.
+void list_remove_row(umans_ctx* c, int r) {
+	open_edits(c);
+	build_row_index(c);
+	const int last = (int)c->row_id.size() - 1;
+	c->row_of.erase(c->row_id[r]);
+	if (r != last) {
+		c->row_id[r] = c->row_id[last];
+		c->src[r] = c->src[last];
+		c->h_policy[r] = c->h_policy[last];
+		c->h_remove[r] = c->h_remove[last];
+		c->row_of[c->row_id[r]] = r;
+	}
+	c->row_id.pop_back();
+	c->src.pop_back();
+	c->h_policy.pop_back();
+	c->h_remove.pop_back();
+	c->remove_dirty = true;
+}
+
+// WorldBase::DoStep, before the frame (worldBase.cpp:81-86): insert the scheduled agents
+// whose start time has come.
+void insert_due(umans_ctx* c) {
+	while (!c->pending.empty() && (double)c->pending.top().second <= c->sim_time) {
+		const int i = c->pending.top().first;
+		c->pending.pop();
+		list_append(c, c->pending_recs[i].a, c->pending_recs[i].remove);
+	}
+	if (c->pending.empty()) c->pending_recs.clear();
+}
+
+// WorldBase::DoStep, after the frame (worldBase.cpp:124-127): agents flagged
+// remove_at_goal that reached their goal leave the list, highest row first.
+int remove_reached(umans_ctx* c) {
+	if (c->n_remove == 0 || c->n == 0) return UMANS_OK;
+	hipLaunchKernelGGL(k_fill, dim3(1), dim3(64), 0, c->stream, c->d_pop_count, 1, 0u);
+	hipLaunchKernelGGL(k_goal_reached, dim3(ceil_div(c->n, 256)), dim3(256), 0, c->stream, c->X.s, c->n, c->d_remove,
+	                   c->d_rows, c->d_pop_count);
+	int rc = c->hip(hipMemcpyAsync(c->h_pop_count, c->d_pop_count, sizeof(unsigned), hipMemcpyDeviceToHost, c->stream),
+	                "goal check");
+	if (!rc) rc = c->hip(hipStreamSynchronize(c->stream), "goal check");
+	if (rc) return rc;
+	const int m = (int)*c->h_pop_count;
+	if (m == 0) return UMANS_OK;
+	std::vector<int> rows(m);
+	if ((rc = c->hip(hipMemcpy(rows.data(), c->d_rows, m * sizeof(int), hipMemcpyDeviceToHost), "goal check"))) return rc;
+	std::sort(rows.begin(), rows.end(), [](int a, int b) { return a > b; });
+	for (int r : rows) list_remove_row(c, r);
+	return sync_population(c);
+}
+
 int run_steps(umans_ctx* c, int nsteps) {
+	const bool single = c->worlds.size() == 1;
 	for (int s = 0; s < nsteps; ++s) {
+		if (single) {
+			insert_due(c);
+			const int rc = sync_population(c);
+			if (rc) return rc;
+		}
 		if (c->n > 0) {
 			int rc;
 			if (c->use_graph && !c->profiling) {
@@ -650,6 +977,10 @@ int run_steps(umans_ctx* c, int nsteps) {
 		}
 		c->steps += 1;
 		c->sim_time += (double)c->worlds[0].delta_time;   // WorldBase::time_ (worldBase.cpp:122)
+		if (single && c->n_remove > 0) {
+			const int rc = remove_reached(c);
+			if (rc) return rc;
+		}
 		if (c->csv_on && c->n > 0) {
 			const int rc = csv_after_step(c);
 			if (rc) return rc;
@@ -737,6 +1068,7 @@ void umans_destroy(umans_ctx* c) {
 	if (c->stream) hipStreamSynchronize(c->stream);
 	if (c->csv_on) csv_flush(c);
 	c->free_csv_buffers();
+	c->free_pop_buffers();
 	c->free_agents();
 	hipFree(c->dgrids); hipFree(c->dworlds); hipFree(c->dbbox); hipFree(c->dstats); hipFree(c->dpol);
 	hipFree(c->d_cls_slot);
@@ -807,7 +1139,7 @@ int umans_upload(umans_ctx* c, const umans_agents* a, int n) {
 			// Agent::setGoal looks at the goal (agent.cpp:120-127)
 			const float dx = goal[i].x - pos[i].x, dy = goal[i].y - pos[i].y;
 			const float m = std::sqrt(dx * dx + dy * dy);
-			view[i] = m > 0.0f ? make_float2(dx / m, dy / m) : make_float2(0.0f, 0.0f);
+			view[i] = m > 0.0f ? make_float2(dx / m, dy / m) : make_float2(dx, dy);
 		}
 		prm[i] = make_float4(a->radius[i], a->pref_speed[i], a->max_speed[i], a->max_accel[i]);
 		uint32_t r;
@@ -839,6 +1171,21 @@ int umans_upload(umans_ctx* c, const umans_agents* a, int n) {
 	c->n = n;
 	c->h_policy.assign(a->policy, a->policy + n);
 	c->have_record = false;
+	// the uploaded agents are the whole list, in row order; nothing is scheduled
+	c->row_id.assign(a->id, a->id + n);
+	c->row_of.clear();
+	c->row_index_built = false;
+	c->next_id = 0;
+	for (int i = 0; i < n; ++i) c->next_id = std::max(c->next_id, (long long)a->id[i] + 1);
+	c->h_remove.assign(n, 0);
+	c->n_remove = 0;
+	c->remove_dirty = false;
+	c->pop_open = false;
+	c->src.clear();
+	c->fresh.clear();
+	c->edits.clear();
+	c->pending_recs.clear();
+	c->pending = decltype(c->pending)();
 	int rc;
 	if ((rc = setup_worlds(c, per_world))) return rc;
 	if ((rc = update_classes(c))) return rc;
@@ -852,7 +1199,7 @@ int umans_upload(umans_ctx* c, const umans_agents* a, int n) {
 	return c->hip(hipStreamSynchronize(c->stream), "upload");
 }
 
-int umans_num_agents(const umans_ctx* c) { return c ? c->n : 0; }
+int umans_num_agents(const umans_ctx* c) { return c ? (int)c->row_id.size() : 0; }
 
 int umans_set_record_neighbors(umans_ctx* c, int enable) {
 	if (!c) return UMANS_EINVAL;
@@ -894,10 +1241,11 @@ int umans_step(umans_ctx* c, int nsteps) {
 
 int umans_autotune(umans_ctx* c, int trial_steps, umans_tuning* out) {
 	if (!c) return UMANS_EINVAL;
-	if (c->n == 0) return c->fail(UMANS_ESTATE, "upload agents before autotuning");
 	if (trial_steps < 1) trial_steps = 3;
 	hipSetDevice(c->device);
 	int rc;
+	if ((rc = sync_population(c))) return rc;
+	if (c->n == 0) return c->fail(UMANS_ESTATE, "upload agents before autotuning");
 	std::vector<int> per_world;
 	if ((rc = agents_per_world(c, per_world))) return rc;
 	if (!c->B.s.pos && (rc = c->hip(c->B.alloc(c->cap), "alloc backup"))) return rc;
@@ -975,7 +1323,6 @@ int umans_set_tuning(umans_ctx* c, const umans_tuning* t) {
 
 int umans_csv_start(umans_ctx* c, const char* dirname, int flush_immediately) {
 	if (!c || !dirname || !*dirname) return UMANS_EINVAL;
-	if (c->n == 0) return c->fail(UMANS_ESTATE, "upload agents before starting CSV output");
 	if (c->worlds.size() != 1) return c->fail(UMANS_EINVAL, "CSV output is per world: use a single-world context");
 	if (c->csv_on) {
 		hipSetDevice(c->device);
@@ -990,22 +1337,14 @@ int umans_csv_start(umans_ctx* c, const char* dirname, int flush_immediately) {
 	if (stat(dir.c_str(), &sb) != 0 || !S_ISDIR(sb.st_mode))
 		return c->fail(UMANS_EINVAL, "could not set CSV output directory to " + dir);
 	hipSetDevice(c->device);
-	int rc = c->hip(hipStreamSynchronize(c->stream), "csv");
+	int rc = sync_population(c);
+	if (!rc) rc = c->hip(hipStreamSynchronize(c->stream), "csv");
 	if (rc) return rc;
-	if (c->csv_cap < c->n) {
-		c->free_csv_buffers();
-		for (int i = 0; i < 2; ++i) {
-			if ((rc = c->hip(hipHostMalloc((void**)&c->csv_pos[i], c->n * sizeof(float2)), "csv pinned"))) return rc;
-			if ((rc = c->hip(hipHostMalloc((void**)&c->csv_meta[i], c->n * sizeof(int4)), "csv pinned"))) return rc;
-			if ((rc = c->hip(hipEventCreate(&c->csv_ev[i]), "csv event"))) return rc;
-		}
-		c->csv_cap = c->n;
-	}
-	std::vector<int4> meta(c->n);
-	if ((rc = c->hip(hipMemcpy(meta.data(), c->X.s.meta, c->n * sizeof(int4), hipMemcpyDeviceToHost), "csv ids"))) return rc;
-	c->csv_ids.assign(c->n, 0);
-	for (const int4& m : meta) c->csv_ids[m.z] = m.x;
-	c->csv_log.assign(c->n, {});
+	if (c->csv_cap < c->n && (rc = csv_alloc(c, c->n))) return rc;
+	c->csv_log.clear();
+	c->csv_slot_id.clear();
+	c->csv_slot_of_id.clear();
+	csv_rows_changed(c);
 	c->csv_dir = dir;
 	c->csv_flush_now = flush_immediately != 0;
 	c->csv_on = true;
@@ -1075,6 +1414,7 @@ int umans_get_stats(umans_ctx* c, umans_stats* out) {
 int umans_download(umans_ctx* c, umans_agents* a) {
 	if (!c || !a) return UMANS_EINVAL;
 	hipSetDevice(c->device);
+	if (int rc0 = sync_population(c)) return rc0;
 	const int n = c->n;
 	std::vector<float2> pos(n), vel(n), view(n), goal(n), pref(n);
 	std::vector<float4> prm(n);
@@ -1139,7 +1479,11 @@ int umans_dump_neighbors(umans_ctx* c, int stride, int32_t* count, int32_t* ids,
 
 int umans_adjoint_forward(umans_ctx* c, const umans_adjoint_desc* d, const float* targets, double* loss) {
 	if (!c || !d || !targets) return UMANS_EINVAL;
+	hipSetDevice(c->device);
+	if (int rc0 = sync_population(c)) return rc0;
 	if (c->n == 0) return c->fail(UMANS_ESTATE, "upload agents first");
+	if (!c->pending.empty() || c->n_remove > 0)
+		return c->fail(UMANS_ESTATE, "the adjoint needs a fixed population (no scheduled agents, no remove_at_goal)");
 	if (d->nsteps < 1 || d->checkpoint_every < 1) return c->fail(UMANS_EINVAL, "bad step or checkpoint count");
 	if (d->param_kind != UMANS_COST_SOCIAL_FORCES && d->param_kind != UMANS_COST_POWER_LAW)
 		return c->fail(UMANS_EINVAL, "parameters are differentiated for SocialForcesAvoidance or PowerLaw");
@@ -1257,6 +1601,204 @@ int umans_adjoint_backward(umans_ctx* c, float* d_params, float* d_pos0, float* 
 	if ((rc = c->hip(hipStreamSynchronize(c->stream), "adjoint"))) return rc;
 	if (d_params) std::memcpy(d_params, hp.data(), hp.size() * sizeof(float));
 	return UMANS_OK;
+}
+
+} // extern "C"
+
+// ------------------------------------------------------------- population API
+namespace {
+
+int single_world(umans_ctx* c) {
+	if (c->worlds.size() != 1) return c->fail(UMANS_EINVAL, "population changes need a single-world context");
+	return UMANS_OK;
+}
+
+// The list row of an agent ID, or -1.
+int find_row(umans_ctx* c, int32_t id) {
+	build_row_index(c);
+	const auto it = c->row_of.find(id);
+	return it == c->row_of.end() ? -1 : it->second;
+}
+
+// Agent::setGoal's look direction (agent.cpp:120-127; Vector2D::normalize, vector2D.h:55-63).
+bool look_at_goal(float2 pos, float2 goal, float2& view) {
+	if (goal.x == pos.x && goal.y == pos.y) return false;
+	const float dx = goal.x - pos.x, dy = goal.y - pos.y;
+	const float mag = std::sqrt(dx * dx + dy * dy);
+	view = mag > 0.0f ? make_float2(dx / mag, dy / mag) : make_float2(dx, dy);
+	return true;
+}
+
+} // namespace
+
+extern "C" {
+
+int umans_add_agents(umans_ctx* c, const umans_agents* a, int n, const float* start_time,
+                     const uint8_t* remove_at_goal, int flags, int32_t* out_ids) {
+	if (!c) return UMANS_EINVAL;
+	if (int rc = single_world(c)) return rc;
+	if (c->hpol.empty()) return c->fail(UMANS_ESTATE, "set policies before adding agents");
+	if (n < 0) return c->fail(UMANS_EINVAL, "bad agent count");
+	if (n > 0 && (!a || !a->pos || !a->radius || !a->pref_speed || !a->max_speed || !a->max_accel || !a->policy))
+		return c->fail(UMANS_EINVAL, "missing agent array");
+	const int npol = (int)c->hpol.size();
+	for (int i = 0; i < n; ++i)
+		if (a->policy[i] < 0 || a->policy[i] >= npol)
+			return c->fail(UMANS_EINVAL, "agent " + std::to_string(i) + " has an unknown policy id");
+	if ((long long)c->row_id.size() + (long long)c->pending.size() + n > (long long)kIndexMask)
+		return c->fail(UMANS_EINVAL, "too many agents");
+	build_row_index(c);
+	for (int i = 0; i < n; ++i) {
+		// WorldBase::AddAgent (worldBase.cpp:161-187): a desired ID is used when no agent
+		// in the list has it, otherwise the next unused ID
+		const long long desired = a->id ? (long long)a->id[i] : -1;
+		long long id;
+		if (flags & UMANS_ADD_KEEP_IDS) {
+			if (desired < 0) return c->fail(UMANS_EINVAL, "UMANS_ADD_KEEP_IDS needs an ID for every agent");
+			id = desired;
+		} else {
+			id = (desired >= 0 && !c->row_of.count((int32_t)desired)) ? desired : c->next_id;
+		}
+		if (id > 0x7fffffffLL) return c->fail(UMANS_EINVAL, "agent IDs exhausted");
+		if (id >= c->next_id) c->next_id = id + 1;
+		umans_ctx::AgentRec r;
+		r.pos = make_float2(a->pos[2 * i], a->pos[2 * i + 1]);
+		r.vel = a->vel ? make_float2(a->vel[2 * i], a->vel[2 * i + 1]) : make_float2(0.0f, 0.0f);
+		// without a goal the agent stays where it is (APIFunctions.cpp:163-164)
+		r.goal = a->goal ? make_float2(a->goal[2 * i], a->goal[2 * i + 1]) : r.pos;
+		r.view = make_float2(0.0f, 0.0f);
+		if (a->viewdir) r.view = make_float2(a->viewdir[2 * i], a->viewdir[2 * i + 1]);
+		else look_at_goal(r.pos, r.goal, r.view);
+		r.prm = make_float4(a->radius[i], a->pref_speed[i], a->max_speed[i], a->max_accel[i]);
+		uint32_t seed;
+		if (a->rng) seed = a->rng[i];
+		else {
+			seed = (uint32_t)((uint32_t)id % 2147483647u);   // minstd_rand0 seeded with the ID (agent.cpp:38)
+			if (seed == 0) seed = 1;
+		}
+		r.meta = make_int4((int)id, a->policy[i], -1, (int)seed);
+		const unsigned char rm = remove_at_goal && remove_at_goal[i] ? 1 : 0;
+		const float t = start_time ? start_time[i] : 0.0f;
+		if ((double)t <= c->sim_time) {
+			list_append(c, r, rm);
+		} else {
+			c->pending.push({(int)c->pending_recs.size(), t});
+			c->pending_recs.push_back({r, rm});
+		}
+		if (out_ids) out_ids[i] = (int32_t)id;
+	}
+	return UMANS_OK;
+}
+
+int umans_remove_agent(umans_ctx* c, int32_t id) {
+	if (!c) return UMANS_EINVAL;
+	if (int rc = single_world(c)) return rc;
+	const int r = find_row(c, id);
+	if (r < 0) return c->fail(UMANS_ENOTFOUND, "no agent with ID " + std::to_string(id));
+	list_remove_row(c, r);
+	return UMANS_OK;
+}
+
+int umans_set_agent_goal(umans_ctx* c, int32_t id, float x, float y) {
+	if (!c) return UMANS_EINVAL;
+	if (int rc = single_world(c)) return rc;
+	const int r = find_row(c, id);
+	if (r < 0) return c->fail(UMANS_ENOTFOUND, "no agent with ID " + std::to_string(id));
+	open_edits(c);
+	const int h = c->src[r];
+	const float2 g = make_float2(x, y);
+	if (h < 0) {
+		umans_ctx::AgentRec& rec = c->fresh[-1 - h];
+		rec.goal = g;
+		look_at_goal(rec.pos, g, rec.view);
+		return UMANS_OK;
+	}
+	auto it = c->edits.find(h);
+	if (it != c->edits.end() && (it->second.mask & EDIT_VIEW_FROM_GOAL)) {
+		// a second goal before the device has the first one's look direction
+		if (int rc = apply_device_edits(c)) return rc;
+		it = c->edits.end();
+	}
+	if (it == c->edits.end()) {
+		PopEdit e{};
+		e.row = h;
+		it = c->edits.emplace(h, e).first;
+	}
+	PopEdit& e = it->second;
+	e.mask |= EDIT_GOAL;
+	e.goal = g;
+	if (e.mask & EDIT_POS) {
+		if (look_at_goal(e.pos, g, e.view)) e.mask |= EDIT_VIEW;
+	} else {
+		e.mask |= EDIT_VIEW_FROM_GOAL;
+	}
+	return UMANS_OK;
+}
+
+int umans_set_agent_states(umans_ctx* c, int n, const int32_t* ids, const float* pos, const float* vel,
+                           const float* viewdir) {
+	if (!c) return UMANS_EINVAL;
+	if (int rc = single_world(c)) return rc;
+	if (n < 0 || (n > 0 && (!ids || !pos || !vel || !viewdir))) return c->fail(UMANS_EINVAL, "bad agent arrays");
+	for (int i = 0; i < n; ++i) {
+		// SetAgentPositions (APIFunctions.cpp:117-137): unknown IDs are skipped
+		const int r = find_row(c, ids[i]);
+		if (r < 0) continue;
+		open_edits(c);
+		const float2 p = make_float2(pos[2 * i], pos[2 * i + 1]);
+		const float2 v = make_float2(vel[2 * i], vel[2 * i + 1]);
+		const float2 w = make_float2(viewdir[2 * i], viewdir[2 * i + 1]);
+		const int h = c->src[r];
+		if (h < 0) {
+			umans_ctx::AgentRec& rec = c->fresh[-1 - h];
+			rec.pos = p; rec.vel = v; rec.view = w;
+			continue;
+		}
+		auto it = c->edits.find(h);
+		if (it == c->edits.end()) {
+			PopEdit e{};
+			e.row = h;
+			it = c->edits.emplace(h, e).first;
+		}
+		PopEdit& e = it->second;
+		e.mask = (e.mask | EDIT_POS | EDIT_VEL | EDIT_VIEW) & ~EDIT_VIEW_FROM_GOAL;
+		e.pos = p; e.vel = v; e.view = w;
+	}
+	return UMANS_OK;
+}
+
+int umans_set_remove_at_goal(umans_ctx* c, int n, const int32_t* ids, const uint8_t* flags) {
+	if (!c) return UMANS_EINVAL;
+	if (int rc = single_world(c)) return rc;
+	if (n < 0 || (n > 0 && (!ids || !flags))) return c->fail(UMANS_EINVAL, "bad arrays");
+	for (int i = 0; i < n; ++i) {
+		const int r = find_row(c, ids[i]);
+		if (r < 0) return c->fail(UMANS_ENOTFOUND, "no agent with ID " + std::to_string(ids[i]));
+		c->h_remove[r] = flags[i] ? 1 : 0;
+	}
+	c->remove_dirty = true;
+	return UMANS_OK;
+}
+
+int umans_num_scheduled(const umans_ctx* c) { return c ? (int)c->pending.size() : 0; }
+
+int umans_download_records(umans_ctx* c, umans_agent_record* out, int capacity) {
+	static_assert(sizeof(umans_agent_record) == sizeof(AgentRecord), "record layout");
+	if (!c || (!out && capacity > 0)) return UMANS_EINVAL;
+	hipSetDevice(c->device);
+	int rc = sync_population(c);
+	if (rc) return rc;
+	const int n = c->n;
+	if (n > capacity) return c->fail(UMANS_EINVAL, "record buffer holds " + std::to_string(capacity) +
+	                                 " agents, the list has " + std::to_string(n));
+	if (n == 0) return UMANS_OK;
+	AgentRecord* d = nullptr;
+	if ((rc = c->hip(dalloc(&d, n), "alloc records"))) return rc;
+	hipLaunchKernelGGL(k_pack_records, dim3(ceil_div(n, 256)), dim3(256), 0, c->stream, c->X.s, n, d);
+	rc = c->hip(hipMemcpyAsync(out, d, n * sizeof(AgentRecord), hipMemcpyDeviceToHost, c->stream), "records");
+	if (!rc) rc = c->hip(hipStreamSynchronize(c->stream), "records");
+	hipFree(d);
+	return rc;
 }
 
 } // extern "C"

@@ -425,6 +425,24 @@ UD V2 sample_best(const DevPolicy& P, const Self& a, const NbList& L, const Prep
 }
 
 // ------------------------------------------------------------------ the step
+// Workgroups go to the 8 XCDs round-robin (workgroup b to XCD b mod 8). Each XCD is
+// given a contiguous run of tiles instead, so the agents its CUs hold are neighbours in
+// the sorted order and its L2 keeps the cells they gather from.
+#ifndef UMANS_XCD_REMAP
+#define UMANS_XCD_REMAP 1
+#endif
+constexpr int kNumXcd = 8;
+UD int xcd_tile(int b, int nb) {
+#if UMANS_XCD_REMAP
+	const int per = nb / kNumXcd, rem = nb % kNumXcd;
+	const int x = b % kNumXcd, i = b / kNumXcd;
+	return x * per + (x < rem ? x : rem) + i;
+#else
+	(void)nb;
+	return b;
+#endif
+}
+
 // One lane per agent of the class's sorted range [A.base, A.base + A.n).
 template <int KMAX, int BS, int CLS>
 __global__ __launch_bounds__(BS) void k_step(StepArgs A) {
@@ -437,7 +455,7 @@ __global__ __launch_bounds__(BS) void k_step(StepArgs A) {
 	constexpr int kLdsLines = kLinesInLds ? KMAX * BS : 1;
 	__shared__ Line s_lines[kLdsLines];
 	const int tid = threadIdx.x;
-	const int t = blockIdx.x * BS + tid;
+	const int t = xcd_tile(blockIdx.x, gridDim.x) * BS + tid;
 	const bool active = t < A.n;
 	const int k = active ? A.base + t : 0;
 	int total = 0, cnt = 0;

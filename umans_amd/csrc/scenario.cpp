@@ -306,12 +306,14 @@ bool load_policy(umans_scenario& sc, const Node* e) {
 }
 
 struct IdAlloc {
-	std::set<long long> used;
+	std::set<long long> listed;
 	long long next = 0;
-	// WorldBase::AddAgent (worldBase.cpp:161-187)
-	long long take(long long desired) {
-		long long id = (desired >= 0 && !used.count(desired)) ? desired : next;
-		used.insert(id);
+	// WorldBase::AddAgent (worldBase.cpp:161-187): a desired ID is refused only when an
+	// agent in the list has it (GetAgent); agents scheduled for later are not in the list
+	// yet, so their IDs can be given again
+	long long take(long long desired, bool joins_list) {
+		long long id = (desired >= 0 && !listed.count(desired)) ? desired : next;
+		if (joins_list) listed.insert(id);
 		if (id >= next) next = id + 1;
 		return id;
 	}
@@ -345,7 +347,7 @@ bool load_agent(umans_scenario& sc, const Node* e, IdAlloc& ids) {
 	const Node* ge = e->child("goal");
 	if (ge) { qfloat(ge, "x", gx); qfloat(ge, "y", gy); }
 	else sc.warnings.push_back("Agent " + std::to_string(id) + " has no goal element; it will not move");
-	sc.ids.push_back((int32_t)ids.take(id >= 0 ? id : -1));
+	sc.ids.push_back((int32_t)ids.take(id >= 0 ? id : -1, st <= 0.0f));   // time_ is 0 while loading
 	sc.pos.push_back(x); sc.pos.push_back(y);
 	sc.goal.push_back(gx); sc.goal.push_back(gy);
 	// Agent::setGoal looks at the goal unless it is the position (agent.cpp:120-127)
@@ -354,6 +356,7 @@ bool load_agent(umans_scenario& sc, const Node* e, IdAlloc& ids) {
 		const float dx = gx - x, dy = gy - y;
 		const float m = std::sqrt(dx * dx + dy * dy);
 		if (m > 0.0f) { vx = dx / m; vy = dy / m; }
+		else { vx = dx; vy = dy; }          // Vector2D::normalize leaves it (vector2D.h:55-63)
 	}
 	sc.viewdir.push_back(vx); sc.viewdir.push_back(vy);
 	sc.radius.push_back(rad);

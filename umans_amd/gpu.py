@@ -29,8 +29,11 @@ EXPORTS = [
     "umans_set_graph", "umans_autotune", "umans_set_tuning", "umans_adjoint_forward", "umans_adjoint_backward",
     "umans_cost_param_read_name", "umans_scenario_load", "umans_scenario_free", "umans_scenario_get_info",
     "umans_scenario_warning", "umans_scenario_get", "umans_scenario_obstacles", "umans_csv_start",
-    "umans_csv_flush", "umans_csv_stop",
+    "umans_csv_flush", "umans_csv_stop", "umans_add_agents", "umans_remove_agent", "umans_set_agent_goal",
+    "umans_set_agent_states", "umans_set_remove_at_goal", "umans_num_scheduled", "umans_download_records",
 ]
+
+ENOTFOUND = -2
 
 
 class WorldDesc(C.Structure):
@@ -132,6 +135,13 @@ def lib():
         L.umans_scenario_warning.restype = C.c_char_p
         L.umans_scenario_get.argtypes = [vp, C.POINTER(PolicyC), _I, C.POINTER(AgentsC), _F, _I]
         L.umans_scenario_obstacles.argtypes = [vp, _I, _F]
+        L.umans_add_agents.argtypes = [vp, C.POINTER(AgentsC), C.c_int, _F, C.POINTER(C.c_uint8), C.c_int, _I]
+        L.umans_remove_agent.argtypes = [vp, C.c_int32]
+        L.umans_set_agent_goal.argtypes = [vp, C.c_int32, C.c_float, C.c_float]
+        L.umans_set_agent_states.argtypes = [vp, C.c_int, _I, _F, _F, _F]
+        L.umans_set_remove_at_goal.argtypes = [vp, C.c_int, _I, C.POINTER(C.c_uint8)]
+        L.umans_num_scheduled.argtypes = [vp]
+        L.umans_download_records.argtypes = [vp, vp, C.c_int]
         L.umans_csv_start.argtypes = [vp, C.c_char_p, C.c_int]
         L.umans_csv_flush.argtypes = [vp]
         L.umans_csv_stop.argtypes = [vp]
@@ -182,7 +192,6 @@ class World:
         self.n_worlds = len(batch)
         self._check(L.umans_set_policies(self._h, (PolicyC * len(policies))(*[policy_struct(p) for p in policies]),
                                          len(policies)))
-        self.n = 0
 
     def _check(self, rc):
         if rc != 0:
@@ -200,22 +209,98 @@ class World:
         except Exception:
             pass
 
+    @property
+    def n(self):
+        """Agents in the list (WorldBase::GetAgents), scheduled ones not counted."""
+        return int(lib().umans_num_agents(self._h))
+
     @classmethod
     def from_scenario_file(cls, path, max_neighbors=0, device=0):
-        """A world loaded from a scenario file of the reference's XML schema, agents
-        uploaded. Obstacles, delayed insertion (start_time > 0) and remove_at_goal are
-        not on the GPU path: such scenarios are refused."""
+        """A world loaded from a scenario file of the reference's XML schema, in the
+        reference's load order: agents whose start time is 0 join the list in file
+        order, later ones are scheduled (CrowdSimulator::FromConfigFile ->
+        WorldBase::AddAgent, crowdSimulator.cpp:417-512); remove_at_goal is kept per
+        agent. Obstacles are not on the GPU path: such scenarios are refused."""
         sc, meta = load_scenario(path)
         if len(meta["obstacle_counts"]):
             raise UmansError(f"{path}: obstacles are not supported on the GPU path")
-        if (meta["start_time"] > 0).any():
-            raise UmansError(f"{path}: agents with a start_time are not supported on the GPU path")
-        if meta["remove_at_goal"].any():
-            raise UmansError(f"{path}: remove_at_goal is not supported on the GPU path")
         w = cls(sc.world_type, sc.width, sc.height, sc.dt, sc.policies, max_neighbors, device=device)
-        w.upload(sc.agents)
+        later = np.asarray(meta["start_time"], np.float32) > 0
+        rm = np.asarray(meta["remove_at_goal"]).astype(bool)
+        now = {k: (np.asarray(v)[~later] if v is not None else None) for k, v in sc.agents.items()}
+        w.upload(now)
+        if rm[~later].any():
+            w.set_remove_at_goal(now["id"], rm[~later])
+        if later.any():
+            sub = {k: (np.asarray(v)[later] if v is not None else None) for k, v in sc.agents.items()}
+            w.add_agents(sub, start_time=np.asarray(meta["start_time"], np.float32)[later], remove_at_goal=rm[later],
+                         keep_ids=True)
         w.scenario = sc
+        w.scenario_meta = meta
         return w
+
+    def add_agents(self, agents, start_time=None, remove_at_goal=None, keep_ids=False):
+        """WorldBase::AddAgent for each agent (dict of arrays; "id" optional = desired
+        IDs, "goal" optional = the position). keep_ids: the IDs are final (assigned by
+        the scenario loader). Returns the IDs given."""
+        n = len(agents["pos"])
+        keep = {}
+
+        def arr(k, t, dtype):
+            if k not in agents or agents[k] is None:
+                return None
+            keep[k] = np.ascontiguousarray(agents[k], dtype=dtype)
+            return _p(keep[k], t)
+        a = AgentsC(arr("id", _I, np.int32), arr("pos", _F, np.float32), arr("vel", _F, np.float32),
+                    arr("goal", _F, np.float32), arr("viewdir", _F, np.float32), arr("radius", _F, np.float32),
+                    arr("pref_speed", _F, np.float32), arr("max_speed", _F, np.float32),
+                    arr("max_accel", _F, np.float32), arr("policy", _I, np.int32), arr("rng", _U, np.uint32),
+                    None, None)
+        st = None if start_time is None else np.ascontiguousarray(start_time, np.float32)
+        rm = None if remove_at_goal is None else np.ascontiguousarray(remove_at_goal, np.uint8)
+        ids = np.zeros(n, np.int32)
+        self._check(lib().umans_add_agents(self._h, C.byref(a), n, None if st is None else _p(st, _F),
+                                           None if rm is None else rm.ctypes.data_as(C.POINTER(C.c_uint8)),
+                                           1 if keep_ids else 0, _p(ids, _I)))
+        return ids
+
+    def remove_agent(self, agent_id):
+        """WorldBase::RemoveAgent: False when no agent has that ID."""
+        rc = lib().umans_remove_agent(self._h, int(agent_id))
+        if rc == ENOTFOUND:
+            return False
+        self._check(rc)
+        return True
+
+    def set_agent_goal(self, agent_id, x, y):
+        rc = lib().umans_set_agent_goal(self._h, int(agent_id), float(x), float(y))
+        if rc == ENOTFOUND:
+            return False
+        self._check(rc)
+        return True
+
+    def set_agent_states(self, ids, pos, vel, viewdir):
+        ids = np.ascontiguousarray(ids, np.int32)
+        pos, vel, viewdir = (np.ascontiguousarray(x, np.float32) for x in (pos, vel, viewdir))
+        self._check(lib().umans_set_agent_states(self._h, len(ids), _p(ids, _I), _p(pos, _F), _p(vel, _F),
+                                                 _p(viewdir, _F)))
+
+    def set_remove_at_goal(self, ids, flags):
+        ids = np.ascontiguousarray(ids, np.int32)
+        f = np.ascontiguousarray(flags, np.uint8)
+        self._check(lib().umans_set_remove_at_goal(self._h, len(ids), _p(ids, _I),
+                                                   f.ctypes.data_as(C.POINTER(C.c_uint8))))
+
+    def records(self):
+        """The list as (n, 7) float32 view of packed records: id (int32 bits), position,
+        velocity, viewing direction."""
+        n = self.n
+        buf = np.zeros((n, 7), np.float32)
+        self._check(lib().umans_download_records(self._h, buf.ctypes.data_as(C.c_void_p), n))
+        return buf[:, 0].view(np.int32).copy(), buf[:, 1:3].copy(), buf[:, 3:5].copy(), buf[:, 5:7].copy()
+
+    def num_scheduled(self):
+        return int(lib().umans_num_scheduled(self._h))
 
     def upload(self, agents):
         n = len(agents["id"])
@@ -235,7 +320,6 @@ class World:
                     arr("policy", _I, np.int32), arr("rng", _U, np.uint32, False), None,
                     arr("world", _I, np.int32, False))
         self._check(lib().umans_upload(self._h, C.byref(a), n))
-        self.n = n
 
     def step(self, nsteps=1):
         self._check(lib().umans_step(self._h, int(nsteps)))
