@@ -20,7 +20,8 @@ LIB_PATH = os.path.join(HERE, "libumans_oracle.so")
 
 class World(C.Structure):
     _fields_ = [("world_type", C.c_int32), ("width", C.c_float), ("height", C.c_float),
-                ("delta_time", C.c_float), ("max_neighbors", C.c_int32), ("pad", C.c_int32)]
+                ("delta_time", C.c_float), ("max_neighbors", C.c_int32), ("n_edges", C.c_int32),
+                ("edges", C.POINTER(C.c_float))]
 
 
 class Policy(C.Structure):
@@ -78,8 +79,27 @@ def _ptr(a, t):
     return a.ctypes.data_as(t) if a is not None else None
 
 
-def make_world(world_type, width, height, dt, max_neighbors):
-    return World(int(world_type), float(width), float(height), float(dt), int(max_neighbors), 0)
+def make_world(world_type, width, height, dt, max_neighbors, edges=None):
+    """edges: optional (E, 4) float32 obstacle edges (ax, ay, bx, by) in polygon order."""
+    w = World(int(world_type), float(width), float(height), float(dt), int(max_neighbors), 0, None)
+    if edges is not None and len(edges):
+        e = np.ascontiguousarray(edges, np.float32).reshape(-1, 4)
+        w._edges = e
+        w.n_edges = len(e)
+        w.edges = e.ctypes.data_as(C.POINTER(C.c_float))
+    return w
+
+
+def polygon_edges(counts, points):
+    """Polygon2D edges (Polygon2D.cpp:27-38): vertex i to vertex (i + 1) mod n, per polygon."""
+    out, o = [], 0
+    pts = np.asarray(points, np.float32).reshape(-1, 2)
+    for c in counts:
+        for i in range(c):
+            a, b = pts[o + i], pts[o + (i + 1) % c]
+            out.append([a[0], a[1], b[0], b[1]])
+        o += c
+    return np.asarray(out, np.float32).reshape(-1, 4)
 
 
 def make_policies(packed):

@@ -79,9 +79,19 @@ def policy_xml(pid, pol):
     return "  <Policy " + " ".join(attrs) + ">\n" + "\n".join(body) + "\n  </Policy>"
 
 
-def scenario_xml(world_type, width, height, dt, policies):
-    w = ('<World type="Toric" width="%r" height="%r" />' % (float(width), float(height))
-         if world_type == 1 else '<World type="Infinite" />')
+def obstacles_xml(polygons):
+    """<Obstacles> block (crowdSimulator.cpp:496-514): one <Obstacle> of <Point>s each."""
+    out = []
+    for poly in polygons:
+        pts = "".join('<Point x="%r" y="%r"/>' % (float(x), float(y)) for x, y in poly)
+        out.append("<Obstacle>" + pts + "</Obstacle>")
+    return "<Obstacles>" + "".join(out) + "</Obstacles>"
+
+
+def scenario_xml(world_type, width, height, dt, policies, obstacles=None):
+    inner = obstacles_xml(obstacles) if obstacles else ""
+    w = ('<World type="Toric" width="%r" height="%r">%s</World>' % (float(width), float(height), inner)
+         if world_type == 1 else '<World type="Infinite">%s</World>' % inner)
     pols = "\n".join(policy_xml(i, p) for i, p in enumerate(policies))
     return (f'<?xml version="1.0" encoding="utf-8"?>\n<Simulation delta_time="{dt!r}">\n  {w}\n'
             f'  <Policies>\n{pols}\n  </Policies>\n</Simulation>\n')
@@ -90,8 +100,8 @@ def scenario_xml(world_type, width, height, dt, policies):
 class RefSim:
     """One reference CrowdSimulator with agents added programmatically."""
 
-    def __init__(self, world_type, width, height, dt, policies, agents, threads=1, xml_path=None):
-        xml = scenario_xml(world_type, width, height, float(np.float32(dt)), policies)
+    def __init__(self, world_type, width, height, dt, policies, agents, threads=1, xml_path=None, obstacles=None):
+        xml = scenario_xml(world_type, width, height, float(np.float32(dt)), policies, obstacles)
         fd, path = tempfile.mkstemp(suffix=".xml")
         with os.fdopen(fd, "w") as f:
             f.write(xml)

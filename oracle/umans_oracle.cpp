@@ -81,10 +81,10 @@ template <typename T> V2<T> clamp_vec(V2<T> v, T maxLength) {
 	if (m <= maxLength) return v;
 	return v.normalized() * maxLength;
 }
-// vector2D.h:249-272 (all double)
+// vector2D.h:249-272: coordinate differences in the vector type, the rest in double
 template <typename T> bool line_intersection(V2<T> a, V2<T> b, V2<T> c, V2<T> d, V2<T>& res) {
-	double a1 = (double)b.y - a.y, b1 = (double)a.x - b.x, c1 = a1 * a.x + b1 * a.y;
-	double a2 = (double)d.y - c.y, b2 = (double)c.x - d.x, c2 = a2 * c.x + b2 * c.y;
+	double a1 = (double)(b.y - a.y), b1 = (double)(a.x - b.x), c1 = a1 * a.x + b1 * a.y;
+	double a2 = (double)(d.y - c.y), b2 = (double)(c.x - d.x), c2 = a2 * c.x + b2 * c.y;
 	double det = a1 * b2 - a2 * b1;
 	if (std::fabs(det) < 0.00001) return false;
 	res = V2<T>((T)((b2 * c1 - b1 * c2) / det), (T)((a1 * c2 - a2 * c1) / det));
@@ -101,6 +101,25 @@ template <typename T> struct Agent {
 };
 
 // PhantomAgent (worldBase.h:40-79): real agent, offset position, float distSqr
+template <typename T> struct Seg { V2<T> a, b; };   // LineSegment2D
+
+// nearestPointOnLine (vector2D.h:274-296); on_seg reports 0 <= k <= 1 of the unclamped k
+template <typename T> V2<T> nearest_point(V2<T> pt, V2<T> la, V2<T> lb, bool onSegment, bool* on_seg = nullptr) {
+	if (la.x == lb.x && la.y == lb.y) return la;
+	const V2<T> line = la - lb;
+	T k = (pt - lb).dot(line) / line.sqr();
+	if (onSegment) {
+		if (k > 1) k = 1;
+		else if (k < 0) k = 0;
+	}
+	if (on_seg) *on_seg = (k >= 0 && k <= 1);
+	return (la * k) + (lb * (T(1) - k));
+}
+// distanceToLineSquared (vector2D.h:303-306)
+template <typename T> T dist_line_sq(V2<T> pt, V2<T> la, V2<T> lb, bool onSegment) {
+	return (pt - nearest_point(pt, la, lb, onSegment)).sqr();
+}
+
 template <typename T> struct Phantom {
 	int idx;
 	V2<T> pos;
@@ -216,6 +235,38 @@ static void compute_neighbors(const oracle_world& w, const CellGrid* grid, const
 	if (full_count) *full_count = total;
 }
 
+// WorldBase::computeNeighboringObstacles_Flat (worldBase.cpp:62-77) per query segment of
+// WorldInfinite / WorldToric::ComputeNeighbors (worldToric.cpp:32-79): every edge whose
+// distance to float(position + displacement) is within the range (float arithmetic, as
+// in the reference in both precisions), moved back by the displacement.
+template <typename T>
+static void compute_obstacles(const oracle_world& w, float px, float py, float range, std::vector<Seg<T>>& out) {
+	out.clear();
+	if (w.n_edges <= 0) return;
+	float disp[5][2] = {{0, 0}, {w.width, 0}, {-w.width, 0}, {0, w.height}, {0, -w.height}};
+	bool use[5] = {true, false, false, false, false};
+	if (w.world_type == 1) {
+		use[1] = px - range < -0.5 * w.width;
+		use[2] = px + range > 0.5 * w.width;
+		use[3] = py - range < -0.5 * w.height;
+		use[4] = py + range > 0.5 * w.height;
+	}
+	const float r2 = range * range;
+	for (int s = 0; s < 5; ++s) {
+		if (!use[s]) continue;
+		const V2<float> q(px + disp[s][0], py + disp[s][1]);
+		for (int e = 0; e < w.n_edges; ++e) {
+			const float* E = w.edges + 4 * e;
+			const V2<float> a(E[0], E[1]), b(E[2], E[3]);
+			if (dist_line_sq(q, a, b, true) <= r2) {
+				const V2<float> d(disp[s][0], disp[s][1]);
+				const V2<float> a2 = a - d, b2 = b - d;
+				out.push_back(Seg<T>{V2<T>((T)a2.x, (T)a2.y), V2<T>((T)b2.x, (T)b2.y)});
+			}
+		}
+	}
+}
+
 // --------------------------------------------------------- shared numerics
 struct Params { const oracle_policy* pol; int c; };
 
@@ -225,7 +276,9 @@ template <typename T> struct AgentView {
 	const std::vector<Phantom<T>>* nbs;
 	T dt;
 	float relaxation;   // Policy::getRelaxationTime
+	const std::vector<Seg<T>>* obs;   // neighbouring obstacle edges (NeighborList::second)
 };
+
 
 // CostFunction::SolveQuadraticEquation (costFunction.cpp:173-191), incl. the
 // D == 0 branch that evaluates (-b/2)*a.
@@ -255,7 +308,33 @@ template <typename T> T time_to_collision(V2<T> p1, V2<T> v1, T r1, V2<T> p2, V2
 	if (t2 < 0) t2 = (T)kMaxFloat;
 	return std::min(t1, t2);
 }
-// CostFunction::ComputeTimeToFirstCollision (costFunction.cpp:229-268), agents only
+// CostFunction::ComputeTimeToCollision_LineSegmentInterior (costFunction.cpp:293-325)
+template <typename T> T ttc_segment_interior(V2<T> pos, V2<T> vel, T radius, const Seg<T>& sg) {
+	if (dist_line_sq(pos, sg.a, sg.b, true) <= radius * radius) return T(0);
+	V2<T> X;
+	if (!line_intersection(pos, pos + vel, sg.a, sg.b, X)) return (T)kMaxFloat;
+	if (vel.dot(X - pos) < 0) return (T)kMaxFloat;
+	bool on = false;
+	nearest_point(X, sg.a, sg.b, false, &on);
+	if (!on) return (T)kMaxFloat;
+	const T speed = vel.mag();
+	const T ttc = (X - pos).mag() / speed;
+	const T lineAngle = angle_between(vel, sg.b - sg.a);
+	const T sub = radius / std::sin(lineAngle);
+	const T tsub = sub / speed;
+	return std::max(T(0), ttc - tsub);
+}
+// CostFunction::ComputeTimeToCollision_LineSegment (costFunction.cpp:270-291)
+template <typename T> T ttc_segment(V2<T> pos, V2<T> vel, T radius, const Seg<T>& sg) {
+	const T t1 = time_to_collision(pos, vel, radius, sg.a, V2<T>(), T(0));
+	if (t1 == 0) return T(0);
+	if (sg.a.x == sg.b.x && sg.a.y == sg.b.y) return t1;
+	const T t2 = time_to_collision(pos, vel, radius, sg.b, V2<T>(), T(0));
+	if (t2 == 0) return T(0);
+	const T ti = ttc_segment_interior(pos, vel, radius, sg);
+	return std::min(std::min(t1, t2), ti);
+}
+// CostFunction::ComputeTimeToFirstCollision (costFunction.cpp:229-268)
 template <typename T>
 T time_to_first_collision(const AgentView<T>& av, V2<T> pos, V2<T> vel, T radius, T maxDist, bool ignoreCurrent) {
 	T minTTC = (T)kMaxFloat;
@@ -266,6 +345,14 @@ T time_to_first_collision(const AgentView<T>& av, V2<T> pos, V2<T> vel, T radius
 		T ttc = time_to_collision(pos, vel, radius, nb.pos, o.vel, o.radius);
 		if (ignoreCurrent && ttc == 0) continue;
 		if (ttc < minTTC) minTTC = ttc;
+	}
+	if (av.obs) {
+		for (const Seg<T>& sg : *av.obs) {
+			if (dist_line_sq(pos, sg.a, sg.b, true) > maxDistSq) continue;
+			T ttc = ttc_segment(pos, vel, radius, sg);
+			if (ignoreCurrent && ttc == 0) continue;
+			if (ttc < minTTC) minTTC = ttc;
+		}
 	}
 	return minTTC;
 }
@@ -335,8 +422,22 @@ template <typename T> V2<T> social_force_pair(const Params& pp, const Agent<T>& 
 	return scale * F;
 }
 
+// SocialForcesAvoidance::ComputeObstacleInteractionForce (SocialForcesAvoidance.cpp:112-140).
+// U0 keeps its default 10: parseParameters reads the "U0" attribute into V0
+// (SocialForcesAvoidance.cpp:142-150); R is parameter slot 4.
+template <typename T> V2<T> social_force_obstacle(const Params& pp, const Agent<T>& a, const Seg<T>& sg) {
+	const T U0 = T(10.0f), R = P<T>(pp, 4);
+	const V2<T> nearest = nearest_point(a.pos, sg.a, sg.b, true);
+	const V2<T> diff = a.pos - nearest;
+	const T dist = diff.mag();
+	if (dist < (T)0.001f) return V2<T>();
+	const T d_U = -U0 / R * std::exp(-dist / R);
+	return -diff * d_U;
+}
+
 // ObjectInteractionForces::ComputeForce (ObjectInteractionForces.cpp:26-48) and
-// GoalReachingForce::ComputeForce (GoalReachingForce.cpp:26-29)
+// GoalReachingForce::ComputeForce (GoalReachingForce.cpp:26-29); PowerLaw's obstacle
+// force is zero (PowerLaw.cpp:68-72)
 template <typename T> V2<T> force_of(const Params& pp, const AgentView<T>& av) {
 	const Agent<T>& a = *av.self;
 	const int kind = pp.pol->cost_kind[pp.c];
@@ -350,7 +451,14 @@ template <typename T> V2<T> force_of(const Params& pp, const AgentView<T>& av) {
 			F = F + (kind == ORC_POWER_LAW ? power_law_pair(pp, a, nb, o) : social_force_pair(pp, a, nb, o));
 		}
 	}
-	return F + V2<T>();
+	V2<T> Fo;
+	if (av.obs) {
+		for (const Seg<T>& sg : *av.obs) {
+			if (dist_line_sq(a.pos, sg.a, sg.b, true) <= rangeSq)
+				Fo = Fo + (kind == ORC_SOCIAL_FORCES ? social_force_obstacle(pp, a, sg) : V2<T>());
+		}
+	}
+	return F + Fo;
 }
 // ForceBasedFunction::ComputeTargetVelocity (ForceBasedFunction.cpp:49-52)
 template <typename T> V2<T> target_velocity(const Params& pp, const AgentView<T>& av) {
@@ -890,6 +998,8 @@ int step_impl(const oracle_world& w, const oracle_policy* pols, int npol, int n,
 		std::vector<Phantom<T>> nbs;
 		int full = 0;
 		compute_neighbors<T>(w, &grid, in.pos, in.id, n, i, interaction_range(pol), nbs, &full, ag);
+		std::vector<Seg<T>> obs;
+		compute_obstacles<T>(w, in.pos[2 * i], in.pos[2 * i + 1], interaction_range(pol), obs);
 		if (out.nb_count) {
 			const int m = std::min((int)nbs.size(), out.list_stride);
 			out.nb_count[i] = (int)nbs.size();
@@ -901,7 +1011,7 @@ int step_impl(const oracle_world& w, const oracle_policy* pols, int npol, int n,
 		}
 		if (out.nb_full) out.nb_full[i] = full;
 
-		AgentView<T> av{&a, &ag, &nbs, dt, pol.relaxation_time};
+		AgentView<T> av{&a, &ag, &nbs, dt, pol.relaxation_time, &obs};
 		uint32_t rng = a.rng;
 		V2<T> next;
 		// Policy::ComputeNewVelocity (policy.cpp:43-76)
@@ -934,12 +1044,17 @@ int step_impl(const oracle_world& w, const oracle_policy* pols, int npol, int n,
 			acc = clamp_vec(acc, a.max_accel);
 			next = clamp_vec(a.vel + acc * dt, a.max_speed);
 		}
-		// Policy::ComputeContactForces (policy.cpp:107-141), agents only
+		// Policy::ComputeContactForces (policy.cpp:107-141)
 		V2<T> contact;
 		if (pol.contact_force_scale > 0) {
 			for (const auto& nb : nbs) {
 				const V2<T> diff = a.pos - nb.pos;
 				const T inter = a.radius + ag[nb.idx].radius - diff.mag();
+				if (inter > 0) contact = contact + diff.normalized() * ((T)pol.contact_force_scale * inter);
+			}
+			for (const Seg<T>& sg : obs) {
+				const V2<T> diff = a.pos - nearest_point(a.pos, sg.a, sg.b, true);
+				const T inter = a.radius - diff.mag();
 				if (inter > 0) contact = contact + diff.normalized() * ((T)pol.contact_force_scale * inter);
 			}
 		}

@@ -39,7 +39,8 @@ typedef struct {
 	float width, height;     /* toric size */
 	float delta_time;
 	int32_t max_neighbors;   /* cap k on the canonical list; <= 0: no cap */
-	int32_t pad;
+	int32_t n_edges;         /* obstacle edges (Polygon2D::GetEdges order), may be 0 */
+	const float* edges;      /* n_edges x (ax, ay, bx, by) */
 } oracle_world;
 
 typedef struct {

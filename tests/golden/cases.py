@@ -32,3 +32,21 @@ CASES = {
     "multi_cost_global": ("global", [("ORCA", 1.0, {"range": 5}), ("GoalReachingForce", 0.5, {})], None, 0.5),
     "goal_only_gradient": ("gradient", [("GoalReachingForce", 1.0, {})], None, 0.5),
 }
+
+# Cases re-run with obstacles (fixtures obst_<case>_w<world>): every model that reads
+# obstacle edges (time-to-collision users, SocialForces' obstacle force) and two that
+# see obstacles only through the contact forces.
+OBSTACLE_CASES = ["karamouzas_sampling", "karamouzas_global", "rvo_random", "moussaid_sampling",
+                  "vantoll_sampling", "vantoll_gradient", "socialforces_gradient", "socialforces_r",
+                  "powerlaw_gradient", "orca_global", "paris_sampling", "ttcadca_sampling"]
+CASES["socialforces_r"] = ("gradient", [("SocialForcesAvoidance", 1.0, {"range": 5, "R": 0.5}),
+                                        ("GoalReachingForce", 1.0, {})], None, 0.5)
+
+# Polygons (vertex lists) inside the 12 m test world: a square, a triangle, a wall given
+# as a two-point polygon (two coincident edges), a one-point polygon (an edge of zero
+# length) and a wall next to the x seam of the torus.
+OBSTACLES = [[(-1.0, -1.0), (1.0, -1.0), (1.0, 1.0), (-1.0, 1.0)],
+             [(3.0, 3.0), (5.0, 3.0), (4.0, 5.0)],
+             [(-5.0, -3.0), (-2.0, -4.5)],
+             [(-4.0, 3.0)],
+             [(5.6, -2.0), (5.9, 2.0)]]

@@ -90,6 +90,7 @@ def main():
     written += examples()
     written += csv_output()
     written += dynamic_population()
+    written += obstacle_cases()
     print(f"wrote {written} fixtures to {HERE}")
 
 
@@ -243,6 +244,29 @@ def dynamic_population():
     rec = E.api_sequence(L, path, 7)
     np.savez_compressed(os.path.join(HERE, "api_Mixed.npz"), **E.flatten(rec))
     return 2
+
+
+def obstacle_cases():
+    """The policy cases of OBSTACLE_CASES in a world with the OBSTACLES polygons, both
+    world types: states and neighbour lists over STEPS reference steps, plus the edges."""
+    from cases import OBSTACLE_CASES, OBSTACLES
+    import oracle_py as O
+    edges = O.polygon_edges([len(p) for p in OBSTACLES], [xy for p in OBSTACLES for xy in p])
+    n = 0
+    for world in (0, 1):
+        for name in OBSTACLE_CASES:
+            method, costs, samp, relax = CASES[name]
+            xmlp = to_reference_xml_policy(method, costs, sampling=samp, relaxation_time=relax)
+            sc = S.random_cloud(N, SIZE, [None], world_type=world, seed=23 + world)
+            sim = R.RefSim(world, SIZE, SIZE, sc.dt, [xmlp], sc.agents, threads=1, obstacles=OBSTACLES)
+            data = record(sim, STEPS)
+            sim.close()
+            data.update(agent_arrays(sc))
+            data["world"] = np.array([world, SIZE, SIZE, sc.dt], np.float64)
+            data["obstacle_edges"] = edges
+            np.savez_compressed(os.path.join(HERE, f"obst_{name}_w{world}.npz"), **data)
+            n += 1
+    return n
 
 
 if __name__ == "__main__":

@@ -37,6 +37,8 @@ def load(name):
         packed = [pack_policy("global", [("ORCA", 1.0, {"timeHorizon": 5.0, "range": 8.0})])]
     else:
         case = name.rsplit("_w", 1)[0]
+        if case.startswith("obst_"):
+            case = case[len("obst_"):]
         m, costs, samp, relax = CASES[case]
         packed = [pack_policy(m, costs, sampling=samp, relaxation_time=relax)]
     return d, agents, packed, int(w[0]), float(w[1]), float(w[2]), float(w[3])

@@ -13,7 +13,7 @@ NAMES = fixture_names()
 @pytest.mark.parametrize("name", NAMES)
 def test_oracle_matches_reference(name):
     d, agents, packed, world, w, h, dt = load(name)
-    wd = O.make_world(world, w, h, dt, 0)
+    wd = O.make_world(world, w, h, dt, 0, edges=d.get("obstacle_edges"))
     rng = np.array([O.rng_seed(i) for i in agents["id"]], np.uint32)
     steps = d["nb_count"].shape[0]
     for s in range(steps):
@@ -74,3 +74,17 @@ def test_oracle_on_reference_examples(name):
             assert tie_equal(rid, out["nb_id"][i, :out["nb_count"][i]], out["nb_d2"][i])
         np.testing.assert_allclose(out["vel"], d["state_vel"][s + 1], rtol=1e-6, atol=1e-6)
         np.testing.assert_allclose(out["pos"], d["state_pos"][s + 1], rtol=1e-6, atol=1e-6)
+
+
+@pytest.mark.parametrize("name", [n for n in NAMES if n.startswith("obst_")])
+def test_obstacle_fixtures_depend_on_obstacles(name):
+    """The obstacle fixtures exercise the obstacle code: the same step without the edges
+    gives a different result for some agents (so the bit-exact match above pins the
+    obstacle search, the segment time-to-collision, SocialForces' obstacle force and the
+    obstacle contact forces, whichever the case uses)."""
+    d, agents, packed, world, w, h, dt = load(name)
+    wd = O.make_world(world, w, h, dt, 0)
+    a = state_at(agents, d, 0)
+    a["rng"] = np.array([O.rng_seed(i) for i in agents["id"]], np.uint32)
+    out = O.step(wd, packed, a, precision=0)
+    assert (out["vel"] != d["state_vel"][1]).any(axis=1).sum() >= 3
