@@ -10,8 +10,9 @@
  *
  * Differences, each a limit of the GPU path:
  *   - nrThreads is ignored (no host threads take part in a step);
- *   - scenarios with obstacles are refused (StartSimulation returns false and prints
- *     why), as are cost functions the GPU path does not implement;
+ *   - scenarios with cost functions the GPU path does not implement are refused
+ *     (StartSimulation returns false and prints why); an agent keeps at most 32
+ *     obstacle edges in range;
  *   - AddAgent returns false for a policy ID the scenario does not define (the
  *     reference would add an agent without a policy);
  *   - the GPU is device 0, or the one named by the environment variable UMANS_DEVICE.

@@ -114,6 +114,7 @@ typedef struct {
 	int64_t overflow_agents;   /* agents whose neighbour list hit the cap, last step */
 	int64_t max_list;          /* longest uncapped neighbour list, last step */
 	double sim_time;           /* WorldBase::time_ */
+	int64_t obstacle_overflow_agents;   /* agents with more obstacle edges in range than kept, last step */
 } umans_stats;
 
 typedef struct umans_ctx umans_ctx;
@@ -202,6 +203,16 @@ typedef struct {
 	float viewdir_x, viewdir_y;
 } umans_agent_record;
 int umans_download_records(umans_ctx* ctx, umans_agent_record* out, int capacity);
+
+/* Obstacles of a single-world context (UMANS_EINVAL for a batch): polygons given as
+ * vertex counts and interleaved x,y vertices, as <Obstacle><Point .../></Obstacle>
+ * (crowdSimulator.cpp:496-514); each polygon contributes the edges vertex i -> vertex
+ * (i + 1) mod n (Polygon2D.cpp:27-38). Replaces any previous obstacles; n_polygons 0
+ * removes them. Every agent within range of an edge sees it: time-to-collision models,
+ * SocialForcesAvoidance's obstacle force and the contact forces (costFunction.cpp:229-325,
+ * SocialForcesAvoidance.cpp:112-140, policy.cpp:126-137). Up to 32 edges per agent are
+ * kept (umans_stats.obstacle_overflow_agents counts agents with more). */
+int umans_set_obstacles(umans_ctx* ctx, int n_polygons, const int32_t* counts, const float* points);
 
 /* CSV trajectory output: CrowdSimulator::StartCSVOutput (crowdSimulator.cpp:37-51) and
  * CSVWriter (csvwriter.cpp:37-87). After every step the positions are copied

@@ -8,9 +8,13 @@ REL_TOL = 1e-5        # relative velocity error bar against the FP64 oracle
 FLOOR = 1e-3          # |v| floor of the relative error (velocities can be 0 or clamped)
 
 
-def gpu_step(world_type, w, h, dt, packed, agents, max_neighbors=0, steps=1, record=True, cell_size=0.0):
+def gpu_step(world_type, w, h, dt, packed, agents, max_neighbors=0, steps=1, record=True, cell_size=0.0,
+             obstacles=None):
+    """obstacles: optional list of polygons (vertex lists)."""
     W = gpu.World(world_type, w, h, dt, packed, max_neighbors=max_neighbors, cell_size=cell_size)
     try:
+        if obstacles:
+            W.set_obstacles([len(p) for p in obstacles], np.array([xy for p in obstacles for xy in p], np.float32))
         W.upload(agents)
         if record:
             W.record_neighbors(True)

@@ -8,13 +8,16 @@
   (a discrete choice that FP32 rounding decides), in which case the GPU must equal the
   FP32 result;
 * neighbour lists bit-exact against the oracle at 10k agents with and without a cap,
-  in infinite and toric worlds (query disks crossing the seams).
+  in infinite and toric worlds (query disks crossing the seams);
+* the obst_* fixtures repeat the cases whose models read obstacle edges in a world with
+  polygons (tests/golden/cases.py OBSTACLES), toric seams included.
 """
 import numpy as np
 import pytest
 
 import oracle_py as O
 from goldens import example_names, fixture_names, load, load_example, neighbor_lists, state_at
+from cases import OBSTACLES
 from gpu_util import REL_TOL, gpu_step, rel_err
 from umans_amd import scenarios as S
 
@@ -28,16 +31,17 @@ def test_gpu_matches_reference_fixture(name):
     d, agents, packed, world, w, h, dt = load(name)
     steps = d["nb_count"].shape[0]
     rng = np.array([O.rng_seed(i) for i in agents["id"]], np.uint32)
+    obstacles = OBSTACLES if name.startswith("obst_") else None
     for s in range(steps):
         a = state_at(agents, d, s)
         a["rng"] = rng
-        out, (cnt, ids, d2, seg), st = gpu_step(world, w, h, dt, packed, a)
+        out, (cnt, ids, d2, seg), st = gpu_step(world, w, h, dt, packed, a, obstacles=obstacles)
         ref_nb = neighbor_lists(d, s)
         for i, (rid, _, _) in enumerate(ref_nb):
             assert cnt[i] == len(rid), (name, s, i)
             np.testing.assert_array_equal(ids[i, :len(rid)], rid, err_msg=f"{name} step {s} agent {i}")
         # velocities against the reference FP32 step and the FP64 oracle
-        o64 = O.step(O.make_world(world, w, h, dt, 0), packed, a, precision=1)
+        o64 = O.step(O.make_world(world, w, h, dt, 0, edges=d.get("obstacle_edges")), packed, a, precision=1)
         ref32 = d["state_vel"][s + 1]
         e64 = rel_err(out["vel"], o64["vel64"])
         e32 = rel_err(out["vel"], ref32)

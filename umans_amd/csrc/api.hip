@@ -154,7 +154,9 @@ struct umans_ctx {
 	GridInfo* dgrids = nullptr;
 	int* dbbox = nullptr;
 	bool any_infinite = false;
-	unsigned long long* dstats = nullptr;
+	unsigned long long* dstats = nullptr;   // 4 counters, see StepArgs::stats
+	float4* d_edges = nullptr;                // obstacle edges (single world)
+	int n_edges = 0;
 	int kcap = 0;
 	// policy classes: one step-kernel instance per class over a bucket of agents
 	std::vector<int> h_policy;           // policy of each uploaded agent
@@ -505,7 +507,8 @@ int setup_worlds(umans_ctx* c, const std::vector<int>& per_world) {
 // [cls_off[q], cls_off[q] + cls_count[q]).
 int update_classes(umans_ctx* c) {
 	for (int q = 0; q < NUM_CLS; ++q) c->cls_count[q] = 0;
-	for (int p : c->h_policy) c->cls_count[c->hpol[p].cls]++;
+	// with obstacles every agent runs the generic class, the only one that reads edges
+	for (int p : c->h_policy) c->cls_count[c->n_edges > 0 ? CLS_GENERIC : c->hpol[p].cls]++;
 	int off = 0, slot = 0;
 	for (int q = 0; q < NUM_CLS; ++q) {
 		c->cls_off[q] = off;
@@ -632,7 +635,7 @@ int enqueue_step(umans_ctx* c, bool events, const State* ysave = nullptr, unsign
 		hipEventRecord(e0, st);
 	}
 	hipLaunchKernelGGL(k_fill, dim3(ceil_div(scan_n, 256)), dim3(256), 0, st, c->cell_count, scan_n, 0u);
-	hipLaunchKernelGGL(k_fill, dim3(1), dim3(64), 0, st, (unsigned*)c->dstats, 4, 0u);
+	hipLaunchKernelGGL(k_fill, dim3(1), dim3(64), 0, st, (unsigned*)c->dstats, 8, 0u);
 	if (c->any_infinite) {
 		hipLaunchKernelGGL(k_bbox_init, dim3(ceil_div(nw, 256)), dim3(256), 0, st, c->dbbox, nw);
 		hipLaunchKernelGGL(k_bbox, dim3(ceil_div(n, 256)), dim3(256), 0, st, c->X.s.pos, c->X.s.meta, c->dworlds, n,
@@ -670,6 +673,8 @@ int enqueue_step(umans_ctx* c, bool events, const State* ysave = nullptr, unsign
 		a.raw_stride = kmax_bucket(c->kcap);
 	}
 	a.stats = c->dstats;
+	a.edges = c->d_edges;
+	a.n_edges = c->n_edges;
 	const int kmax = kmax_bucket(c->kcap);
 	const int bs = c->block > 0 ? c->block : default_block(kmax);
 	if (e1) hipEventRecord(e1, st);
@@ -1041,7 +1046,7 @@ int create_impl(const umans_world_desc* descs, int nworlds, umans_ctx** out) {
 		             hipEventCreateWithFlags(&c->join_ev[q], hipEventDisableTiming) == hipSuccess;
 	if (!streams_ok || hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess ||
 	    dalloc(&c->dgrids, nworlds) != hipSuccess || dalloc(&c->dworlds, nworlds) != hipSuccess ||
-	    dalloc(&c->dbbox, 4 * nworlds) != hipSuccess || dalloc(&c->dstats, 2) != hipSuccess ||
+	    dalloc(&c->dbbox, 4 * nworlds) != hipSuccess || dalloc(&c->dstats, 4) != hipSuccess ||
 	    dalloc(&c->dpol, kMaxPolicies) != hipSuccess || dalloc(&c->d_cls_slot, NUM_CLS) != hipSuccess) {
 		umans_destroy(c);
 		return UMANS_EHIP;
@@ -1071,6 +1076,7 @@ void umans_destroy(umans_ctx* c) {
 	c->free_pop_buffers();
 	c->free_agents();
 	hipFree(c->dgrids); hipFree(c->dworlds); hipFree(c->dbbox); hipFree(c->dstats); hipFree(c->dpol);
+	hipFree(c->d_edges);
 	hipFree(c->d_cls_slot);
 	hipFree(c->cell_count); hipFree(c->cell_start); hipFree(c->scan_sums);
 	c->free_events();
@@ -1400,7 +1406,7 @@ int umans_synchronize(umans_ctx* c) {
 int umans_get_stats(umans_ctx* c, umans_stats* out) {
 	if (!c || !out) return UMANS_EINVAL;
 	hipSetDevice(c->device);
-	unsigned long long h[2] = {0, 0};
+	unsigned long long h[3] = {0, 0, 0};
 	int rc = c->hip(hipMemcpyAsync(h, c->dstats, sizeof(h), hipMemcpyDeviceToHost, c->stream), "stats");
 	if (rc) return rc;
 	if ((rc = c->hip(hipStreamSynchronize(c->stream), "stats"))) return rc;
@@ -1408,6 +1414,7 @@ int umans_get_stats(umans_ctx* c, umans_stats* out) {
 	out->overflow_agents = (int64_t)h[0];
 	out->max_list = (int64_t)h[1];
 	out->sim_time = c->sim_time;
+	out->obstacle_overflow_agents = (int64_t)h[2];
 	return UMANS_OK;
 }
 
@@ -1484,6 +1491,7 @@ int umans_adjoint_forward(umans_ctx* c, const umans_adjoint_desc* d, const float
 	if (c->n == 0) return c->fail(UMANS_ESTATE, "upload agents first");
 	if (!c->pending.empty() || c->n_remove > 0)
 		return c->fail(UMANS_ESTATE, "the adjoint needs a fixed population (no scheduled agents, no remove_at_goal)");
+	if (c->n_edges > 0) return c->fail(UMANS_ESTATE, "the adjoint does not differentiate obstacle forces");
 	if (d->nsteps < 1 || d->checkpoint_every < 1) return c->fail(UMANS_EINVAL, "bad step or checkpoint count");
 	if (d->param_kind != UMANS_COST_SOCIAL_FORCES && d->param_kind != UMANS_COST_POWER_LAW)
 		return c->fail(UMANS_EINVAL, "parameters are differentiated for SocialForcesAvoidance or PowerLaw");
@@ -1781,6 +1789,41 @@ int umans_set_remove_at_goal(umans_ctx* c, int n, const int32_t* ids, const uint
 }
 
 int umans_num_scheduled(const umans_ctx* c) { return c ? (int)c->pending.size() : 0; }
+
+int umans_set_obstacles(umans_ctx* c, int n_polygons, const int32_t* counts, const float* points) {
+	if (!c) return UMANS_EINVAL;
+	if (c->worlds.size() != 1) return c->fail(UMANS_EINVAL, "obstacles need a single-world context");
+	if (n_polygons < 0 || (n_polygons > 0 && (!counts || !points))) return c->fail(UMANS_EINVAL, "bad polygon arrays");
+	// Polygon2D::Polygon2D (Polygon2D.cpp:27-38): edge i joins vertex i and vertex (i + 1) mod n
+	std::vector<float4> edges;
+	long long o = 0;
+	for (int p = 0; p < n_polygons; ++p) {
+		const int m = counts[p];
+		if (m < 0) return c->fail(UMANS_EINVAL, "negative vertex count");
+		for (int i = 0; i < m; ++i) {
+			const float* a = points + 2 * (o + i);
+			const float* b = points + 2 * (o + (i + 1) % m);
+			edges.push_back(make_float4(a[0], a[1], b[0], b[1]));
+		}
+		o += m;
+	}
+	if (edges.size() > (size_t)(1 << 24)) return c->fail(UMANS_EINVAL, "too many obstacle edges");
+	hipSetDevice(c->device);
+	int rc = c->hip(hipStreamSynchronize(c->stream), "obstacles");
+	if (rc) return rc;
+	hipFree(c->d_edges);
+	c->d_edges = nullptr;
+	c->n_edges = 0;
+	if (!edges.empty()) {
+		if ((rc = c->hip(dalloc(&c->d_edges, edges.size()), "alloc obstacles"))) return rc;
+		if ((rc = c->hip(hipMemcpy(c->d_edges, edges.data(), edges.size() * sizeof(float4), hipMemcpyHostToDevice),
+		                 "obstacles"))) return rc;
+		c->n_edges = (int)edges.size();
+	}
+	c->drop_graph();
+	c->adj_ready = false;
+	return update_classes(c);
+}
 
 int umans_download_records(umans_ctx* c, umans_agent_record* out, int capacity) {
 	static_assert(sizeof(umans_agent_record) == sizeof(AgentRecord), "record layout");

@@ -29,6 +29,10 @@ struct NbList {
 	const float2* pref;
 	float W, H;
 	V2 self;
+	// neighbouring obstacle edges (NeighborList::second): (ax, ay, bx, by), already moved
+	// back by their query displacement; only the generic class collects them
+	const float4* obs;
+	int n_obs;
 
 	UD unsigned raw(int e) const { return list[e * stride]; }
 	UD Nb get(int e) const {
@@ -89,7 +93,53 @@ UD float time_to_collision(V2 p1, V2 v1, float r1, V2 p2, V2 v2_, float r2) {
 	if (t2 < 0.0f) t2 = kMaxFloat;
 	return smin(t1, t2);
 }
-// CostFunction::ComputeTimeToFirstCollision (costFunction.cpp:229-268), agents only
+// ------------------------------------------------------------ obstacle edges
+// nearestPointOnLine (vector2D.h:274-296); on_seg reports 0 <= k <= 1 of the unclamped k
+UD V2 nearest_point(V2 pt, V2 la, V2 lb, bool on_segment, bool* on_seg = nullptr) {
+	if (la.x == lb.x && la.y == lb.y) return la;
+	const V2 line = la - lb;
+	float k = dot(pt - lb, line) / sqr_mag(line);
+	if (on_segment) {
+		if (k > 1.0f) k = 1.0f;
+		else if (k < 0.0f) k = 0.0f;
+	}
+	if (on_seg) *on_seg = (k >= 0.0f && k <= 1.0f);
+	return (la * k) + (lb * (1.0f - k));
+}
+// distanceToLineSquared (vector2D.h:303-306)
+UD float dist_line_sq(V2 pt, V2 la, V2 lb, bool on_segment) {
+	return sqr_mag(pt - nearest_point(pt, la, lb, on_segment));
+}
+UD V2 seg_a(float4 e) { return V2{e.x, e.y}; }
+UD V2 seg_b(float4 e) { return V2{e.z, e.w}; }
+
+// CostFunction::ComputeTimeToCollision_LineSegmentInterior (costFunction.cpp:293-325)
+UD float ttc_segment_interior(V2 pos, V2 vel, float radius, V2 sa, V2 sb) {
+	if (dist_line_sq(pos, sa, sb, true) <= radius * radius) return 0.0f;
+	V2 X;
+	if (!line_intersection(pos, pos + vel, sa, sb, X)) return kMaxFloat;
+	if (dot(vel, X - pos) < 0.0f) return kMaxFloat;
+	bool on = false;
+	nearest_point(X, sa, sb, false, &on);
+	if (!on) return kMaxFloat;
+	const float speed = mag(vel);
+	const float ttc = mag(X - pos) / speed;
+	const float line_angle = angle_between(vel, sb - sa);
+	const float sub = radius / sinf(line_angle);
+	const float tsub = sub / speed;
+	return smax(0.0f, ttc - tsub);
+}
+// CostFunction::ComputeTimeToCollision_LineSegment (costFunction.cpp:270-291)
+UD float ttc_segment(V2 pos, V2 vel, float radius, V2 sa, V2 sb) {
+	const float t1 = time_to_collision(pos, vel, radius, sa, V2{0.0f, 0.0f}, 0.0f);
+	if (t1 == 0.0f) return 0.0f;
+	if (sa.x == sb.x && sa.y == sb.y) return t1;
+	const float t2 = time_to_collision(pos, vel, radius, sb, V2{0.0f, 0.0f}, 0.0f);
+	if (t2 == 0.0f) return 0.0f;
+	return smin(smin(t1, t2), ttc_segment_interior(pos, vel, radius, sa, sb));
+}
+
+// CostFunction::ComputeTimeToFirstCollision (costFunction.cpp:229-268)
 UD float time_to_first_collision(const NbList& L, V2 pos, V2 vel, float radius, float maxDist, bool ignoreCurrent) {
 	float minTTC = kMaxFloat;
 	const float maxDistSq = maxDist * maxDist;
@@ -97,6 +147,13 @@ UD float time_to_first_collision(const NbList& L, V2 pos, V2 vel, float radius, 
 		const Nb n = L.get(e);
 		if (n.dsq > maxDistSq) continue;
 		const float ttc = time_to_collision(pos, vel, radius, n.pos, n.vel, n.radius);
+		if (ignoreCurrent && ttc == 0.0f) continue;
+		if (ttc < minTTC) minTTC = ttc;
+	}
+	for (int e = 0; e < L.n_obs; ++e) {
+		const float4 sg = L.obs[e];
+		if (dist_line_sq(pos, seg_a(sg), seg_b(sg), true) > maxDistSq) continue;
+		const float ttc = ttc_segment(pos, vel, radius, seg_a(sg), seg_b(sg));
 		if (ignoreCurrent && ttc == 0.0f) continue;
 		if (ttc < minTTC) minTTC = ttc;
 	}
@@ -163,9 +220,21 @@ UD V2 social_force_pair(const float* prm, const Self& a, const Nb& o, V2 o_pref,
 	return scale * F;
 }
 
+// SocialForcesAvoidance::ComputeObstacleInteractionForce (SocialForcesAvoidance.cpp:112-140).
+// U0 keeps its default 10: parseParameters reads the "U0" attribute into V0
+// (SocialForcesAvoidance.cpp:142-150); R is parameter slot 4.
+UD V2 social_force_obstacle(const float* prm, const Self& a, V2 sa, V2 sb) {
+	const float U0 = 10.0f, R = prm[4];
+	const V2 diff = a.pos - nearest_point(a.pos, sa, sb, true);
+	const float dist = mag(diff);
+	if (dist < 0.001f) return V2{0.0f, 0.0f};
+	const float d_U = -U0 / R * expf(-dist / R);
+	return (-diff) * d_U;
+}
+
 // ObjectInteractionForces::ComputeForce (ObjectInteractionForces.cpp:26-48) for
 // PowerLaw / SocialForcesAvoidance, GoalReachingForce::ComputeForce
-// (GoalReachingForce.cpp:26-29).
+// (GoalReachingForce.cpp:26-29). PowerLaw's obstacle force is zero (PowerLaw.cpp:68-72).
 UD V2 force_of(const DevPolicy& P, int c, const Self& a, const NbList& L) {
 	const int kind = P.kind[c];
 	if (kind == KIND_GOAL_REACHING)
@@ -180,7 +249,14 @@ UD V2 force_of(const DevPolicy& P, int c, const Self& a, const NbList& L) {
 			F = F + f;
 		}
 	}
-	return F + V2{0.0f, 0.0f};
+	V2 Fo{0.0f, 0.0f};
+	for (int e = 0; e < L.n_obs; ++e) {
+		const float4 sg = L.obs[e];
+		if (dist_line_sq(a.pos, seg_a(sg), seg_b(sg), true) <= rangeSq)
+			Fo = Fo + (kind == KIND_SOCIAL_FORCES ? social_force_obstacle(P.params[c], a, seg_a(sg), seg_b(sg))
+			                                      : V2{0.0f, 0.0f});
+	}
+	return F + Fo;
 }
 // ForceBasedFunction::ComputeTargetVelocity (ForceBasedFunction.cpp:49-52)
 UD V2 target_velocity(const DevPolicy& P, int c, const Self& a, const NbList& L) {

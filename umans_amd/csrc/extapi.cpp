@@ -60,11 +60,6 @@ Simulation* load(const char* path) {
 	}
 	umans_scenario_info info{};
 	umans_scenario_get_info(sc, &info);
-	if (info.n_obstacles > 0) {
-		std::fprintf(stderr, "Error: %s has obstacles, which this backend does not simulate.\n", path);
-		umans_scenario_free(sc);
-		return nullptr;
-	}
 	const int n = info.n_agents, np = info.n_policies;
 	std::vector<umans_policy> pol(np);
 	std::vector<int32_t> xml_ids(np), id(n), policy(n), rm(n);
@@ -75,6 +70,15 @@ Simulation* load(const char* path) {
 	a.radius = rad.data(); a.pref_speed = ps.data(); a.max_speed = ms.data(); a.max_accel = ma.data();
 	a.policy = policy.data(); a.rng = rng.data();
 	const int grc = umans_scenario_get(sc, pol.data(), xml_ids.data(), &a, st.data(), rm.data());
+	// obstacle polygons: vertex counts, then the vertices
+	std::vector<int32_t> ocount(info.n_obstacles);
+	int nverts = 0;
+	if (info.n_obstacles > 0) {
+		umans_scenario_obstacles(sc, ocount.data(), nullptr);
+		for (int32_t v : ocount) nverts += v;
+	}
+	std::vector<float> opts(2 * (size_t)nverts);
+	if (info.n_obstacles > 0) umans_scenario_obstacles(sc, ocount.data(), opts.data());
 	umans_scenario_free(sc);
 	if (grc != UMANS_OK) {
 		std::fprintf(stderr, "Error: could not read the scenario %s\n", path);
@@ -91,6 +95,11 @@ Simulation* load(const char* path) {
 	d.device = device_from_env();
 	if (!report(nullptr, umans_create(&d, &s->ctx), "umans_create") ||
 	    !report(s->ctx, umans_set_policies(s->ctx, pol.data(), np), "loading the policies")) {
+		delete s;
+		return nullptr;
+	}
+	if (info.n_obstacles > 0 &&
+	    !report(s->ctx, umans_set_obstacles(s->ctx, info.n_obstacles, ocount.data(), opts.data()), "loading the obstacles")) {
 		delete s;
 		return nullptr;
 	}

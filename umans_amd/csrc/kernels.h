@@ -312,6 +312,35 @@ UD int find_neighbors(const StepArgs& A, const WorldInfo& wi, const GridInfo& g,
 	return filled;
 }
 
+// WorldBase::computeNeighboringObstacles_Flat (worldBase.cpp:62-77) for each query of
+// WorldInfinite / WorldToric::ComputeNeighbors (worldToric.cpp:32-79): the edges whose
+// distance to float(position + displacement) is within the range, in edge order, moved
+// back by the displacement. Returns the number kept (at most kMaxObstacleEdges); `total`
+// counts them all.
+UD int find_obstacles(const StepArgs& A, const WorldInfo& wi, V2 p, float range, float4* out, int& total) {
+	const float r2 = range * range;
+	int n = 0;
+	total = 0;
+	for (int s = 0; s < 5; ++s) {
+		float dx = 0.0f, dy = 0.0f;
+		if (s > 0) {
+			if (wi.world_type != 1) break;
+			if (s == 1) { if (!((double)(p.x - range) < -0.5 * (double)wi.W)) continue; dx = wi.W; }
+			else if (s == 2) { if (!((double)(p.x + range) > 0.5 * (double)wi.W)) continue; dx = -wi.W; }
+			else if (s == 3) { if (!((double)(p.y - range) < -0.5 * (double)wi.H)) continue; dy = wi.H; }
+			else { if (!((double)(p.y + range) > 0.5 * (double)wi.H)) continue; dy = -wi.H; }
+		}
+		const V2 q{p.x + dx, p.y + dy};
+		for (int e = 0; e < A.n_edges; ++e) {
+			const float4 E = A.edges[e];
+			if (!(dist_line_sq(q, seg_a(E), seg_b(E), true) <= r2)) continue;
+			++total;
+			if (n < kMaxObstacleEdges) out[n++] = make_float4(E.x - dx, E.y - dy, E.z - dx, E.w - dy);
+		}
+	}
+	return n;
+}
+
 // --------------------------------------------------------------- per-agent state
 // Policy classes: agents are bucketed by class each step and every class runs its own
 // instance of the step kernel with only its code path compiled in.
@@ -479,6 +508,16 @@ __global__ __launch_bounds__(BS) void k_step(StepArgs A) {
 		a.relaxation = A.pol[pi].relaxation;
 		cnt = find_neighbors<KMAX, BS>(A, wi, g, k, a.pos, A.pol[pi].query_range, s_d2 + tid, s_nb + tid, total);
 	}
+	// obstacle edges: only the generic class runs when a world has obstacles
+	float4 obs_priv[CLS == CLS_GENERIC ? kMaxObstacleEdges : 1];
+	int n_obs = 0;
+	if constexpr (CLS == CLS_GENERIC) {
+		if (active && A.n_edges > 0) {
+			int obs_total = 0;
+			n_obs = find_obstacles(A, wi, a.pos, A.pol[pi].query_range, obs_priv, obs_total);
+			if (obs_total > n_obs) atomicAdd(&A.stats[2], 1ull);
+		}
+	}
 	// stats: one atomic per wave
 	{
 		const unsigned long long ov = __ballot(active && total > A.kcap);
@@ -506,7 +545,7 @@ __global__ __launch_bounds__(BS) void k_step(StepArgs A) {
 		A.raw_count[k] = cnt;
 		for (int e = 0; e < cnt; ++e) A.raw_nb[(size_t)k * A.raw_stride + e] = s_nb[e * BS + tid];
 	}
-	const NbList L{s_nb + tid, BS, cnt, S.pos, S.vel, S.prm, S.pref, wi.W, wi.H, a.pos};
+	const NbList L{s_nb + tid, BS, cnt, S.pos, S.vel, S.prm, S.pref, wi.W, wi.H, a.pos, obs_priv, n_obs};
 	uint32_t rng = (uint32_t)meta.w;
 
 	// Policy::ComputeNewVelocity (policy.cpp:43-76)
@@ -556,13 +595,18 @@ __global__ __launch_bounds__(BS) void k_step(StepArgs A) {
 		}
 	}
 
-	// Policy::ComputeContactForces (policy.cpp:107-141), agents only
+	// Policy::ComputeContactForces (policy.cpp:107-141)
 	V2 contact{0.0f, 0.0f};
 	if (P.contact_scale > 0.0f) {
 		for (int e = 0; e < cnt; ++e) {
 			const Nb n = L.get(e);
 			const V2 diff = a.pos - n.pos;
 			const float inter = a.radius + n.radius - mag(diff);
+			if (inter > 0.0f) contact = contact + normalized(diff) * (P.contact_scale * inter);
+		}
+		for (int e = 0; e < n_obs; ++e) {
+			const V2 diff = a.pos - nearest_point(a.pos, seg_a(obs_priv[e]), seg_b(obs_priv[e]), true);
+			const float inter = a.radius - mag(diff);
 			if (inter > 0.0f) contact = contact + normalized(diff) * (P.contact_scale * inter);
 		}
 	}

@@ -104,7 +104,13 @@ struct StepArgs {
 	unsigned* raw_nb;
 	int* raw_count;
 	int raw_stride;
-	unsigned long long* stats;    // [0] overflow agents, [1] max list length
+	unsigned long long* stats;    // [0] overflow agents, [1] max list length, [2] obstacle overflow agents
+	// obstacle edges of a single-world context (Polygon2D::GetEdges order), or none
+	const float4* edges;
+	int n_edges;
 };
+
+// Obstacle edges kept per agent (the generic class only); more are counted in stats[2].
+constexpr int kMaxObstacleEdges = 32;
 
 } // namespace umans

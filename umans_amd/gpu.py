@@ -31,6 +31,7 @@ EXPORTS = [
     "umans_scenario_warning", "umans_scenario_get", "umans_scenario_obstacles", "umans_csv_start",
     "umans_csv_flush", "umans_csv_stop", "umans_add_agents", "umans_remove_agent", "umans_set_agent_goal",
     "umans_set_agent_states", "umans_set_remove_at_goal", "umans_num_scheduled", "umans_download_records",
+    "umans_set_obstacles",
 ]
 
 ENOTFOUND = -2
@@ -60,7 +61,7 @@ class AgentsC(C.Structure):
 
 class Stats(C.Structure):
     _fields_ = [("steps", C.c_int64), ("overflow_agents", C.c_int64), ("max_list", C.c_int64),
-                ("sim_time", C.c_double)]
+                ("sim_time", C.c_double), ("obstacle_overflow_agents", C.c_int64)]
 
 
 class Profile(C.Structure):
@@ -142,6 +143,7 @@ def lib():
         L.umans_set_remove_at_goal.argtypes = [vp, C.c_int, _I, C.POINTER(C.c_uint8)]
         L.umans_num_scheduled.argtypes = [vp]
         L.umans_download_records.argtypes = [vp, vp, C.c_int]
+        L.umans_set_obstacles.argtypes = [vp, C.c_int, _I, _F]
         L.umans_csv_start.argtypes = [vp, C.c_char_p, C.c_int]
         L.umans_csv_flush.argtypes = [vp]
         L.umans_csv_stop.argtypes = [vp]
@@ -220,11 +222,11 @@ class World:
         reference's load order: agents whose start time is 0 join the list in file
         order, later ones are scheduled (CrowdSimulator::FromConfigFile ->
         WorldBase::AddAgent, crowdSimulator.cpp:417-512); remove_at_goal is kept per
-        agent. Obstacles are not on the GPU path: such scenarios are refused."""
+        agent; obstacles are set on the world."""
         sc, meta = load_scenario(path)
-        if len(meta["obstacle_counts"]):
-            raise UmansError(f"{path}: obstacles are not supported on the GPU path")
         w = cls(sc.world_type, sc.width, sc.height, sc.dt, sc.policies, max_neighbors, device=device)
+        if len(meta["obstacle_counts"]):
+            w.set_obstacles(meta["obstacle_counts"], meta["obstacle_points"])
         later = np.asarray(meta["start_time"], np.float32) > 0
         rm = np.asarray(meta["remove_at_goal"]).astype(bool)
         now = {k: (np.asarray(v)[~later] if v is not None else None) for k, v in sc.agents.items()}
@@ -298,6 +300,12 @@ class World:
         buf = np.zeros((n, 7), np.float32)
         self._check(lib().umans_download_records(self._h, buf.ctypes.data_as(C.c_void_p), n))
         return buf[:, 0].view(np.int32).copy(), buf[:, 1:3].copy(), buf[:, 3:5].copy(), buf[:, 5:7].copy()
+
+    def set_obstacles(self, counts, points):
+        """Polygons as vertex counts and (sum(counts), 2) vertices; edges i -> (i + 1) mod n."""
+        cnt = np.ascontiguousarray(counts, np.int32)
+        pts = np.ascontiguousarray(points, np.float32).reshape(-1)
+        self._check(lib().umans_set_obstacles(self._h, len(cnt), _p(cnt, _I), _p(pts, _F)))
 
     def num_scheduled(self):
         return int(lib().umans_num_scheduled(self._h))
@@ -411,7 +419,7 @@ class World:
         s = Stats()
         self._check(lib().umans_get_stats(self._h, C.byref(s)))
         return {"steps": s.steps, "overflow_agents": s.overflow_agents, "max_list": s.max_list,
-                "sim_time": s.sim_time}
+                "sim_time": s.sim_time, "obstacle_overflow_agents": s.obstacle_overflow_agents}
 
 
 def world_from_scenario(sc, cell_size=0.0, device=0):
