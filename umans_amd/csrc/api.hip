@@ -67,7 +67,7 @@ int kmax_bucket(int kcap) {
 }
 
 // Block sizes instantiated per neighbour-list capacity (the LDS list is KMAX x block
-// entries of 12 B).
+// entries of 8 B).
 bool block_supported(int kmax, int bs) {
 	switch (kmax) {
 	case 12: case 16: return bs == 64 || bs == 128 || bs == 256;
@@ -105,7 +105,23 @@ void launch_step_c(int kmax, int bs, const StepArgs& a, hipStream_t st) {
 	}
 }
 
+template <int KMAX>
+void launch_coop_t(const StepArgs& a, hipStream_t st) {
+	const int blocks = (a.n + 63) / 64;
+	if (blocks > 0) hipLaunchKernelGGL((k_step_coop<KMAX>), dim3(blocks), dim3(64), 0, st, a);
+}
+
 void launch_step(int cls, int kmax, int bs, const StepArgs& a, hipStream_t st) {
+	if (cls == CLS_COOP) {
+		switch (kmax) {
+		case 12: launch_coop_t<12>(a, st); break;
+		case 16: launch_coop_t<16>(a, st); break;
+		case 32: launch_coop_t<32>(a, st); break;
+		case 64: launch_coop_t<64>(a, st); break;
+		default: launch_coop_t<128>(a, st); break;
+		}
+		return;
+	}
 	switch (cls) {
 	case CLS_FORCE_GRAD: launch_step_c<CLS_FORCE_GRAD>(kmax, bs, a, st); break;
 	case CLS_FORCE_SAMPLE: launch_step_c<CLS_FORCE_SAMPLE>(kmax, bs, a, st); break;
@@ -116,6 +132,7 @@ void launch_step(int cls, int kmax, int bs, const StepArgs& a, hipStream_t st) {
 
 // The ORCA class keeps its half-planes in LDS (2 x KMAX x block x 16 B): smaller blocks.
 int class_block(int cls, int kmax, int bs) {
+	if (cls == CLS_COOP) return 64;                 // one wave per block (k_step_coop)
 	if (cls == CLS_ORCA && kmax <= 32) return 64;
 	return bs;
 }
@@ -160,9 +177,9 @@ struct umans_ctx {
 	int kcap = 0;
 	// policy classes: one step-kernel instance per class over a bucket of agents
 	std::vector<int> h_policy;           // policy of each uploaded agent
-	int cls_count[NUM_CLS] = {0, 0, 0, 0};
-	int cls_off[NUM_CLS] = {0, 0, 0, 0};    // first sorted index of each class
-	int cls_slot[NUM_CLS] = {0, 0, 0, 0};   // slot of each present class in the sort key
+	int cls_count[NUM_CLS] = {};
+	int cls_off[NUM_CLS] = {};              // first sorted index of each class
+	int cls_slot[NUM_CLS] = {};             // slot of each present class in the sort key
 	int nslots = 1;
 	int* d_cls_slot = nullptr;
 	float cell_factor = 1.0f;    // cell width = factor x max interaction range (autotuned)
@@ -428,6 +445,17 @@ int make_dev_policy(const umans_policy& p, DevPolicy& d, std::string& err) {
 	else if (all_force && d.mode == MODE_SAMPLE) d.cls = CLS_FORCE_SAMPLE;
 	else if (d.mode == MODE_GLOBAL_ORCA) d.cls = CLS_ORCA;
 	else d.cls = CLS_GENERIC;
+	// regular sampling over costs that only read the neighbour list per candidate:
+	// candidates spread over the lanes of a wave
+	if (d.cls == CLS_GENERIC && d.mode == MODE_SAMPLE && s.s_type == 0 && s.s_angle_samples <= kMaxAngleSamples) {
+		bool coop = true;
+		for (int c = 0; c < d.n_cost; ++c) {
+			const int kd = d.kind[c];
+			coop = coop && (kd == KIND_KARAMOUZAS || kd == KIND_RVO || kd == KIND_MOUSSAID || kd == KIND_TTCA_DCA ||
+			                kd == KIND_PARIS || kd == KIND_VAN_TOLL);
+		}
+		if (coop) d.cls = CLS_COOP;
+	}
 	// costFunction.cpp:90 and :122-125, in the reference's float/double mix
 	const float maxAngle = (float)(s.s_angle / 360.0 * 3.1415926535897);
 	d.max_angle = maxAngle;

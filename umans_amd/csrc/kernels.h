@@ -236,9 +236,36 @@ __global__ __launch_bounds__(256) void k_reorder(State src, State dst, int n, co
 // concatenate the results (worldToric.cpp:61-79). Each segment is kept in canonical
 // order (d2 ascending, ties by agent ID); the list keeps the first kcap entries of the
 // concatenation. Returns the list length; `total` is the uncapped length.
+//
+// The list holds float(d2) beside each entry (8 B per entry in LDS instead of 12, for
+// occupancy). Rounding to float is monotonic, so unequal floats order the entries as
+// their doubles do; equal floats are resolved by recomputing both exact doubles.
+
+// Query displacement of segment s (0: none, 1: +W, 2: -W, 3: +H, 4: -H).
+UD V2 seg_disp(const WorldInfo& wi, unsigned s) {
+	return V2{s == 1 ? wi.W : (s == 2 ? -wi.W : 0.0f), s == 3 ? wi.H : (s == 4 ? -wi.H : 0.0f)};
+}
+// The exact double d2 of a list entry (sorted index | segment << kSegShift), as the
+// search computed it.
+UD double entry_d2(const State& S, V2 p, const WorldInfo& wi, unsigned u) {
+	const V2 d = seg_disp(wi, u >> kSegShift);
+	const V2 q{p.x + d.x, p.y + d.y};
+	const float2 pj = S.pos[u & kIndexMask];
+	const double ddx = (double)q.x - (double)pj.x;
+	const double ddy = (double)q.y - (double)pj.y;
+	return ddx * ddx + ddy * ddy;
+}
+// Does candidate j (d2, fd2 = float(d2)) order after entry (fe, u)?
+UD bool orders_after(const State& S, V2 p, const WorldInfo& wi, double d2, float fd2, int j, float fe, unsigned u) {
+	if (fd2 != fe) return fd2 > fe;
+	const double de = entry_d2(S, p, wi, u);
+	if (d2 != de) return d2 > de;
+	return S.meta[j].x > S.meta[u & kIndexMask].x;
+}
+
 template <int KMAX, int BS>
 UD int find_neighbors(const StepArgs& A, const WorldInfo& wi, const GridInfo& g, int self, V2 p, float range,
-                      double* d2s, unsigned* nbs, int& total) {
+                      float* d2s, unsigned* nbs, int& total) {
 	const State& S = A.in;
 	const float r2f = range * range;
 	const double r2 = (double)r2f;
@@ -283,27 +310,25 @@ UD int find_neighbors(const StepArgs& A, const WorldInfo& wi, const GridInfo& g,
 				if (!(d2 < r2)) continue;
 				++total;
 				if (cap <= 0) continue;
+				const float fd2 = (float)d2;
 				int at;
 				if (seg_cnt < cap) {
 					at = seg_base + seg_cnt;
 					++seg_cnt;
 				} else {
 					const int last = seg_base + cap - 1;
-					const double dl = d2s[last * BS];
-					if (d2 > dl) continue;
-					if (d2 == dl && S.meta[j].x > S.meta[nbs[last * BS] & kIndexMask].x) continue;
+					if (orders_after(S, p, wi, d2, fd2, (int)j, d2s[last * BS], nbs[last * BS])) continue;
 					at = last;
 				}
 				while (at > seg_base) {
-					const double dp = d2s[(at - 1) * BS];
+					const float fp = d2s[(at - 1) * BS];
 					const unsigned np = nbs[(at - 1) * BS];
-					if (d2 > dp) break;
-					if (d2 == dp && S.meta[j].x > S.meta[np & kIndexMask].x) break;
-					d2s[at * BS] = dp;
+					if (orders_after(S, p, wi, d2, fd2, (int)j, fp, np)) break;
+					d2s[at * BS] = fp;
 					nbs[at * BS] = np;
 					--at;
 				}
-				d2s[at * BS] = d2;
+				d2s[at * BS] = fd2;
 				nbs[at * BS] = j | ((unsigned)s << kSegShift);
 			}
 		}
@@ -344,7 +369,10 @@ UD int find_obstacles(const StepArgs& A, const WorldInfo& wi, V2 p, float range,
 // --------------------------------------------------------------- per-agent state
 // Policy classes: agents are bucketed by class each step and every class runs its own
 // instance of the step kernel with only its code path compiled in.
-enum { CLS_FORCE_GRAD = 0, CLS_FORCE_SAMPLE = 1, CLS_ORCA = 2, CLS_GENERIC = 3, NUM_CLS = 4 };
+// CLS_COOP: regular sampling over costs that read the neighbour list per candidate
+// (Karamouzas, RVO, Moussaid, TtcaDca, Paris, VanToll): the candidates of a wave's agents
+// are spread over its lanes (k_step_coop).
+enum { CLS_FORCE_GRAD = 0, CLS_FORCE_SAMPLE = 1, CLS_ORCA = 2, CLS_GENERIC = 3, CLS_COOP = 4, NUM_CLS = 5 };
 
 // Loop over a policy's cost functions: fully unrolled over the 4 slots for the force
 // classes (so per-cost values stay in registers), a plain loop otherwise.
@@ -475,12 +503,15 @@ UD int xcd_tile(int b, int nb) {
 // One lane per agent of the class's sorted range [A.base, A.base + A.n).
 template <int KMAX, int BS, int CLS>
 __global__ __launch_bounds__(BS) void k_step(StepArgs A) {
-	__shared__ double s_d2[KMAX * BS];
+	__shared__ float s_d2[KMAX * BS];
 	__shared__ unsigned s_nb[KMAX * BS];
 	// ORCA half-planes in LDS when they fit beside the neighbour lists (16 KB at
 	// KMAX x block = 1024); larger instances keep them in private memory. The projected
 	// lines of linearProgram3 (only for infeasible programs) stay in private memory.
-	constexpr bool kLinesInLds = (CLS == CLS_ORCA) && (KMAX * BS <= 1024);
+#ifndef UMANS_ORCA_LINES_LDS
+#define UMANS_ORCA_LINES_LDS 1
+#endif
+	constexpr bool kLinesInLds = UMANS_ORCA_LINES_LDS && (CLS == CLS_ORCA) && (KMAX * BS <= 1024);
 	constexpr int kLdsLines = kLinesInLds ? KMAX * BS : 1;
 	__shared__ Line s_lines[kLdsLines];
 	const int tid = threadIdx.x;
@@ -537,7 +568,7 @@ __global__ __launch_bounds__(BS) void k_step(StepArgs A) {
 			const unsigned u = s_nb[e * BS + tid];
 			const long long o = (long long)k * A.rec_stride + e;
 			A.rec_id[o] = S.meta[u & kIndexMask].x;
-			A.rec_d2[o] = s_d2[e * BS + tid];
+			A.rec_d2[o] = entry_d2(S, a.pos, wi, u);
 			A.rec_seg[o] = (int)(u >> kSegShift);
 		}
 	}
@@ -629,6 +660,228 @@ __global__ __launch_bounds__(BS) void k_step(StepArgs A) {
 	A.out.prm[k] = S.prm[k];
 	A.out.pref[k] = f2(a.pref);
 	meta.w = (int)rng;
+	A.out.meta[k] = meta;
+}
+
+
+// --------------------------------------------------------- cooperative sampling
+// CostFunction::ApproximateGlobalMinimumBySampling (costFunction.cpp:66-164) for the
+// regular grid with the candidates of the 64 agents of a wave spread over its 64 lanes:
+// candidate c of agent i is work item items[i] + c. Per item, the agent's state and
+// its neighbour list are read from LDS and the neighbours' state from memory; lanes
+// working on one agent read the same neighbours, so those loads are broadcasts instead
+// of 64 separate gathers. The winner is the first strict minimum in candidate order,
+// as the sequential loop finds it: the smallest key (cost in an order-preserving
+// encoding, candidate index) wins, costs not below MaxFloat never do.
+UD unsigned long long cost_key(float tc, int c) {
+	if (!(tc < kMaxFloat)) return ~0ull;
+	unsigned u = __float_as_uint(tc == 0.0f ? 0.0f : tc);   // -0 and +0 are equal costs
+	u = (u & 0x80000000u) ? ~u : (u | 0x80000000u);
+	return ((unsigned long long)u << 32) | (unsigned)c;
+}
+
+// Candidate c of the regular grid (the loop order of sample_best).
+UD V2 regular_candidate(const DevPolicy& P, int c, V2 base, V2 bdir, float dL) {
+	int s, k;
+	if (P.s_include_base) {
+		if (c == 0) { s = 1; k = 0; }
+		else { s = 2 + (c - 1) / P.s_angle_samples; k = (c - 1) % P.s_angle_samples; }
+	} else {
+		s = 1 + c / P.s_angle_samples;
+		k = c % P.s_angle_samples;
+	}
+	const float len = dL * (float)(P.s_include_base ? s - 1 : s);
+	return base + rotate_cs(bdir, P.ang_cos[k], P.ang_sin[k]) * len;
+}
+
+UD int regular_count(const DevPolicy& P) {
+	return P.s_include_base ? 1 + (P.s_speed_samples - 1) * P.s_angle_samples : P.s_speed_samples * P.s_angle_samples;
+}
+
+template <int KMAX>
+__global__ __launch_bounds__(64) void k_step_coop(StepArgs A) {
+	constexpr int BS = 64;
+	__shared__ float s_d2[KMAX * BS];
+	__shared__ unsigned s_nb[KMAX * BS];
+	__shared__ float s_self[12][BS];    // pos, vel, pref, radius, pref speed, max speed, max accel, dt, relaxation
+	__shared__ float s_ttcp[4][BS];
+	__shared__ float s_wh[2][BS];
+	__shared__ int s_cnt[BS], s_pi[BS], s_items[BS + 1];
+	__shared__ unsigned long long s_key[BS];
+	const int tid = threadIdx.x;
+	const int t = xcd_tile(blockIdx.x, gridDim.x) * BS + tid;
+	const bool active = t < A.n;
+	const int k = active ? A.base + t : 0;
+	int total = 0, cnt = 0;
+	const State& S = A.in;
+	int4 meta = make_int4(0, 0, 0, 0);
+	int pi = 0;
+	Self a{};
+	WorldInfo wi{};
+	if (active) {
+		meta = S.meta[k];
+		pi = meta.y & kPolicyMask;
+		const int wix = meta.y >> kPolicyBits;
+		wi = A.worlds[wix];
+		const GridInfo g = A.grids[wix];
+		const float4 prm = S.prm[k];
+		a.pos = v2(S.pos[k]);
+		a.vel = v2(S.vel[k]);
+		a.pref = v2(S.pref[k]);
+		a.radius = prm.x; a.pref_speed = prm.y; a.max_speed = prm.z; a.max_accel = prm.w;
+		a.dt = wi.dt;
+		a.relaxation = A.pol[pi].relaxation;
+		cnt = find_neighbors<KMAX, BS>(A, wi, g, k, a.pos, A.pol[pi].query_range, s_d2 + tid, s_nb + tid, total);
+	}
+	{
+		const unsigned long long ov = __ballot(active && total > A.kcap);
+		const int mx = wave_max_i(total);
+		if (tid == 0) {
+			const int c = __popcll(ov);
+			if (c) atomicAdd(&A.stats[0], (unsigned long long)c);
+			atomicMax(&A.stats[1], (unsigned long long)mx);
+		}
+	}
+	const DevPolicy& P = A.pol[pi];
+	if (active && A.rec_count) {
+		A.rec_count[k] = cnt;
+		const int m = cnt < A.rec_stride ? cnt : A.rec_stride;
+		for (int e = 0; e < m; ++e) {
+			const unsigned u = s_nb[e * BS + tid];
+			const long long o = (long long)k * A.rec_stride + e;
+			A.rec_id[o] = S.meta[u & kIndexMask].x;
+			A.rec_d2[o] = entry_d2(S, a.pos, wi, u);
+			A.rec_seg[o] = (int)(u >> kSegShift);
+		}
+	}
+	const NbList L{s_nb + tid, BS, cnt, S.pos, S.vel, S.prm, S.pref, wi.W, wi.H, a.pos, nullptr, 0};
+	// Policy::ComputeNewVelocity (policy.cpp:43-76): agents with a zero preferred
+	// velocity keep a zero new velocity and have no candidates
+	const bool sample = active && !is_zero(a.pref);
+	const int ns = sample ? regular_count(P) : 0;
+	float ttcp[4] = {0.0f, 0.0f, 0.0f, 0.0f};
+	if (sample) {
+		for (int c = 0; c < P.n_cost; ++c)
+			if (P.kind[c] == KIND_KARAMOUZAS) ttcp[c] = time_to_first_collision(L, a.pos, a.pref, a.radius, P.range[c], true);
+	}
+	const float self_v[12] = {a.pos.x, a.pos.y, a.vel.x, a.vel.y, a.pref.x, a.pref.y, a.radius, a.pref_speed,
+	                          a.max_speed, a.max_accel, a.dt, a.relaxation};
+#pragma unroll
+	for (int q = 0; q < 12; ++q) s_self[q][tid] = self_v[q];
+#pragma unroll
+	for (int q = 0; q < 4; ++q) s_ttcp[q][tid] = ttcp[q];
+	s_wh[0][tid] = wi.W;
+	s_wh[1][tid] = wi.H;
+	s_cnt[tid] = cnt;
+	s_pi[tid] = pi;
+	s_key[tid] = ~0ull;
+	// exclusive prefix sum of the candidate counts over the wave
+	int incl = ns;
+	for (int o = 1; o < 64; o <<= 1) {
+		const int y = __shfl_up(incl, o);
+		if (tid >= o) incl += y;
+	}
+	s_items[tid + 1] = incl;
+	if (tid == 0) s_items[0] = 0;
+	__syncthreads();
+	const int n_items = s_items[BS];
+
+	int ag = -1;                                   // agent whose state the lane holds
+	unsigned long long local = ~0ull;
+	Self b{};
+	int bcnt = 0, bpi = 0;
+	float bW = 0.0f, bH = 0.0f;
+	Prep<KMAX, CLS_GENERIC> pr;
+	pr.n_lines = 0;
+	V2 base{0.0f, 0.0f}, bdir{1.0f, 0.0f};
+	float dL = 0.0f;
+	int cur = 0;
+	for (int it = tid; it < n_items; it += 64) {
+		while (it >= s_items[cur + 1]) ++cur;
+		if (cur != ag) {
+			if (ag >= 0 && local != ~0ull) atomicMin(&s_key[ag], local);
+			local = ~0ull;
+			ag = cur;
+			b.pos = V2{s_self[0][ag], s_self[1][ag]};
+			b.vel = V2{s_self[2][ag], s_self[3][ag]};
+			b.pref = V2{s_self[4][ag], s_self[5][ag]};
+			b.radius = s_self[6][ag]; b.pref_speed = s_self[7][ag]; b.max_speed = s_self[8][ag];
+			b.max_accel = s_self[9][ag]; b.dt = s_self[10][ag]; b.relaxation = s_self[11][ag];
+			bcnt = s_cnt[ag];
+			bpi = s_pi[ag];
+			bW = s_wh[0][ag];
+			bH = s_wh[1][ag];
+#pragma unroll
+			for (int q = 0; q < 4; ++q) pr.ttcp[q] = s_ttcp[q][ag];
+			const DevPolicy& Q = A.pol[bpi];
+			// sample_best's set-up: base, radius, base direction, speed step
+			base = Q.s_base == 1 ? b.vel : V2{0.0f, 0.0f};
+			float radius = b.pref_speed;
+			if (Q.s_radius == 1) radius = b.max_speed;
+			else if (Q.s_radius == 2) radius = smin(2.0f * b.max_speed, b.max_accel * b.dt);
+			bdir = V2{1.0f, 0.0f};
+			if (Q.s_base_dir == 1) bdir = normalized(b.vel);
+			else if (Q.s_base_dir == 2) bdir = normalized(b.pref);
+			dL = radius / (float)(Q.s_include_base ? (Q.s_speed_samples - 1) : Q.s_speed_samples);
+		}
+		const DevPolicy& Q = A.pol[bpi];
+		const int c = it - s_items[ag];
+		const V2 v = regular_candidate(Q, c, base, bdir, dL);
+		const NbList Lb{s_nb + ag, BS, bcnt, S.pos, S.vel, S.prm, S.pref, bW, bH, b.pos, nullptr, 0};
+		float tc = 0.0f;
+		for (int q = 0; q < Q.n_cost; ++q) tc += Q.coeff[q] * cost_of<KMAX, CLS_GENERIC>(Q, q, b, Lb, pr, v);
+		const unsigned long long key = cost_key(tc, c);
+		if (key < local) local = key;
+	}
+	if (ag >= 0 && local != ~0ull) atomicMin(&s_key[ag], local);
+	__syncthreads();
+	if (!active) return;
+
+	V2 next{0.0f, 0.0f};
+	if (sample) {
+		const unsigned long long key = s_key[tid];
+		V2 best{0.0f, 0.0f};
+		if (key != ~0ull) {
+			const V2 bs = P.s_base == 1 ? a.vel : V2{0.0f, 0.0f};
+			float radius = a.pref_speed;
+			if (P.s_radius == 1) radius = a.max_speed;
+			else if (P.s_radius == 2) radius = smin(2.0f * a.max_speed, a.max_accel * a.dt);
+			V2 bd{1.0f, 0.0f};
+			if (P.s_base_dir == 1) bd = normalized(a.vel);
+			else if (P.s_base_dir == 2) bd = normalized(a.pref);
+			const float dl = radius / (float)(P.s_include_base ? (P.s_speed_samples - 1) : P.s_speed_samples);
+			best = regular_candidate(P, (int)(key & 0xffffffffu), bs, bd, dl);
+		}
+		const V2 acc = clamp_vec((best - a.vel) / smax(a.relaxation, a.dt), a.max_accel);
+		next = clamp_vec(a.vel + acc * a.dt, a.max_speed);
+	}
+	// Policy::ComputeContactForces (policy.cpp:107-141)
+	V2 contact{0.0f, 0.0f};
+	if (P.contact_scale > 0.0f) {
+		for (int e = 0; e < cnt; ++e) {
+			const Nb n = L.get(e);
+			const V2 diff = a.pos - n.pos;
+			const float inter = a.radius + n.radius - mag(diff);
+			if (inter > 0.0f) contact = contact + normalized(diff) * (P.contact_scale * inter);
+		}
+	}
+	// Agent::UpdateVelocityAndPosition (agent.cpp:71-93) + toric wrap (worldToric.cpp:81-108)
+	const V2 vel = next + ((contact / 80.0f) * a.dt);
+	V2 view = v2(S.viewdir[k]);
+	const V2 vd = (2.0f * vel + a.pref) / 3.0f;
+	if ((double)sqr_mag(vd) > 0.01) view = normalized(vd);
+	V2 pos = a.pos + (vel * a.dt);
+	if (wi.world_type == 1) {
+		const float hw = 0.5f * wi.W, hh = 0.5f * wi.H;
+		if (pos.x > hw) pos.x -= wi.W; else if (pos.x < -hw) pos.x += wi.W;
+		if (pos.y > hh) pos.y -= wi.H; else if (pos.y < -hh) pos.y += wi.H;
+	}
+	A.out.pos[k] = f2(pos);
+	A.out.vel[k] = f2(vel);
+	A.out.viewdir[k] = f2(view);
+	A.out.goal[k] = S.goal[k];
+	A.out.prm[k] = S.prm[k];
+	A.out.pref[k] = f2(a.pref);
 	A.out.meta[k] = meta;
 }
 
