@@ -130,10 +130,9 @@ void launch_step(int cls, int kmax, int bs, const StepArgs& a, hipStream_t st) {
 	}
 }
 
-// The ORCA class keeps its half-planes in LDS (2 x KMAX x block x 16 B): smaller blocks.
+// The cooperative class works on one wave per block.
 int class_block(int cls, int kmax, int bs) {
 	if (cls == CLS_COOP) return 64;                 // one wave per block (k_step_coop)
-	if (cls == CLS_ORCA && kmax <= 32) return 64;
 	return bs;
 }
 

@@ -456,6 +456,33 @@ UD V2 sample_best(const DevPolicy& P, const Self& a, const NbList& L, const Prep
 		for_costs<CLS>(P, [&](int c) { tc += P.coeff[c] * cost_of<KMAX, CLS>(P, c, a, L, pr, v); });
 		if (tc < bestCost) { best = v; bestCost = tc; }
 	};
+	if constexpr (CLS == CLS_FORCE_SAMPLE) {
+		if (P.s_type == 0 && P.s_angle_samples <= kMaxAngleSamples) {
+			// The same grid in (angle, speed) order, so each direction is rotated once
+			// instead of once per speed. The candidate kept is the one the (speed, angle)
+			// loop below keeps: the smallest cost, ties to the earlier candidate in that
+			// loop's order; a cost not below MaxFloat never wins.
+			const int S = P.s_speed_samples, NA = P.s_angle_samples, inc = P.s_include_base;
+			const float dL = radius / (float)(inc ? (S - 1) : S);
+			int bestIdx = 0x7fffffff;
+			auto eval_at = [&](V2 v, int idx) {
+				float tc = 0.0f;
+				for_costs<CLS>(P, [&](int c) { tc += P.coeff[c] * cost_of<KMAX, CLS>(P, c, a, L, pr, v); });
+				if (tc < bestCost || (tc == bestCost && bestIdx != 0x7fffffff && idx < bestIdx)) {
+					best = v; bestCost = tc; bestIdx = idx;
+				}
+			};
+			for (int k = 0; k < NA; ++k) {
+				const V2 dir = rotate_cs(bdir, P.ang_cos[k], P.ang_sin[k]);
+				if (inc && k == 0) eval_at(base + dir * (dL * 0.0f), 0);
+				for (int s = inc ? 2 : 1; s <= S; ++s) {
+					const float len = dL * (float)(inc ? s - 1 : s);
+					eval_at(base + dir * len, inc ? 1 + (s - 2) * NA + k : (s - 1) * NA + k);
+				}
+			}
+			return best;
+		}
+	}
 	if (P.s_type == 1) {
 		for (int i = 0; i < P.s_random_samples; ++i) {
 			const float ra = rng_uniform(rng, -P.max_angle, P.max_angle);
@@ -505,15 +532,8 @@ template <int KMAX, int BS, int CLS>
 __global__ __launch_bounds__(BS) void k_step(StepArgs A) {
 	__shared__ float s_d2[KMAX * BS];
 	__shared__ unsigned s_nb[KMAX * BS];
-	// ORCA half-planes in LDS when they fit beside the neighbour lists (16 KB at
-	// KMAX x block = 1024); larger instances keep them in private memory. The projected
-	// lines of linearProgram3 (only for infeasible programs) stay in private memory.
-#ifndef UMANS_ORCA_LINES_LDS
-#define UMANS_ORCA_LINES_LDS 1
-#endif
-	constexpr bool kLinesInLds = UMANS_ORCA_LINES_LDS && (CLS == CLS_ORCA) && (KMAX * BS <= 1024);
-	constexpr int kLdsLines = kLinesInLds ? KMAX * BS : 1;
-	__shared__ Line s_lines[kLdsLines];
+	// The ORCA half-planes live in private memory: kept in LDS they bound the class to
+	// ~2 waves per SIMD, and the private copy (L1-resident) measured faster at 6.
 	const int tid = threadIdx.x;
 	const int t = xcd_tile(blockIdx.x, gridDim.x) * BS + tid;
 	const bool active = t < A.n;
@@ -579,15 +599,49 @@ __global__ __launch_bounds__(BS) void k_step(StepArgs A) {
 	const NbList L{s_nb + tid, BS, cnt, S.pos, S.vel, S.prm, S.pref, wi.W, wi.H, a.pos, obs_priv, n_obs};
 	uint32_t rng = (uint32_t)meta.w;
 
+	// Force classes: one pass over the neighbours gathers each neighbour once for the
+	// pair forces of every cost (ObjectInteractionForces::ComputeForce,
+	// ObjectInteractionForces.cpp:26-48) and for the contact forces (policy.cpp:107-141),
+	// each sum still in list order.
+	constexpr bool kFused = (CLS == CLS_FORCE_GRAD || CLS == CLS_FORCE_SAMPLE);
+	V2 fsum[kFused ? 4 : 1];
+	V2 contact{0.0f, 0.0f};
+	if constexpr (kFused) {
+		float rsq[4];
+#pragma unroll
+		for (int c = 0; c < 4; ++c) {
+			fsum[c] = V2{0.0f, 0.0f};
+			rsq[c] = P.range[c] * P.range[c];
+		}
+		const bool do_contact = P.contact_scale > 0.0f;
+		for (int e = 0; e < cnt; ++e) {
+			const Nb n = L.get(e);
+#pragma unroll
+			for (int c = 0; c < 4; ++c) {
+				if (c < P.n_cost && P.kind[c] != KIND_GOAL_REACHING && n.dsq <= rsq[c]) {
+					const V2 f = (P.kind[c] == KIND_POWER_LAW)
+					                 ? power_law_pair(P.params[c], a, n)
+					                 : social_force_pair(P.params[c], a, n, L.pref_of(n.j), P.view_cos);
+					fsum[c] = fsum[c] + f;
+				}
+			}
+			if (do_contact) {
+				const V2 diff = a.pos - n.pos;
+				const float inter = a.radius + n.radius - mag(diff);
+				if (inter > 0.0f) contact = contact + normalized(diff) * (P.contact_scale * inter);
+			}
+		}
+	}
+
 	// Policy::ComputeNewVelocity (policy.cpp:43-76)
 	V2 next{0.0f, 0.0f};
 	if (!is_zero(a.pref)) {
 		Prep<KMAX, CLS> pr;
 		pr.n_lines = 0;
 		if constexpr (CLS == CLS_ORCA) {
-			Line lines_priv[kLinesInLds ? 1 : KMAX];
+			Line lines_priv[KMAX];
 			Line proj_priv[KMAX];
-			pr.lines = kLinesInLds ? Lines{s_lines + tid, BS} : Lines{lines_priv, 1};
+			pr.lines = Lines{lines_priv, 1};
 			const Lines proj{proj_priv, 1};
 			pr.n_lines = orca_lines(P.params[0], P.range[0], a, L, pr.lines);
 			const V2 best = orca_solve(pr.lines, pr.n_lines, a, proj);
@@ -599,7 +653,12 @@ __global__ __launch_bounds__(BS) void k_step(StepArgs A) {
 			pr.lines = Lines{lines_priv, 1};
 			for_costs<CLS>(P, [&](int c) {
 				const int kind = P.kind[c];
-				if (CLS != CLS_GENERIC || is_force_based(kind)) pr.target[c] = target_velocity(P, c, a, L);
+				if constexpr (kFused) {
+					// ForceBasedFunction::ComputeTargetVelocity (ForceBasedFunction.cpp:49-52)
+					const V2 f = (kind == KIND_GOAL_REACHING) ? (a.pref - a.vel) / smax(a.relaxation, a.dt)
+					                                          : fsum[c] + V2{0.0f, 0.0f};
+					pr.target[c] = a.vel + f / P.params[c][0] * a.dt;
+				} else if (CLS != CLS_GENERIC || is_force_based(kind)) pr.target[c] = target_velocity(P, c, a, L);
 				else if (kind == KIND_ORCA) pr.n_lines = orca_lines(P.params[c], P.range[c], a, L, pr.lines);
 				else if (kind == KIND_KARAMOUZAS)
 					pr.ttcp[c] = time_to_first_collision(L, a.pos, a.pref, a.radius, P.range[c], true);
@@ -627,8 +686,7 @@ __global__ __launch_bounds__(BS) void k_step(StepArgs A) {
 	}
 
 	// Policy::ComputeContactForces (policy.cpp:107-141)
-	V2 contact{0.0f, 0.0f};
-	if (P.contact_scale > 0.0f) {
+	if (!kFused && P.contact_scale > 0.0f) {
 		for (int e = 0; e < cnt; ++e) {
 			const Nb n = L.get(e);
 			const V2 diff = a.pos - n.pos;
