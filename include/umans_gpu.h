@@ -159,6 +159,11 @@ typedef struct {
 	float cell_factor;
 	int32_t block_size;
 	float ms_per_step;         /* autotune: time of the chosen configuration */
+	int32_t list_capacity;     /* lists longer than 32 (no cap, or a cap above 32): capacity
+	                              of the first launch, 32 or 64, the agents whose list is
+	                              longer being redone at the next capacity; 128: a single
+	                              launch at full capacity; 0 in umans_set_tuning: unchanged.
+	                              Results do not depend on it. */
 } umans_tuning;
 int umans_autotune(umans_ctx* ctx, int trial_steps, umans_tuning* out);
 int umans_set_tuning(umans_ctx* ctx, const umans_tuning* tuning);

@@ -1,0 +1,47 @@
+"""Long neighbour lists (no cap) are computed in tiers: a launch with 32- or 64-entry
+lists hands the agents whose list is longer to a launch with longer lists. The tier a
+list is finished in must not show: every list capacity gives the same lists and the
+same new state, bit for bit, as one launch at full capacity."""
+import numpy as np
+import pytest
+
+from cases import CASES
+from umans_amd import gpu
+from umans_amd import scenarios as S
+from umans_amd.registry import pack_policy
+
+pytestmark = pytest.mark.gpu
+
+
+def dense_world(case, world_type, list_capacity, size=50.0, steps=3):
+    method, costs, samp, relax = CASES[case]
+    # lists of every tier: ~48 neighbours on average (4 m range at 0.96 agents per m^2,
+    # 5 m at 0.6), a few above 64 and a few at or below 32
+    n = 2400 if case == "powerlaw_sampling" else 1500
+    sc = S.random_cloud(n, size, [None], world_type=world_type, seed=31)
+    w = gpu.World(world_type, size, size, sc.dt, [pack_policy(method, costs, sampling=samp, relaxation_time=relax)])
+    w.upload(sc.agents)
+    w.set_tuning(1.0, 0, list_capacity)
+    w.record_neighbors(True)
+    w.step(steps)
+    out = w.download()
+    nb = w.neighbors()
+    st = w.stats()
+    w.close()
+    return out, nb, st
+
+
+@pytest.mark.parametrize("case", ["powerlaw_sampling", "socialforces_gradient", "orca_global", "rvo_random"])
+@pytest.mark.parametrize("world_type", [0, 1])
+def test_list_capacity_tiers_do_not_change_results(case, world_type):
+    ref, (rc, rid, rd2, rseg), rst = dense_world(case, world_type, 128)
+    assert rst["max_list"] > 64, "the scenario must reach the last tier"
+    for cap in (32, 64):
+        out, (cnt, ids, d2, seg), st = dense_world(case, world_type, cap)
+        np.testing.assert_array_equal(cnt, rc)
+        np.testing.assert_array_equal(ids, rid)
+        np.testing.assert_array_equal(d2, rd2)
+        for k in ("pos", "vel", "viewdir", "rng"):
+            np.testing.assert_array_equal(out[k], ref[k], err_msg=f"{case} capacity {cap} {k}")
+        assert st["overflow_agents"] == rst["overflow_agents"]
+        assert st["max_list"] == rst["max_list"]

@@ -164,6 +164,11 @@ struct umans_ctx {
 	unsigned* scan_sums = nullptr;
 	unsigned* cell_of = nullptr;
 	unsigned* rank = nullptr;
+	// list-capacity tiers (uncapped and large caps): agents whose list outgrew a tier,
+	// per tier, in their class's range; and the counts per class and tier
+	int* ovf[2] = {nullptr, nullptr};
+	unsigned* ovf_count = nullptr;            // [NUM_CLS][2]
+	int tier_first = 2;                       // first tier: 0 (32 entries), 1 (64), 2 (single launch; autotuned)
 	int cell_alloc = 0;          // allocated cell counters (>= total_cells + 1)
 	int total_cells = 0;         // sum of the worlds' cell_cap
 	WorldInfo* dworlds = nullptr;
@@ -316,7 +321,8 @@ struct umans_ctx {
 		drop_graph();
 		free_adjoint();
 		X.release(); Y.release(); B.release();
-		hipFree(cell_of); hipFree(rank);
+		hipFree(cell_of); hipFree(rank); hipFree(ovf[0]); hipFree(ovf[1]);
+		ovf[0] = ovf[1] = nullptr;
 		hipFree(rec_count); hipFree(rec_id); hipFree(rec_d2); hipFree(rec_seg);
 		cell_of = rank = nullptr;
 		rec_count = rec_id = rec_seg = nullptr;
@@ -663,6 +669,7 @@ int enqueue_step(umans_ctx* c, bool events, const State* ysave = nullptr, unsign
 	}
 	hipLaunchKernelGGL(k_fill, dim3(ceil_div(scan_n, 256)), dim3(256), 0, st, c->cell_count, scan_n, 0u);
 	hipLaunchKernelGGL(k_fill, dim3(1), dim3(64), 0, st, (unsigned*)c->dstats, 8, 0u);
+	hipLaunchKernelGGL(k_fill, dim3(1), dim3(64), 0, st, c->ovf_count, 2 * NUM_CLS, 0u);
 	if (c->any_infinite) {
 		hipLaunchKernelGGL(k_bbox_init, dim3(ceil_div(nw, 256)), dim3(256), 0, st, c->dbbox, nw);
 		hipLaunchKernelGGL(k_bbox, dim3(ceil_div(n, 256)), dim3(256), 0, st, c->X.s.pos, c->X.s.meta, c->dworlds, n,
@@ -702,6 +709,7 @@ int enqueue_step(umans_ctx* c, bool events, const State* ysave = nullptr, unsign
 	a.stats = c->dstats;
 	a.edges = c->d_edges;
 	a.n_edges = c->n_edges;
+	a.tier_final = 1;
 	const int kmax = kmax_bucket(c->kcap);
 	const int bs = c->block > 0 ? c->block : default_block(kmax);
 	if (e1) hipEventRecord(e1, st);
@@ -719,7 +727,33 @@ int enqueue_step(umans_ctx* c, bool events, const State* ysave = nullptr, unsign
 			sq = c->cls_stream[q];
 			hipStreamWaitEvent(sq, c->fork_ev, 0);
 		}
-		launch_step(q, kmax, class_block(q, kmax, bs), aq, sq);
+		if (q == CLS_COOP || kmax <= 32 || c->tier_first >= 2) {
+			launch_step(q, kmax, class_block(q, kmax, bs), aq, sq);
+		} else {
+			// Long lists (no cap, or a cap above 32): a first launch with 32-entry lists
+			// (4x the occupancy of 128) hands the agents whose list is longer to a 64-entry
+			// launch, which hands on those longer still to the final capacity. Every agent
+			// is computed by the first launch whose capacity holds its whole list; the
+			// lists, and so the results, are those of a single launch at full capacity.
+			const int tiers[3] = {32, 64, 128};
+			const int nt = kmax <= 64 ? 2 : 3;
+			const int t0 = c->tier_first;
+			for (int ti = t0 < nt - 1 ? t0 : nt - 1; ti < nt; ++ti) {
+				StepArgs at = aq;
+				if (ti > t0) {
+					at.agent_list = c->ovf[(ti - 1) & 1];
+					at.agent_count = c->ovf_count + 2 * q + ((ti - 1) & 1);
+				}
+				at.tier_final = ti == nt - 1 ? 1 : 0;
+				if (!at.tier_final) {
+					at.ovf_list = c->ovf[ti & 1];
+					at.ovf_count = c->ovf_count + 2 * q + (ti & 1);
+					at.ovf_base = c->cls_off[q];
+				}
+				const int tk = tiers[ti];
+				launch_step(q, tk, class_block(q, tk, tk <= 32 ? bs : 64), at, sq);
+			}
+		}
 		if (!first) hipEventRecord(c->join_ev[q], sq);
 		first = false;
 	}
@@ -803,6 +837,8 @@ int grow_agents(umans_ctx* c, int n, int keep) {
 	if ((rc = c->hip(c->Y.alloc(cap), "alloc state"))) return rc;
 	if ((rc = c->hip(dalloc(&c->cell_of, cap), "alloc grid"))) return rc;
 	if ((rc = c->hip(dalloc(&c->rank, cap), "alloc grid"))) return rc;
+	if ((rc = c->hip(dalloc(&c->ovf[0], cap), "alloc grid"))) return rc;
+	if ((rc = c->hip(dalloc(&c->ovf[1], cap), "alloc grid"))) return rc;
 	c->cap = cap;
 	if (rec) {
 		const int stride = kmax_bucket(c->kcap);
@@ -1073,7 +1109,7 @@ int create_impl(const umans_world_desc* descs, int nworlds, umans_ctx** out) {
 		             hipEventCreateWithFlags(&c->join_ev[q], hipEventDisableTiming) == hipSuccess;
 	if (!streams_ok || hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess ||
 	    dalloc(&c->dgrids, nworlds) != hipSuccess || dalloc(&c->dworlds, nworlds) != hipSuccess ||
-	    dalloc(&c->dbbox, 4 * nworlds) != hipSuccess || dalloc(&c->dstats, 4) != hipSuccess ||
+	    dalloc(&c->dbbox, 4 * nworlds) != hipSuccess || dalloc(&c->dstats, 4) != hipSuccess || dalloc(&c->ovf_count, 2 * NUM_CLS) != hipSuccess ||
 	    dalloc(&c->dpol, kMaxPolicies) != hipSuccess || dalloc(&c->d_cls_slot, NUM_CLS) != hipSuccess) {
 		umans_destroy(c);
 		return UMANS_EHIP;
@@ -1103,7 +1139,7 @@ void umans_destroy(umans_ctx* c) {
 	c->free_pop_buffers();
 	c->free_agents();
 	hipFree(c->dgrids); hipFree(c->dworlds); hipFree(c->dbbox); hipFree(c->dstats); hipFree(c->dpol);
-	hipFree(c->d_edges);
+	hipFree(c->d_edges); hipFree(c->ovf_count);
 	hipFree(c->d_cls_slot);
 	hipFree(c->cell_count); hipFree(c->cell_start); hipFree(c->scan_sums);
 	c->free_events();
@@ -1199,6 +1235,8 @@ int umans_upload(umans_ctx* c, const umans_agents* a, int n) {
 		if ((rc = c->hip(c->Y.alloc(n), "alloc state"))) return rc;
 		if ((rc = c->hip(dalloc(&c->cell_of, n), "alloc grid"))) return rc;
 		if ((rc = c->hip(dalloc(&c->rank, n), "alloc grid"))) return rc;
+		if ((rc = c->hip(dalloc(&c->ovf[0], n), "alloc grid"))) return rc;
+		if ((rc = c->hip(dalloc(&c->ovf[1], n), "alloc grid"))) return rc;
 		c->cap = n;
 	}
 	c->n = n;
@@ -1288,6 +1326,9 @@ int umans_autotune(umans_ctx* c, int trial_steps, umans_tuning* out) {
 	const int blocks[] = {64, 128, 256};
 	const float f0 = c->cell_factor;
 	const int b0 = c->block;
+	const int tier0 = c->tier_first;
+	const int n_tiers = kmax > 32 ? 3 : 1;   // tier choices only matter for long lists
+	int best_t = tier0;
 	const int rec0 = c->record;
 	c->record = 0;
 	float best_ms = 3.4e38f, best_f = f0;
@@ -1296,10 +1337,12 @@ int umans_autotune(umans_ctx* c, int trial_steps, umans_tuning* out) {
 	hipEventCreate(&t0);
 	hipEventCreate(&t1);
 	for (float f : factors) {
-		for (int b : blocks) {
+		for (int b : blocks)
+		for (int ti = 0; ti < n_tiers; ++ti) {
 			if (!block_supported(kmax, b)) continue;
 			c->cell_factor = f;
 			c->block = b;
+			c->tier_first = n_tiers > 1 ? ti : tier0;
 			c->drop_graph();
 			if ((rc = setup_worlds(c, per_world))) break;
 			if ((rc = c->hip(copy_state(c->X.s, c->B.s, c->n, c->stream), "restore"))) break;
@@ -1311,7 +1354,7 @@ int umans_autotune(umans_ctx* c, int trial_steps, umans_tuning* out) {
 			if ((rc = c->hip(hipEventSynchronize(t1), "autotune"))) break;
 			float ms = 0.0f;
 			hipEventElapsedTime(&ms, t0, t1);
-			if (ms < best_ms) { best_ms = ms; best_f = f; best_b = b; }
+			if (ms < best_ms) { best_ms = ms; best_f = f; best_b = b; best_t = c->tier_first; }
 		}
 		if (rc) break;
 	}
@@ -1320,6 +1363,7 @@ int umans_autotune(umans_ctx* c, int trial_steps, umans_tuning* out) {
 	// restore the configuration chosen (or the previous one on failure) and the state
 	c->cell_factor = rc ? f0 : best_f;
 	c->block = rc ? b0 : best_b;
+	c->tier_first = rc ? tier0 : best_t;
 	c->record = rec0;
 	c->drop_graph();
 	int rc2 = setup_worlds(c, per_world);
@@ -1331,6 +1375,7 @@ int umans_autotune(umans_ctx* c, int trial_steps, umans_tuning* out) {
 		out->cell_factor = c->cell_factor;
 		out->block_size = c->block;
 		out->ms_per_step = best_ms / trial_steps;
+		out->list_capacity = c->tier_first == 0 ? 32 : (c->tier_first == 1 ? 64 : 128);
 	}
 	return UMANS_OK;
 }
@@ -1339,11 +1384,14 @@ int umans_set_tuning(umans_ctx* c, const umans_tuning* t) {
 	if (!c || !t) return UMANS_EINVAL;
 	if (!(t->cell_factor > 0.0f) || (t->block_size != 0 && !block_supported(kmax_bucket(c->kcap), t->block_size)))
 		return c->fail(UMANS_EINVAL, "unsupported tuning");
+	if (t->list_capacity != 0 && t->list_capacity != 32 && t->list_capacity != 64 && t->list_capacity != 128)
+		return c->fail(UMANS_EINVAL, "list_capacity must be 0, 32, 64 or 128");
 	hipSetDevice(c->device);
 	int rc = c->hip(hipStreamSynchronize(c->stream), "tuning");
 	if (rc) return rc;
 	c->cell_factor = t->cell_factor;
 	c->block = t->block_size;
+	if (t->list_capacity) c->tier_first = t->list_capacity == 32 ? 0 : (t->list_capacity == 64 ? 1 : 2);
 	c->drop_graph();
 	if (c->n > 0) {
 		std::vector<int> per_world;

@@ -274,7 +274,7 @@ UD int find_neighbors(const StepArgs& A, const WorldInfo& wi, const GridInfo& g,
 	// every other candidate
 	const float r2_reject = r2f * 1.00001f;
 	const float rs = range * 1.0001f + 1e-3f;
-	const int kcap = A.kcap;
+	const int kcap = A.kcap < KMAX ? A.kcap : KMAX;   // a tier's list holds at most KMAX
 	int filled = 0;
 	total = 0;
 	for (int s = 0; s < 5; ++s) {
@@ -536,8 +536,9 @@ __global__ __launch_bounds__(BS) void k_step(StepArgs A) {
 	// ~2 waves per SIMD, and the private copy (L1-resident) measured faster at 6.
 	const int tid = threadIdx.x;
 	const int t = xcd_tile(blockIdx.x, gridDim.x) * BS + tid;
-	const bool active = t < A.n;
-	const int k = active ? A.base + t : 0;
+	const int n_here = A.agent_count ? (int)*A.agent_count : A.n;
+	const bool active = t < n_here;
+	const int k = active ? (A.agent_list ? A.agent_list[A.base + t] : A.base + t) : 0;
 	int total = 0, cnt = 0;
 	const State& S = A.in;
 	int4 meta = make_int4(0, 0, 0, 0);
@@ -571,7 +572,7 @@ __global__ __launch_bounds__(BS) void k_step(StepArgs A) {
 	}
 	// stats: one atomic per wave
 	{
-		const unsigned long long ov = __ballot(active && total > A.kcap);
+		const unsigned long long ov = __ballot(active && A.tier_final && total > A.kcap);
 		const int mx = wave_max_i(total);
 		if ((tid & 63) == 0) {
 			const int c = __popcll(ov);
@@ -580,6 +581,11 @@ __global__ __launch_bounds__(BS) void k_step(StepArgs A) {
 		}
 	}
 	if (!active) return;
+	// a list that outgrew this launch's capacity is redone by the next tier
+	if (!A.tier_final && total > KMAX) {
+		A.ovf_list[A.ovf_base + atomicAdd(A.ovf_count, 1u)] = k;
+		return;
+	}
 	const DevPolicy& P = A.pol[pi];
 	if (A.rec_count) {
 		A.rec_count[k] = cnt;

@@ -108,6 +108,15 @@ struct StepArgs {
 	// obstacle edges of a single-world context (Polygon2D::GetEdges order), or none
 	const float4* edges;
 	int n_edges;
+	// list-capacity tiers: a launch may take its agents from a list (sorted indices at
+	// agent_list[base + t], t < *agent_count) and may hand on the agents whose neighbour
+	// list outgrew its capacity (ovf_list[ovf_base + i], i counted in *ovf_count)
+	const int* agent_list;
+	const unsigned* agent_count;
+	int* ovf_list;
+	unsigned* ovf_count;
+	int ovf_base;
+	int tier_final;               // 1: this launch's lists are final (counts go to stats)
 };
 
 // Obstacle edges kept per agent (the generic class only); more are counted in stats[2].

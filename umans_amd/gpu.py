@@ -69,7 +69,8 @@ class Profile(C.Structure):
 
 
 class Tuning(C.Structure):
-    _fields_ = [("cell_factor", C.c_float), ("block_size", C.c_int32), ("ms_per_step", C.c_float)]
+    _fields_ = [("cell_factor", C.c_float), ("block_size", C.c_int32), ("ms_per_step", C.c_float),
+                ("list_capacity", C.c_int32)]
 
 
 class AdjointDesc(C.Structure):
@@ -378,10 +379,11 @@ class World:
     def autotune(self, trial_steps=3):
         t = Tuning()
         self._check(lib().umans_autotune(self._h, int(trial_steps), C.byref(t)))
-        return {"cell_factor": t.cell_factor, "block_size": t.block_size, "ms_per_step": t.ms_per_step}
+        return {"cell_factor": t.cell_factor, "block_size": t.block_size, "ms_per_step": t.ms_per_step,
+                "list_capacity": t.list_capacity}
 
-    def set_tuning(self, cell_factor, block_size=0):
-        t = Tuning(float(cell_factor), int(block_size), 0.0)
+    def set_tuning(self, cell_factor, block_size=0, list_capacity=0):
+        t = Tuning(float(cell_factor), int(block_size), 0.0, int(list_capacity))
         self._check(lib().umans_set_tuning(self._h, C.byref(t)))
 
     def set_policies(self, policies):
