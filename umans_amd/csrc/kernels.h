@@ -269,10 +269,6 @@ UD int find_neighbors(const StepArgs& A, const WorldInfo& wi, const GridInfo& g,
 	const State& S = A.in;
 	const float r2f = range * range;
 	const double r2 = (double)r2f;
-	// float pre-filter: the float d2 of a candidate is within 5 ulp of its double d2, so
-	// one beyond r2 * (1 + 1e-5) is certainly outside; the exact double test decides
-	// every other candidate
-	const float r2_reject = r2f * 1.00001f;
 	const float rs = range * 1.0001f + 1e-3f;
 	const int kcap = A.kcap < KMAX ? A.kcap : KMAX;   // a tier's list holds at most KMAX
 	int filled = 0;
@@ -300,16 +296,15 @@ UD int find_neighbors(const StepArgs& A, const WorldInfo& wi, const GridInfo& g,
 			const int row = slot0 + wi.cell_offset + cy * g.ncx;
 			const unsigned b = A.cell_start[row + cx0], e = A.cell_start[row + cx1 + 1];
 			for (unsigned j = b; j < e; ++j) {
-				if ((int)j == self) continue;
+				// one divergent branch per candidate: the exact test costs a few FP64 ops
+				// (full rate) where nested early-outs cost scalar exec-mask bookkeeping
 				const float2 pj = S.pos[j];
-				const float fx = q.x - pj.x, fy = q.y - pj.y;
-				if (fx * fx + fy * fy > r2_reject) continue;
 				const double ddx = qx - (double)pj.x;
 				const double ddy = qy - (double)pj.y;
 				const double d2 = ddx * ddx + ddy * ddy;
-				if (!(d2 < r2)) continue;
-				++total;
-				if (cap <= 0) continue;
+				const bool hit = d2 < r2 && (int)j != self;
+				total += hit ? 1 : 0;
+				if (!hit || cap <= 0) continue;
 				const float fd2 = (float)d2;
 				int at;
 				if (seg_cnt < cap) {
