@@ -159,11 +159,13 @@ struct umans_ctx {
 	int n = 0, cap = 0;
 	DeviceState X, Y;            // X: state between steps, Y: cell-sorted state of a step
 	DeviceState B;               // backup of X used by the autotuner
-	unsigned* cell_count = nullptr;
-	unsigned* cell_start = nullptr;
+	unsigned* cell_count = nullptr;           // per (class slot, cell)
+	unsigned* cell_start = nullptr;           // class-major exclusive scan of cell_count
+	unsigned* cell_first = nullptr;           // per cell over all slots: the search's ranges
 	unsigned* scan_sums = nullptr;
 	unsigned* cell_of = nullptr;
 	unsigned* rank = nullptr;
+	int* class_list = nullptr;                // sorted index of each class-list entry
 	// list-capacity tiers (uncapped and large caps): agents whose list outgrew a tier,
 	// per tier, in their class's range; and the counts per class and tier
 	int* ovf[2] = {nullptr, nullptr};
@@ -321,8 +323,9 @@ struct umans_ctx {
 		drop_graph();
 		free_adjoint();
 		X.release(); Y.release(); B.release();
-		hipFree(cell_of); hipFree(rank); hipFree(ovf[0]); hipFree(ovf[1]);
+		hipFree(cell_of); hipFree(rank); hipFree(ovf[0]); hipFree(ovf[1]); hipFree(class_list);
 		ovf[0] = ovf[1] = nullptr;
+		class_list = nullptr;
 		hipFree(rec_count); hipFree(rec_id); hipFree(rec_d2); hipFree(rec_seg);
 		cell_of = rank = nullptr;
 		rec_count = rec_id = rec_seg = nullptr;
@@ -521,13 +524,14 @@ int setup_worlds(umans_ctx* c, const std::vector<int>& per_world) {
 	c->total_cells = (int)off;
 	if ((long long)NUM_CLS * off + 1 > (1LL << 31) - 1) return c->fail(UMANS_EINVAL, "grid too large");
 	if (NUM_CLS * c->total_cells + 1 > c->cell_alloc) {
-		hipFree(c->cell_count); hipFree(c->cell_start); hipFree(c->scan_sums);
-		c->cell_count = c->cell_start = c->scan_sums = nullptr;
+		hipFree(c->cell_count); hipFree(c->cell_start); hipFree(c->cell_first); hipFree(c->scan_sums);
+		c->cell_count = c->cell_start = c->cell_first = c->scan_sums = nullptr;
 		c->cell_alloc = 0;
 		const int cells = NUM_CLS * c->total_cells + 1;
 		int rc;
 		if ((rc = c->hip(dalloc(&c->cell_count, cells), "alloc grid"))) return rc;
 		if ((rc = c->hip(dalloc(&c->cell_start, cells), "alloc grid"))) return rc;
+		if ((rc = c->hip(dalloc(&c->cell_first, c->total_cells + 1), "alloc grid"))) return rc;
 		if ((rc = c->hip(dalloc(&c->scan_sums, ceil_div(cells, kScanTile) + 1), "alloc grid"))) return rc;
 		c->cell_alloc = cells;
 	}
@@ -682,8 +686,12 @@ int enqueue_step(umans_ctx* c, bool events, const State* ysave = nullptr, unsign
 	hipLaunchKernelGGL(k_scan_local, dim3(nb), dim3(kScanThreads), 0, st, c->cell_count, c->cell_start, c->scan_sums, scan_n);
 	hipLaunchKernelGGL(k_scan_sums, dim3(1), dim3(kScanThreads), 0, st, c->scan_sums, nb);
 	hipLaunchKernelGGL(k_scan_add, dim3(nb), dim3(kScanThreads), 0, st, c->cell_start, c->scan_sums, scan_n);
-	hipLaunchKernelGGL(k_reorder, dim3(ceil_div(n, 256)), dim3(256), 0, st, c->X.s, Ys, n, c->cell_of, c->rank,
-	                   c->cell_start);
+	// one class slot: the per-cell starts are the class-major ones; with several the
+	// reorder also writes them
+	const unsigned* cell_first = c->nslots > 1 ? c->cell_first : c->cell_start;
+	const int nr = c->nslots > 1 ? std::max(n, c->total_cells + 1) : n;
+	hipLaunchKernelGGL(k_reorder, dim3(ceil_div(nr, 256)), dim3(256), 0, st, c->X.s, Ys, n, c->cell_of, c->rank,
+	                   c->cell_start, c->cell_count, c->nslots, c->total_cells, c->cell_first, c->class_list);
 
 	StepArgs a{};
 	a.in = Ys;
@@ -691,9 +699,8 @@ int enqueue_step(umans_ctx* c, bool events, const State* ysave = nullptr, unsign
 	a.pol = c->dpol;
 	a.worlds = c->dworlds;
 	a.grids = c->dgrids;
-	a.cell_start = c->cell_start;
-	a.nslots = c->nslots;
-	a.slot_cells = c->total_cells;
+	a.cell_start = cell_first;
+	a.class_list = c->class_list;
 	a.base = 0;
 	a.n = n;
 	a.kcap = c->kcap;
@@ -837,6 +844,7 @@ int grow_agents(umans_ctx* c, int n, int keep) {
 	if ((rc = c->hip(c->Y.alloc(cap), "alloc state"))) return rc;
 	if ((rc = c->hip(dalloc(&c->cell_of, cap), "alloc grid"))) return rc;
 	if ((rc = c->hip(dalloc(&c->rank, cap), "alloc grid"))) return rc;
+	if ((rc = c->hip(dalloc(&c->class_list, cap), "alloc grid"))) return rc;
 	if ((rc = c->hip(dalloc(&c->ovf[0], cap), "alloc grid"))) return rc;
 	if ((rc = c->hip(dalloc(&c->ovf[1], cap), "alloc grid"))) return rc;
 	c->cap = cap;
@@ -1141,7 +1149,7 @@ void umans_destroy(umans_ctx* c) {
 	hipFree(c->dgrids); hipFree(c->dworlds); hipFree(c->dbbox); hipFree(c->dstats); hipFree(c->dpol);
 	hipFree(c->d_edges); hipFree(c->ovf_count);
 	hipFree(c->d_cls_slot);
-	hipFree(c->cell_count); hipFree(c->cell_start); hipFree(c->scan_sums);
+	hipFree(c->cell_count); hipFree(c->cell_start); hipFree(c->cell_first); hipFree(c->scan_sums);
 	c->free_events();
 	for (int q = 0; q < NUM_CLS; ++q) {
 		if (c->cls_stream[q]) hipStreamDestroy(c->cls_stream[q]);
@@ -1235,6 +1243,7 @@ int umans_upload(umans_ctx* c, const umans_agents* a, int n) {
 		if ((rc = c->hip(c->Y.alloc(n), "alloc state"))) return rc;
 		if ((rc = c->hip(dalloc(&c->cell_of, n), "alloc grid"))) return rc;
 		if ((rc = c->hip(dalloc(&c->rank, n), "alloc grid"))) return rc;
+		if ((rc = c->hip(dalloc(&c->class_list, n), "alloc grid"))) return rc;
 		if ((rc = c->hip(dalloc(&c->ovf[0], n), "alloc grid"))) return rc;
 		if ((rc = c->hip(dalloc(&c->ovf[1], n), "alloc grid"))) return rc;
 		c->cap = n;

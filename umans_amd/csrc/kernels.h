@@ -4,9 +4,13 @@
 //   k_fill         clear cell counters (a kernel, never a memset, so the step can be
 //                  captured in a graph)
 //   k_bbox/k_grid  infinite worlds only: bounding box -> uniform cell grid
-//   k_count        cell key of every agent + its rank inside the cell (atomic)
-//   k_scan_*       exclusive scan of the counts -> cell ranges
-//   k_reorder      scatter the state into cell order; preferred velocity
+//   k_count        (class, cell) key of every agent + its rank among the agents of
+//                  that class in that cell (atomic)
+//   k_scan_*       class-major exclusive scan of the counts: each class's agents in
+//                  cell order (the class kernels' work lists), and per cell over all
+//                  classes by differences (the ranges the neighbour search walks)
+//   k_reorder      scatter the state into cell order, classes interleaved within a
+//                  cell, and fill the class lists; preferred velocity
 //                  (Agent::ComputePreferredVelocity, agent.cpp:52-59)
 //   k_step         per agent (one lane): k-nearest-within-radius query in canonical
 //                  order, new velocity, contact forces, integration, toric wrap
@@ -125,8 +129,10 @@ UD int bin_axis(float x, float o, float inv, int nc) {
 	return (int)t;
 }
 
-// Sort key of an agent: (class slot, world, cell). Agents of one policy class end up
-// contiguous, each class in cell order, so every class kernel reads a dense range.
+// Key of an agent: (class slot, world, cell). The class-major scan of these counts
+// numbers each class's agents in cell order (the class kernels' work lists); the state
+// itself is stored cell-major, every class of a cell together, so the neighbour search
+// walks one contiguous range per row of cells whatever the classes.
 __global__ __launch_bounds__(256) void k_count(const float2* pos, const int4* meta, int n, const WorldInfo* worlds,
                                               const GridInfo* grids, const DevPolicy* pol, const int* cls_slot,
                                               int slot_cells, unsigned* cell_count, unsigned* cell_of,
@@ -207,11 +213,37 @@ __global__ __launch_bounds__(kScanThreads) void k_scan_add(unsigned* out, const 
 }
 
 // --------------------------------------------------------------------- reorder
+// Cell-major start of cell c from the class-major scan cs of the (slot, cell) counts:
+// the agents of every slot in the cells before c, sum over q of cs[q, c] - cs[q, 0]
+// (c = slot_cells gives the agent count).
+UD unsigned cell_first_of(const unsigned* cs, unsigned c, int nslots, unsigned slot_cells) {
+	unsigned t = 0;
+	for (int q = 0; q < nslots; ++q) t += cs[q * slot_cells + c] - cs[q * slot_cells];
+	return t;
+}
+
+// Thread i < n: agent i, with key (slot, cell) and rank r, goes to d = cell_first(cell)
+// + (agents of the lower slots in that cell) + r and is entry class_start[key] + r of
+// the class lists. Thread i <= slot_cells (with several slots): cell_first[i] for the
+// neighbour search.
 __global__ __launch_bounds__(256) void k_reorder(State src, State dst, int n, const unsigned* cell_of,
-                                                const unsigned* rank, const unsigned* cell_start) {
+                                                const unsigned* rank, const unsigned* class_start,
+                                                const unsigned* cell_count, int nslots, int slot_cells,
+                                                unsigned* cell_first, int* class_list) {
 	const int k = blockIdx.x * blockDim.x + threadIdx.x;
+	const unsigned sc = (unsigned)slot_cells;
+	if (nslots > 1 && k <= slot_cells) cell_first[k] = cell_first_of(class_start, (unsigned)k, nslots, sc);
 	if (k >= n) return;
-	const unsigned d = cell_start[cell_of[k]] + rank[k];
+	const unsigned key = cell_of[k], r = rank[k];
+	const unsigned slot = key / sc, cell = key - slot * sc;
+	unsigned d = r;
+	if (nslots > 1) {
+		d += cell_first_of(class_start, cell, nslots, sc);
+		for (unsigned q = 0; q < slot; ++q) d += cell_count[cell + q * sc];
+	} else {
+		d += class_start[key];
+	}
+	class_list[class_start[key] + r] = (int)d;
 	const float2 p = src.pos[k];
 	const float2 g = src.goal[k];
 	const float4 prm = src.prm[k];
@@ -291,9 +323,8 @@ UD int find_neighbors(const StepArgs& A, const WorldInfo& wi, const GridInfo& g,
 		const int seg_base = filled;
 		const int cap = kcap - filled;
 		int seg_cnt = 0;
-		for (int slot0 = 0; slot0 < A.nslots * A.slot_cells; slot0 += A.slot_cells)
 		for (int cy = cy0; cy <= cy1; ++cy) {
-			const int row = slot0 + wi.cell_offset + cy * g.ncx;
+			const int row = wi.cell_offset + cy * g.ncx;
 			const unsigned b = A.cell_start[row + cx0], e = A.cell_start[row + cx1 + 1];
 			for (unsigned j = b; j < e; ++j) {
 				// one divergent branch per candidate: the exact test costs a few FP64 ops
@@ -522,7 +553,7 @@ UD int xcd_tile(int b, int nb) {
 #endif
 }
 
-// One lane per agent of the class's sorted range [A.base, A.base + A.n).
+// One lane per agent of the class's list entries [A.base, A.base + A.n).
 template <int KMAX, int BS, int CLS>
 __global__ __launch_bounds__(BS) void k_step(StepArgs A) {
 	__shared__ float s_d2[KMAX * BS];
@@ -533,7 +564,7 @@ __global__ __launch_bounds__(BS) void k_step(StepArgs A) {
 	const int t = xcd_tile(blockIdx.x, gridDim.x) * BS + tid;
 	const int n_here = A.agent_count ? (int)*A.agent_count : A.n;
 	const bool active = t < n_here;
-	const int k = active ? (A.agent_list ? A.agent_list[A.base + t] : A.base + t) : 0;
+	const int k = active ? (A.agent_list ? A.agent_list[A.base + t] : A.class_list[A.base + t]) : 0;
 	int total = 0, cnt = 0;
 	const State& S = A.in;
 	int4 meta = make_int4(0, 0, 0, 0);
@@ -770,7 +801,7 @@ __global__ __launch_bounds__(64) void k_step_coop(StepArgs A) {
 	const int tid = threadIdx.x;
 	const int t = xcd_tile(blockIdx.x, gridDim.x) * BS + tid;
 	const bool active = t < A.n;
-	const int k = active ? A.base + t : 0;
+	const int k = active ? A.class_list[A.base + t] : 0;
 	int total = 0, cnt = 0;
 	const State& S = A.in;
 	int4 meta = make_int4(0, 0, 0, 0);

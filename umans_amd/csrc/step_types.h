@@ -88,9 +88,8 @@ struct StepArgs {
 	const DevPolicy* pol;
 	const WorldInfo* worlds;
 	const GridInfo* grids;
-	const unsigned* cell_start;   // nslots * slot_cells + 1
-	int nslots;                   // policy classes present; agents are sorted by (class, cell)
-	int slot_cells;               // cells per class slot (all worlds)
+	const unsigned* cell_start;   // first sorted index of each cell (all worlds), + total
+	const int* class_list;        // sorted index of each class-list entry
 	int base;                     // first sorted index of this launch's agents
 	int n;                        // agents of this launch
 	int kcap;                // neighbour cap (<= KMAX of the kernel instance)
