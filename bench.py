@@ -6,10 +6,12 @@ A step is one WorldBase::DoStep (neighbour grid rebuild, neighbour search, new
 velocity, contact forces, integration) over every agent, on synthetic agents already
 resident in HBM. Default workload: the large mixed crowd, 1M agents in a 2000 x 2000 m
 toric world (40% PowerLaw + sampling, 30% SocialForces + gradient, 30% ORCA, range 4 m,
-at most 12 neighbours, dt 0.05 s). Prints one JSON line with the aggregate
-agent-steps/s, the roofline of the dominant kernel (the fused step kernel, timed with
-HIP events on the library's stream) and the reference's own CPU path timed on the host
-cores on a bounded sample of the same workload.
+at most 12 neighbours, dt 0.05 s). The timed steps replay the library's captured
+hipGraph of one step (umans_set_graph; --no-graph launches them one by one). Prints one
+JSON line with the aggregate agent-steps/s, the roofline of the dominant kernel (the
+class kernels of the step phase, timed with HIP events on the library's stream over
+further eager steps) and the reference's own CPU path timed on the host cores on a
+bounded sample of the same workload.
 """
 import argparse
 import json
@@ -181,8 +183,10 @@ def main():
                     "autotuning (profiling runs use the tuning the timed run chose)")
     ap.add_argument("--no-extras", action="store_true",
                     help="skip the batched-sweep and adjoint legs added to the default line")
-    ap.add_argument("--graph", action="store_true", help="replay a captured hipGraph per step "
-                    "(no per-kernel timing then)")
+    ap.add_argument("--no-graph", action="store_true", help="time eagerly launched steps instead of "
+                    "replays of the captured hipGraph of one step")
+    ap.add_argument("--phase-steps", type=int, default=20,
+                    help="eager steps timed per phase with HIP events after the timed run (roofline)")
     ap.add_argument("--copy-probe-mib", type=int, default=0,
                     help="only run a device-to-device copy of this size (3 times): the known-size "
                          "reference that scales the FETCH_SIZE / WRITE_SIZE counters")
@@ -211,24 +215,27 @@ def main():
         tuning = {"cell_factor": float(cf), "block_size": int(bs), "fixed": True}
     else:
         tuning = None if args.no_autotune else w.autotune(trial_steps=3)
-    if args.graph:
-        w.set_graph(True)
+    graph = not args.no_graph
+    # the timed run: replays of the captured step (or eager launches with --no-graph)
+    w.set_graph(graph)
     w.step(args.warmup)
-    w.synchronize()
-    w.set_profiling(not args.graph)
     w.synchronize()
     t0 = time.perf_counter()
     w.step(args.steps)
     w.synchronize()
     t = time.perf_counter() - t0
-    prof = w.profile()
     st = w.stats()
+    # per-phase timing for the roofline: eager steps with HIP events on the library's
+    # stream around the grid phases and around the class kernels' fork and join
+    w.set_graph(False)
+    w.set_profiling(True)
+    w.step(max(args.phase_steps, 1))
+    w.synchronize()
+    prof = w.profile()
     w.close()
 
     n = sc.n
     value = n * args.steps / t
-    if args.graph:      # no per-kernel events inside a graph: attribute the whole step
-        prof = {"steps": args.steps, "step_ms": t * 1e3, "grid_ms": 0.0}
     step_s = prof["step_ms"] / 1e3 / max(prof["steps"], 1)
     grid_s = prof["grid_ms"] / 1e3 / max(prof["steps"], 1)
     achieved = RL.STEP_BYTES * n / step_s / 1e9
@@ -255,7 +262,7 @@ def main():
                     "size_m": sc.width, "dt": sc.dt, "max_neighbors": sc.max_neighbors,
                     "policies": "40% PowerLaw+sampling 12x6, 30% SocialForces+gradient, 30% ORCA"
                     if sc.name.startswith("mixed") else sc.name, "parallelism": "single GPU"}),
-        "tuning": tuning, "graph": bool(args.graph),
+        "tuning": tuning, "graph": graph,
         "roofline": {"bound": "hbm", "kernel": "k_step", "achieved": achieved, "peak": RL.HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": achieved / RL.HBM_PEAK_GBS,
                      "traffic": measured_traffic() if sc.name.startswith("mixed") else None,

@@ -172,6 +172,7 @@ struct umans_ctx {
 	unsigned* ovf_count = nullptr;            // [NUM_CLS][2]
 	int tier_first = 2;                       // first tier: 0 (32 entries), 1 (64), 2 (single launch; autotuned)
 	int cell_alloc = 0;          // allocated cell counters (>= total_cells + 1)
+	int counts_zero_n = -1;      // the counters are zero over this key space (-1: unknown)
 	int total_cells = 0;         // sum of the worlds' cell_cap
 	WorldInfo* dworlds = nullptr;
 	GridInfo* dgrids = nullptr;
@@ -534,6 +535,7 @@ int setup_worlds(umans_ctx* c, const std::vector<int>& per_world) {
 		if ((rc = c->hip(dalloc(&c->cell_first, c->total_cells + 1), "alloc grid"))) return rc;
 		if ((rc = c->hip(dalloc(&c->scan_sums, ceil_div(cells, kScanTile) + 1), "alloc grid"))) return rc;
 		c->cell_alloc = cells;
+		c->counts_zero_n = -1;
 	}
 	return c->hip(hipMemcpyAsync(c->dworlds, wi.data(), nw * sizeof(WorldInfo), hipMemcpyHostToDevice, c->stream),
 	              "world upload");
@@ -671,15 +673,18 @@ int enqueue_step(umans_ctx* c, bool events, const State* ysave = nullptr, unsign
 		e0 = c->next_event(); e1 = c->next_event(); e2 = c->next_event();
 		hipEventRecord(e0, st);
 	}
-	hipLaunchKernelGGL(k_fill, dim3(ceil_div(scan_n, 256)), dim3(256), 0, st, c->cell_count, scan_n, 0u);
-	hipLaunchKernelGGL(k_fill, dim3(1), dim3(64), 0, st, (unsigned*)c->dstats, 8, 0u);
-	hipLaunchKernelGGL(k_fill, dim3(1), dim3(64), 0, st, c->ovf_count, 2 * NUM_CLS, 0u);
+	// the scan leaves the counters zero; clear them here only when the key space changed
+	// (or a step did not complete)
+	if (c->counts_zero_n != scan_n)
+		hipLaunchKernelGGL(k_fill, dim3(ceil_div(c->cell_alloc, 256)), dim3(256), 0, st, c->cell_count, c->cell_alloc, 0u);
+	c->counts_zero_n = -1;
 	if (c->any_infinite) {
 		hipLaunchKernelGGL(k_bbox_init, dim3(ceil_div(nw, 256)), dim3(256), 0, st, c->dbbox, nw);
 		hipLaunchKernelGGL(k_bbox, dim3(ceil_div(n, 256)), dim3(256), 0, st, c->X.s.pos, c->X.s.meta, c->dworlds, n,
 		                   c->dbbox);
 	}
-	hipLaunchKernelGGL(k_grid_setup, dim3(ceil_div(nw, 64)), dim3(64), 0, st, c->dworlds, c->dbbox, c->dgrids, nw);
+	hipLaunchKernelGGL(k_grid_setup, dim3(ceil_div(std::max(nw, 2 * NUM_CLS), 64)), dim3(64), 0, st, c->dworlds, c->dbbox,
+	                   c->dgrids, nw, (unsigned*)c->dstats, 8, c->ovf_count, 2 * NUM_CLS);
 	hipLaunchKernelGGL(k_count, dim3(ceil_div(n, 256)), dim3(256), 0, st, c->X.s.pos, c->X.s.meta, n, c->dworlds,
 	                   c->dgrids, c->dpol, c->d_cls_slot, c->total_cells, c->cell_count, c->cell_of, c->rank);
 	const int nb = ceil_div(scan_n, kScanTile);
@@ -691,7 +696,7 @@ int enqueue_step(umans_ctx* c, bool events, const State* ysave = nullptr, unsign
 	const unsigned* cell_first = c->nslots > 1 ? c->cell_first : c->cell_start;
 	const int nr = c->nslots > 1 ? std::max(n, c->total_cells + 1) : n;
 	hipLaunchKernelGGL(k_reorder, dim3(ceil_div(nr, 256)), dim3(256), 0, st, c->X.s, Ys, n, c->cell_of, c->rank,
-	                   c->cell_start, c->cell_count, c->nslots, c->total_cells, c->cell_first, c->class_list);
+	                   c->cell_start, c->nslots, c->total_cells, c->cell_first, c->class_list);
 
 	StepArgs a{};
 	a.in = Ys;
@@ -773,7 +778,9 @@ int enqueue_step(umans_ctx* c, bool events, const State* ysave = nullptr, unsign
 		}
 	}
 	if (e2) hipEventRecord(e2, st);
-	return c->hip(hipGetLastError(), "step launch");
+	const int rc = c->hip(hipGetLastError(), "step launch");
+	if (!rc) c->counts_zero_n = scan_n;
+	return rc;
 }
 
 int capture_graph(umans_ctx* c) {

@@ -1,8 +1,9 @@
 // Kernels of one WorldBase::DoStep (src/Engine/core/worldBase.cpp:79-128) on gfx950.
 //
 // Per step, on the context's stream:
-//   k_fill         clear cell counters (a kernel, never a memset, so the step can be
-//                  captured in a graph)
+//   k_fill         clear the cell counters when the key space changed (a kernel, never
+//                  a memset, so the step can be captured in a graph; afterwards the
+//                  scan leaves them zero)
 //   k_bbox/k_grid  infinite worlds only: bounding box -> uniform cell grid
 //   k_count        (class, cell) key of every agent + its rank among the agents of
 //                  that class in that cell (atomic)
@@ -84,8 +85,12 @@ __global__ __launch_bounds__(256) void k_bbox(const float2* pos, const int4* met
 // One thread per world. Infinite worlds: a grid over the bounding box with cells of
 // at least cell_size metres and at most cell_cap cells. Toric worlds: the world
 // rectangle centred on the origin with cells at least cell_size wide.
-__global__ void k_grid_setup(const WorldInfo* worlds, const int* bbox, GridInfo* grids, int nworlds) {
+// Also clears the step's counters (zero0[0..n0), zero1[0..n1)), saving their launches.
+__global__ void k_grid_setup(const WorldInfo* worlds, const int* bbox, GridInfo* grids, int nworlds, unsigned* zero0,
+                             int n0, unsigned* zero1, int n1) {
 	const int wi = blockIdx.x * blockDim.x + threadIdx.x;
+	if (wi < n0) zero0[wi] = 0u;
+	if (wi < n1) zero1[wi] = 0u;
 	if (wi >= nworlds) return;
 	const WorldInfo W = worlds[wi];
 	GridInfo r;
@@ -175,13 +180,18 @@ UD unsigned block_exclusive_scan(unsigned v, unsigned* sh, unsigned& total) {
 	return wofs + x - v;
 }
 
-__global__ __launch_bounds__(kScanThreads) void k_scan_local(const unsigned* in, unsigned* out, unsigned* sums, int n) {
+// Reads the counts and leaves them zero for the next step's k_count.
+__global__ __launch_bounds__(kScanThreads) void k_scan_local(unsigned* in, unsigned* out, unsigned* sums, int n) {
 	__shared__ unsigned sh[kScanThreads / 64];
 	const int base = blockIdx.x * kScanTile + threadIdx.x * kScanItems;
 	unsigned v[kScanItems];
 	unsigned s = 0;
 	for (int i = 0; i < kScanItems; ++i) {
-		v[i] = (base + i < n) ? in[base + i] : 0u;
+		v[i] = 0u;
+		if (base + i < n) {
+			v[i] = in[base + i];
+			in[base + i] = 0u;
+		}
 		s += v[i];
 	}
 	unsigned total;
@@ -223,13 +233,12 @@ UD unsigned cell_first_of(const unsigned* cs, unsigned c, int nslots, unsigned s
 }
 
 // Thread i < n: agent i, with key (slot, cell) and rank r, goes to d = cell_first(cell)
-// + (agents of the lower slots in that cell) + r and is entry class_start[key] + r of
-// the class lists. Thread i <= slot_cells (with several slots): cell_first[i] for the
-// neighbour search.
+// + (agents of the lower slots in that cell: differences of class_start) + r and is
+// entry class_start[key] + r of the class lists. Thread i <= slot_cells (with several
+// slots): cell_first[i] for the neighbour search.
 __global__ __launch_bounds__(256) void k_reorder(State src, State dst, int n, const unsigned* cell_of,
-                                                const unsigned* rank, const unsigned* class_start,
-                                                const unsigned* cell_count, int nslots, int slot_cells,
-                                                unsigned* cell_first, int* class_list) {
+                                                const unsigned* rank, const unsigned* class_start, int nslots,
+                                                int slot_cells, unsigned* cell_first, int* class_list) {
 	const int k = blockIdx.x * blockDim.x + threadIdx.x;
 	const unsigned sc = (unsigned)slot_cells;
 	if (nslots > 1 && k <= slot_cells) cell_first[k] = cell_first_of(class_start, (unsigned)k, nslots, sc);
@@ -239,7 +248,7 @@ __global__ __launch_bounds__(256) void k_reorder(State src, State dst, int n, co
 	unsigned d = r;
 	if (nslots > 1) {
 		d += cell_first_of(class_start, cell, nslots, sc);
-		for (unsigned q = 0; q < slot; ++q) d += cell_count[cell + q * sc];
+		for (unsigned q = 0; q < slot; ++q) d += class_start[q * sc + cell + 1] - class_start[q * sc + cell];
 	} else {
 		d += class_start[key];
 	}
