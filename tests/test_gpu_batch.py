@@ -124,3 +124,50 @@ def test_autotune_keeps_state_and_results():
         o = W.download()
         np.testing.assert_array_equal(o["vel"], r["vel"], err_msg=f"tuning {f} {bsz}")
     W.close()
+
+
+@pytest.mark.parametrize("kind", ["mixed", "toric_batch", "dynamic"])
+def test_steps_in_one_call_equal_single_steps(kind):
+    """Within one step call every step also counts the agents for the next one (fixed
+    grids), and the next step skips its own count; a call of K steps gives exactly the
+    results of K calls of one step (which all count afresh), with scheduled insertions
+    and remove_at_goal removals in between for "dynamic"."""
+    import os
+    from goldens import GOLDEN
+    if kind == "dynamic":
+        path = os.path.join(GOLDEN, "examples", "Dynamic-Mixed.xml")
+        mk = lambda: gpu.World.from_scenario_file(path)
+        agents = None
+        steps = 40
+    elif kind == "mixed":
+        sc = S.mixed(n=20_000, size=283.0)
+        mk = lambda: gpu.World(sc.world_type, sc.width, sc.height, sc.dt, sc.policies, sc.max_neighbors)
+        agents = sc.agents
+        steps = 6
+    else:
+        sc = S.mixed(n=6_000, size=160.0)
+        half = sc.n // 2
+        worlds = [(1, sc.width, sc.height, sc.dt)] * 2
+        a = {k: v.copy() for k, v in sc.agents.items()}
+        a["world"] = (np.arange(sc.n) >= half).astype(np.int32)
+        mk = lambda: gpu.World(0, 0, 0, sc.dt, sc.policies, max_neighbors=sc.max_neighbors, batch=worlds)
+        agents = a
+        steps = 6
+    one = mk()
+    many = mk()
+    if agents is not None:
+        one.upload(agents)
+        many.upload(agents)
+    one.step(steps)
+    for _ in range(steps):
+        many.step(1)
+    if kind == "dynamic":
+        r1, r2 = one.records(), many.records()
+        for x, y, name in zip(r1, r2, ("id", "pos", "vel", "viewdir")):
+            np.testing.assert_array_equal(x, y, err_msg=name)
+    else:
+        d1, d2 = one.download(), many.download()
+        for name in ("pos", "vel", "viewdir"):
+            np.testing.assert_array_equal(d1[name], d2[name], err_msg=name)
+    one.close()
+    many.close()

@@ -164,7 +164,6 @@ struct umans_ctx {
 	unsigned* cell_first = nullptr;           // per cell over all slots: the search's ranges
 	unsigned* scan_sums = nullptr;
 	unsigned* cell_of = nullptr;
-	unsigned* rank = nullptr;
 	int* class_list = nullptr;                // sorted index of each class-list entry
 	// list-capacity tiers (uncapped and large caps): agents whose list outgrew a tier,
 	// per tier, in their class's range; and the counts per class and tier
@@ -173,6 +172,9 @@ struct umans_ctx {
 	int tier_first = 2;                       // first tier: 0 (32 entries), 1 (64), 2 (single launch; autotuned)
 	int cell_alloc = 0;          // allocated cell counters (>= total_cells + 1)
 	int counts_zero_n = -1;      // the counters are zero over this key space (-1: unknown)
+	// the previous step of this run_steps call counted X for this step (cell_count,
+	// cell_of): the step skips k_count
+	bool keys_ready = false;
 	int total_cells = 0;         // sum of the worlds' cell_cap
 	WorldInfo* dworlds = nullptr;
 	GridInfo* dgrids = nullptr;
@@ -324,11 +326,11 @@ struct umans_ctx {
 		drop_graph();
 		free_adjoint();
 		X.release(); Y.release(); B.release();
-		hipFree(cell_of); hipFree(rank); hipFree(ovf[0]); hipFree(ovf[1]); hipFree(class_list);
+		hipFree(cell_of); hipFree(ovf[0]); hipFree(ovf[1]); hipFree(class_list);
 		ovf[0] = ovf[1] = nullptr;
 		class_list = nullptr;
 		hipFree(rec_count); hipFree(rec_id); hipFree(rec_d2); hipFree(rec_seg);
-		cell_of = rank = nullptr;
+		cell_of = nullptr;
 		rec_count = rec_id = rec_seg = nullptr;
 		rec_d2 = nullptr;
 		n = cap = 0;
@@ -656,8 +658,10 @@ int csv_after_step(umans_ctx* c) {
 	return UMANS_OK;
 }
 
+// fuse_next: the step kernels also do the next step's k_count (fixed grids only; the
+// caller guarantees nothing changes X before that step).
 int enqueue_step(umans_ctx* c, bool events, const State* ysave = nullptr, unsigned* raw_nb = nullptr,
-                 int* raw_count = nullptr) {
+                 int* raw_count = nullptr, bool fuse_next = false) {
 	const int n = c->n;
 	const State& Ys = ysave ? *ysave : c->Y.s;
 	hipStream_t st = c->stream;
@@ -673,9 +677,12 @@ int enqueue_step(umans_ctx* c, bool events, const State* ysave = nullptr, unsign
 		e0 = c->next_event(); e1 = c->next_event(); e2 = c->next_event();
 		hipEventRecord(e0, st);
 	}
+	const bool have_keys = c->keys_ready && !c->any_infinite;
+	const bool fuse = fuse_next && !c->any_infinite;
+	c->keys_ready = false;
 	// the scan leaves the counters zero; clear them here only when the key space changed
-	// (or a step did not complete)
-	if (c->counts_zero_n != scan_n)
+	// (or a step did not complete, or counted for a step that did not come)
+	if (!have_keys && c->counts_zero_n != scan_n)
 		hipLaunchKernelGGL(k_fill, dim3(ceil_div(c->cell_alloc, 256)), dim3(256), 0, st, c->cell_count, c->cell_alloc, 0u);
 	c->counts_zero_n = -1;
 	if (c->any_infinite) {
@@ -685,8 +692,9 @@ int enqueue_step(umans_ctx* c, bool events, const State* ysave = nullptr, unsign
 	}
 	hipLaunchKernelGGL(k_grid_setup, dim3(ceil_div(std::max(nw, 2 * NUM_CLS), 64)), dim3(64), 0, st, c->dworlds, c->dbbox,
 	                   c->dgrids, nw, (unsigned*)c->dstats, 8, c->ovf_count, 2 * NUM_CLS);
-	hipLaunchKernelGGL(k_count, dim3(ceil_div(n, 256)), dim3(256), 0, st, c->X.s.pos, c->X.s.meta, n, c->dworlds,
-	                   c->dgrids, c->dpol, c->d_cls_slot, c->total_cells, c->cell_count, c->cell_of, c->rank);
+	if (!have_keys)
+		hipLaunchKernelGGL(k_count, dim3(ceil_div(n, 256)), dim3(256), 0, st, c->X.s.pos, c->X.s.meta, n, c->dworlds,
+		                   c->dgrids, c->dpol, c->d_cls_slot, c->total_cells, c->cell_count, c->cell_of);
 	const int nb = ceil_div(scan_n, kScanTile);
 	hipLaunchKernelGGL(k_scan_local, dim3(nb), dim3(kScanThreads), 0, st, c->cell_count, c->cell_start, c->scan_sums, scan_n);
 	hipLaunchKernelGGL(k_scan_sums, dim3(1), dim3(kScanThreads), 0, st, c->scan_sums, nb);
@@ -695,7 +703,7 @@ int enqueue_step(umans_ctx* c, bool events, const State* ysave = nullptr, unsign
 	// reorder also writes them
 	const unsigned* cell_first = c->nslots > 1 ? c->cell_first : c->cell_start;
 	const int nr = c->nslots > 1 ? std::max(n, c->total_cells + 1) : n;
-	hipLaunchKernelGGL(k_reorder, dim3(ceil_div(nr, 256)), dim3(256), 0, st, c->X.s, Ys, n, c->cell_of, c->rank,
+	hipLaunchKernelGGL(k_reorder, dim3(ceil_div(nr, 256)), dim3(256), 0, st, c->X.s, Ys, n, c->cell_of, c->cell_count,
 	                   c->cell_start, c->nslots, c->total_cells, c->cell_first, c->class_list);
 
 	StepArgs a{};
@@ -722,6 +730,10 @@ int enqueue_step(umans_ctx* c, bool events, const State* ysave = nullptr, unsign
 	a.edges = c->d_edges;
 	a.n_edges = c->n_edges;
 	a.tier_final = 1;
+	if (fuse) {
+		a.next_count = c->cell_count;
+		a.next_cell_of = c->cell_of;
+	}
 	const int kmax = kmax_bucket(c->kcap);
 	const int bs = c->block > 0 ? c->block : default_block(kmax);
 	if (e1) hipEventRecord(e1, st);
@@ -734,6 +746,7 @@ int enqueue_step(umans_ctx* c, bool events, const State* ysave = nullptr, unsign
 		StepArgs aq = a;
 		aq.base = c->cls_off[q];
 		aq.n = c->cls_count[q];
+		aq.next_key_base = c->cls_slot[q] * c->total_cells;
 		hipStream_t sq = st;
 		if (!first) {
 			sq = c->cls_stream[q];
@@ -779,15 +792,18 @@ int enqueue_step(umans_ctx* c, bool events, const State* ysave = nullptr, unsign
 	}
 	if (e2) hipEventRecord(e2, st);
 	const int rc = c->hip(hipGetLastError(), "step launch");
-	if (!rc) c->counts_zero_n = scan_n;
+	if (!rc) {
+		c->counts_zero_n = fuse ? -1 : scan_n;
+		c->keys_ready = fuse;
+	}
 	return rc;
 }
 
-int capture_graph(umans_ctx* c) {
+int capture_graph(umans_ctx* c, bool fuse_next) {
 	c->drop_graph();
 	int rc = c->hip(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal), "graph capture");
 	if (rc) return rc;
-	const int erc = enqueue_step(c, false);
+	const int erc = enqueue_step(c, false, nullptr, nullptr, nullptr, fuse_next);
 	hipGraph_t g = nullptr;
 	const hipError_t e = hipStreamEndCapture(c->stream, &g);
 	if (erc) {
@@ -850,7 +866,6 @@ int grow_agents(umans_ctx* c, int n, int keep) {
 	c->X = nx;
 	if ((rc = c->hip(c->Y.alloc(cap), "alloc state"))) return rc;
 	if ((rc = c->hip(dalloc(&c->cell_of, cap), "alloc grid"))) return rc;
-	if ((rc = c->hip(dalloc(&c->rank, cap), "alloc grid"))) return rc;
 	if ((rc = c->hip(dalloc(&c->class_list, cap), "alloc grid"))) return rc;
 	if ((rc = c->hip(dalloc(&c->ovf[0], cap), "alloc grid"))) return rc;
 	if ((rc = c->hip(dalloc(&c->ovf[1], cap), "alloc grid"))) return rc;
@@ -906,6 +921,7 @@ int apply_device_edits(umans_ctx* c) {
 // added agents (appended), then the per-row tables.
 int sync_population(umans_ctx* c) {
 	if (!c->pop_open) return c->remove_dirty ? upload_remove_flags(c) : UMANS_OK;
+	c->keys_ready = false;     // rows move, agents come and go: the next step counts again
 	int rc;
 	if (c->csv_on && (rc = csv_drain(c))) return rc;   // logged copies refer to the old rows
 	const int n_dev = c->n;
@@ -1041,8 +1057,22 @@ int remove_reached(umans_ctx* c) {
 	return sync_population(c);
 }
 
+int run_steps_inner(umans_ctx* c, int nsteps);
+
+// Within one call each step also counts the agents for the next one (fixed grids), so
+// from the second step on k_count is skipped; a population change in between makes the
+// next step count again. A captured graph is the steady-state step (no k_count): it is
+// replayed only on steps whose keys are ready.
 int run_steps(umans_ctx* c, int nsteps) {
+	const int rc0 = run_steps_inner(c, nsteps);
+	c->keys_ready = false;     // keys never carry over to other entry points
+	return rc0;
+}
+
+int run_steps_inner(umans_ctx* c, int nsteps) {
 	const bool single = c->worlds.size() == 1;
+	const bool fuse = !c->any_infinite;
+	c->keys_ready = false;
 	for (int s = 0; s < nsteps; ++s) {
 		if (single) {
 			insert_due(c);
@@ -1051,10 +1081,12 @@ int run_steps(umans_ctx* c, int nsteps) {
 		}
 		if (c->n > 0) {
 			int rc;
-			if (c->use_graph && !c->profiling) {
-				if (!c->graph_exec && (rc = capture_graph(c))) return rc;
+			if (c->use_graph && !c->profiling && (!fuse || c->keys_ready)) {
+				if (!c->graph_exec && (rc = capture_graph(c, fuse))) return rc;
 				if ((rc = c->hip(hipGraphLaunch(c->graph_exec, c->stream), "graph launch"))) return rc;
-			} else if ((rc = enqueue_step(c, c->profiling != 0))) {
+				c->keys_ready = fuse;
+				c->counts_zero_n = fuse ? -1 : c->counts_zero_n;
+			} else if ((rc = enqueue_step(c, c->profiling != 0, nullptr, nullptr, nullptr, fuse))) {
 				return rc;
 			}
 		}
@@ -1249,7 +1281,6 @@ int umans_upload(umans_ctx* c, const umans_agents* a, int n) {
 		if ((rc = c->hip(c->X.alloc(n), "alloc state"))) return rc;
 		if ((rc = c->hip(c->Y.alloc(n), "alloc state"))) return rc;
 		if ((rc = c->hip(dalloc(&c->cell_of, n), "alloc grid"))) return rc;
-		if ((rc = c->hip(dalloc(&c->rank, n), "alloc grid"))) return rc;
 		if ((rc = c->hip(dalloc(&c->class_list, n), "alloc grid"))) return rc;
 		if ((rc = c->hip(dalloc(&c->ovf[0], n), "alloc grid"))) return rc;
 		if ((rc = c->hip(dalloc(&c->ovf[1], n), "alloc grid"))) return rc;

@@ -5,13 +5,14 @@
 //                  a memset, so the step can be captured in a graph; afterwards the
 //                  scan leaves them zero)
 //   k_bbox/k_grid  infinite worlds only: bounding box -> uniform cell grid
-//   k_count        (class, cell) key of every agent + its rank among the agents of
-//                  that class in that cell (atomic)
+//   k_count        (class, cell) key of every agent, counted per key (atomic, no
+//                  return)
 //   k_scan_*       class-major exclusive scan of the counts: each class's agents in
 //                  cell order (the class kernels' work lists), and per cell over all
 //                  classes by differences (the ranges the neighbour search walks)
 //   k_reorder      scatter the state into cell order, classes interleaved within a
-//                  cell, and fill the class lists; preferred velocity
+//                  cell (the rank inside a key taken by counting the key back down to
+//                  zero), and fill the class lists; preferred velocity
 //                  (Agent::ComputePreferredVelocity, agent.cpp:52-59)
 //   k_step         per agent (one lane): k-nearest-within-radius query in canonical
 //                  order, new velocity, contact forces, integration, toric wrap
@@ -140,8 +141,7 @@ UD int bin_axis(float x, float o, float inv, int nc) {
 // walks one contiguous range per row of cells whatever the classes.
 __global__ __launch_bounds__(256) void k_count(const float2* pos, const int4* meta, int n, const WorldInfo* worlds,
                                               const GridInfo* grids, const DevPolicy* pol, const int* cls_slot,
-                                              int slot_cells, unsigned* cell_count, unsigned* cell_of,
-                                              unsigned* rank) {
+                                              int slot_cells, unsigned* cell_count, unsigned* cell_of) {
 	const int k = blockIdx.x * blockDim.x + threadIdx.x;
 	if (k >= n) return;
 	const int4 m = meta[k];
@@ -152,7 +152,17 @@ __global__ __launch_bounds__(256) void k_count(const float2* pos, const int4* me
 	const int c = slot * slot_cells + worlds[w].cell_offset + bin_axis(p.y, g.oy, g.invy, g.ncy) * g.ncx +
 	              bin_axis(p.x, g.ox, g.invx, g.ncx);
 	cell_of[k] = (unsigned)c;
-	rank[k] = atomicAdd(&cell_count[c], 1u);
+	atomicAdd(&cell_count[c], 1u);
+}
+
+// k_count's work for agent k of the next step, done by the step kernel that computed its
+// new position (same key: the class slot of the launch, the world's fixed grid).
+UD void count_next(const StepArgs& A, const WorldInfo& wi, int wix, int k, float2 p) {
+	const GridInfo g = A.grids[wix];
+	const int c = A.next_key_base + wi.cell_offset + bin_axis(p.y, g.oy, g.invy, g.ncy) * g.ncx +
+	              bin_axis(p.x, g.ox, g.invx, g.ncx);
+	A.next_cell_of[k] = (unsigned)c;
+	atomicAdd(&A.next_count[c], 1u);
 }
 
 // ---------------------------------------------------------------- scan (3 passes)
@@ -180,18 +190,13 @@ UD unsigned block_exclusive_scan(unsigned v, unsigned* sh, unsigned& total) {
 	return wofs + x - v;
 }
 
-// Reads the counts and leaves them zero for the next step's k_count.
-__global__ __launch_bounds__(kScanThreads) void k_scan_local(unsigned* in, unsigned* out, unsigned* sums, int n) {
+__global__ __launch_bounds__(kScanThreads) void k_scan_local(const unsigned* in, unsigned* out, unsigned* sums, int n) {
 	__shared__ unsigned sh[kScanThreads / 64];
 	const int base = blockIdx.x * kScanTile + threadIdx.x * kScanItems;
 	unsigned v[kScanItems];
 	unsigned s = 0;
 	for (int i = 0; i < kScanItems; ++i) {
-		v[i] = 0u;
-		if (base + i < n) {
-			v[i] = in[base + i];
-			in[base + i] = 0u;
-		}
+		v[i] = (base + i < n) ? in[base + i] : 0u;
 		s += v[i];
 	}
 	unsigned total;
@@ -232,18 +237,21 @@ UD unsigned cell_first_of(const unsigned* cs, unsigned c, int nslots, unsigned s
 	return t;
 }
 
-// Thread i < n: agent i, with key (slot, cell) and rank r, goes to d = cell_first(cell)
-// + (agents of the lower slots in that cell: differences of class_start) + r and is
-// entry class_start[key] + r of the class lists. Thread i <= slot_cells (with several
-// slots): cell_first[i] for the neighbour search.
+// Thread i < n: agent i, with key (slot, cell), takes a rank r inside its key by counting
+// the key down (the counters end at zero, ready for the next count; the order inside a
+// key is that of the atomics, which no result depends on: neighbour lists are
+// canonical). It goes to d = cell_first(cell) + (agents of the lower slots in that cell:
+// differences of class_start) + r and is entry class_start[key] + r of the class lists.
+// Thread i <= slot_cells (with several slots): cell_first[i] for the neighbour search.
 __global__ __launch_bounds__(256) void k_reorder(State src, State dst, int n, const unsigned* cell_of,
-                                                const unsigned* rank, const unsigned* class_start, int nslots,
+                                                unsigned* cell_count, const unsigned* class_start, int nslots,
                                                 int slot_cells, unsigned* cell_first, int* class_list) {
 	const int k = blockIdx.x * blockDim.x + threadIdx.x;
 	const unsigned sc = (unsigned)slot_cells;
 	if (nslots > 1 && k <= slot_cells) cell_first[k] = cell_first_of(class_start, (unsigned)k, nslots, sc);
 	if (k >= n) return;
-	const unsigned key = cell_of[k], r = rank[k];
+	const unsigned key = cell_of[k];
+	const unsigned r = atomicSub(&cell_count[key], 1u) - 1u;
 	const unsigned slot = key / sc, cell = key - slot * sc;
 	unsigned d = r;
 	if (nslots > 1) {
@@ -760,6 +768,7 @@ __global__ __launch_bounds__(BS) void k_step(StepArgs A) {
 	A.out.pref[k] = f2(a.pref);
 	meta.w = (int)rng;
 	A.out.meta[k] = meta;
+	if (A.next_count) count_next(A, wi, wix, k, f2(pos));
 }
 
 
@@ -982,6 +991,7 @@ __global__ __launch_bounds__(64) void k_step_coop(StepArgs A) {
 	A.out.prm[k] = S.prm[k];
 	A.out.pref[k] = f2(a.pref);
 	A.out.meta[k] = meta;
+	if (A.next_count) count_next(A, wi, meta.y >> kPolicyBits, k, f2(pos));
 }
 
 } // namespace umans

@@ -116,6 +116,12 @@ struct StepArgs {
 	unsigned* ovf_count;
 	int ovf_base;
 	int tier_final;               // 1: this launch's lists are final (counts go to stats)
+	// optional: the next step's grid count done here (fixed grids only): the key of every
+	// agent, from its new position, written into next_cell_of (indexed like out) and
+	// counted in next_count; next_key_base = this class's slot x slot cells
+	unsigned* next_count;
+	unsigned* next_cell_of;
+	int next_key_base;
 };
 
 // Obstacle edges kept per agent (the generic class only); more are counted in stats[2].
