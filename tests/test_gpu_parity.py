@@ -91,6 +91,27 @@ def test_neighbor_lists_bit_exact_10k(world_type, cap):
         assert st["overflow_agents"] == int((o["nb_full"] > cap).sum())
 
 
+@pytest.mark.parametrize("world_type", [0, 1])
+@pytest.mark.parametrize("cap", [12, 16])
+def test_orca_capped_lists_dense(world_type, cap):
+    """ORCA with lists of at most 12 or 16 neighbours keeps its half-planes in registers
+    (the longer lists of the other tests use the solver over memory). A crowd dense
+    enough that agents overlap and many linear programs are infeasible (linearProgram3)
+    against the FP32 and FP64 oracles."""
+    pol = S.orca_policy(time_horizon=5.0, rng=6.0)
+    sc = S.random_cloud(4000, 40.0, [pol], world_type=world_type, max_neighbors=cap, seed=70 + cap, speed=1.5)
+    a = sc.agents
+    out, _, st = gpu_step(world_type, sc.width, sc.height, sc.dt, [pol], a, cap, record=False)
+    wd = O.make_world(world_type, sc.width, sc.height, sc.dt, cap)
+    o32 = O.step(wd, [pol], a, precision=0)
+    o64 = O.step(wd, [pol], a, precision=1)
+    e64 = rel_err(out["vel"], o64["vel64"])
+    e32 = rel_err(out["vel"], o32["vel"])
+    ok = (e64 <= REL_TOL) | (e32 <= 1e-6)
+    assert ok.all(), f"{(~ok).sum()} agents off"
+    assert st["overflow_agents"] > 0      # the lists are full: the cap is what is tested
+
+
 def test_mixed_crowd_100k_one_step():
     """The mixed crowd (PowerLaw sampling / SocialForces gradient / ORCA) at 100k agents
     and the same density: one step against the FP32 and FP64 oracles."""

@@ -279,8 +279,46 @@ struct Lines {
 };
 constexpr float kOrcaEps = 0.00001f;   // ORCASolver.h:62
 
-// Solver::createAgentOrcaLines (ORCASolver.cpp:96-171), including the collision branch
-// that divides by the squared length (ORCASolver.cpp:161-165).
+// The half-plane of one neighbour (ORCASolver.cpp:104-168), including the collision
+// branch that divides by the squared length (ORCASolver.cpp:161-165).
+UD Line orca_line(const Self& a, const Nb& o, float tau) {
+	const V2 relPos = o.pos - a.pos;
+	const V2 relVel = a.vel - o.vel;
+	const float distSq = sqr_mag(relPos);
+	const float cr = a.radius + o.radius;
+	const float crSq = cr * cr;
+	Line line;
+	V2 u;
+	if (distSq > crSq) {
+		const V2 w = relVel - (relPos / tau);
+		const float wLenSq = sqr_mag(w);
+		const float dp1 = dot(w, relPos);
+		if (dp1 < 0.0f && dp1 * dp1 > crSq * wLenSq) {
+			const float wLen = sqrtf(wLenSq);
+			const V2 unitW = w / wLen;
+			line.dir = V2{unitW.y, -unitW.x};
+			u = unitW * (cr / tau - wLen);
+		} else {
+			const float leg = sqrtf(distSq - crSq);
+			if (det(relPos, w) > 0.0f)
+				line.dir = V2{relPos.x * leg - relPos.y * cr, relPos.x * cr + relPos.y * leg} / distSq;
+			else
+				line.dir = V2{relPos.x * leg + relPos.y * cr, -relPos.x * cr + relPos.y * leg} * -1.0f / distSq;
+			const float dp2 = dot(relVel, line.dir);
+			u = (line.dir * dp2) - relVel;
+		}
+	} else {
+		const V2 w = relVel - (relPos / a.dt);
+		const float wLen = sqr_mag(w);
+		const V2 unitW = w / wLen;
+		line.dir = V2{unitW.y, -unitW.x};
+		u = unitW * (cr / a.dt - wLen);
+	}
+	line.point = a.vel + (u * 0.5f);
+	return line;
+}
+// Solver::createAgentOrcaLines (ORCASolver.cpp:96-171): the neighbours within range, in
+// list order.
 UD int orca_lines(const float* prm, float range, const Self& a, const NbList& L, Lines lines) {
 	const float tau = prm[0];
 	const float maxDistSq = range * range;
@@ -288,40 +326,7 @@ UD int orca_lines(const float* prm, float range, const Self& a, const NbList& L,
 	for (int e = 0; e < L.count; ++e) {
 		const Nb o = L.get(e);
 		if (o.dsq > maxDistSq) continue;
-		const V2 relPos = o.pos - a.pos;
-		const V2 relVel = a.vel - o.vel;
-		const float distSq = sqr_mag(relPos);
-		const float cr = a.radius + o.radius;
-		const float crSq = cr * cr;
-		Line line;
-		V2 u;
-		if (distSq > crSq) {
-			const V2 w = relVel - (relPos / tau);
-			const float wLenSq = sqr_mag(w);
-			const float dp1 = dot(w, relPos);
-			if (dp1 < 0.0f && dp1 * dp1 > crSq * wLenSq) {
-				const float wLen = sqrtf(wLenSq);
-				const V2 unitW = w / wLen;
-				line.dir = V2{unitW.y, -unitW.x};
-				u = unitW * (cr / tau - wLen);
-			} else {
-				const float leg = sqrtf(distSq - crSq);
-				if (det(relPos, w) > 0.0f)
-					line.dir = V2{relPos.x * leg - relPos.y * cr, relPos.x * cr + relPos.y * leg} / distSq;
-				else
-					line.dir = V2{relPos.x * leg + relPos.y * cr, -relPos.x * cr + relPos.y * leg} * -1.0f / distSq;
-				const float dp2 = dot(relVel, line.dir);
-				u = (line.dir * dp2) - relVel;
-			}
-		} else {
-			const V2 w = relVel - (relPos / a.dt);
-			const float wLen = sqr_mag(w);
-			const V2 unitW = w / wLen;
-			line.dir = V2{unitW.y, -unitW.x};
-			u = unitW * (cr / a.dt - wLen);
-		}
-		line.point = a.vel + (u * 0.5f);
-		lines[n++] = line;
+		lines[n++] = orca_line(a, o, tau);
 	}
 	return n;
 }
@@ -396,6 +401,128 @@ UD V2 orca_solve(Lines L, int n, const Self& a, Lines proj) {
 	V2 res;
 	const int fail = lp2(L, n, a.max_speed, a.pref, false, res);
 	if (fail != n) lp3(L, n, fail, a.max_speed, res, proj);
+	return res;
+}
+// The same solver with the half-planes in registers (lists of at most 16 entries): slot
+// e holds neighbour e's half-plane when bit e of `valid` is set, so every index is a
+// compile-time constant once the loops are unrolled, and the lines never touch memory.
+// The valid slots in slot order are the lines of orca_lines in list order, so each loop
+// below visits the lines the loop it mirrors visits, in the same order.
+template <int K>
+struct RegLines {
+	Line l[K];
+	unsigned valid;
+	UD bool has(int e) const { return (valid >> e) & 1u; }
+};
+template <int K>
+UD void orca_lines_reg(const float* prm, float range, const Self& a, const NbList& L, RegLines<K>& R) {
+	const float tau = prm[0];
+	const float maxDistSq = range * range;
+	R.valid = 0u;
+#pragma unroll
+	for (int e = 0; e < K; ++e) {
+		R.l[e] = Line{V2{0.0f, 0.0f}, V2{0.0f, 0.0f}};
+		if (e < L.count) {
+			const Nb o = L.get(e);
+			if (!(o.dsq > maxDistSq)) {
+				R.l[e] = orca_line(a, o, tau);
+				R.valid |= 1u << e;
+			}
+		}
+	}
+}
+// lp1 over the lines before slot `no` (a constant after unrolling)
+template <int K>
+UD bool lp1_reg(const RegLines<K>& R, int no, float radius, V2 opt, bool dirOpt, V2& res) {
+	const Line Ln = R.l[no];
+	const float dp = dot(Ln.point, Ln.dir);
+	const float disc = dp * dp + radius * radius - sqr_mag(Ln.point);
+	if (disc < 0.0f) return false;
+	const float sd = sqrtf(disc);
+	float tl = -dp - sd, tr = -dp + sd;
+	bool ok = true;
+#pragma unroll
+	for (int i = 0; i < K; ++i) {
+		if (i < no && R.has(i) && ok) {
+			const float den = det(Ln.dir, R.l[i].dir);
+			const float num = det(R.l[i].dir, Ln.point - R.l[i].point);
+			if (fabsf(den) <= kOrcaEps) {
+				if (num < 0.0f) ok = false;
+			} else {
+				const float t = num / den;
+				if (den >= 0.0f) tr = smin(tr, t);
+				else tl = smax(tl, t);
+				if (tl > tr) ok = false;
+			}
+		}
+	}
+	if (!ok) return false;
+	if (dirOpt) {
+		res = dot(opt, Ln.dir) > 0.0f ? Ln.point + (Ln.dir * tr) : Ln.point + (Ln.dir * tl);
+	} else {
+		const float t = dot(Ln.dir, opt - Ln.point);
+		if (t < tl) res = Ln.point + (Ln.dir * tl);
+		else if (t > tr) res = Ln.point + (Ln.dir * tr);
+		else res = Ln.point + (Ln.dir * t);
+	}
+	return true;
+}
+// lp2: returns the slot of the line that failed, or K
+template <int K>
+UD int lp2_reg(const RegLines<K>& R, float radius, V2 opt, bool dirOpt, V2& res) {
+	if (dirOpt) res = opt * radius;
+	else if (sqr_mag(opt) > radius * radius) res = normalized(opt) * radius;
+	else res = opt;
+	int fail = K;
+#pragma unroll
+	for (int i = 0; i < K; ++i) {
+		if (fail == K && R.has(i) && det(R.l[i].dir, R.l[i].point - res) > 0.0f) {
+			const V2 tmp = res;
+			if (!lp1_reg<K>(R, i, radius, opt, dirOpt, res)) { res = tmp; fail = i; }
+		}
+	}
+	return fail;
+}
+// lp2 over projected lines in memory, out of line: lp3 calls it from each unrolled slot
+__device__ __noinline__ int lp2_mem(Lines L, int n, float radius, V2 opt, V2* res) {
+	V2 r = *res;
+	const int f = lp2(L, n, radius, opt, true, r);
+	*res = r;
+	return f;
+}
+// lp3 from slot `begin`; the projected lines of a slot go to private memory (rare path)
+template <int K>
+UD void lp3_reg(const RegLines<K>& R, int begin, float radius, V2& res, Lines proj) {
+	float distance = 0.0f;
+#pragma unroll
+	for (int i = 0; i < K; ++i) {
+		if (i >= begin && R.has(i) && det(R.l[i].dir, R.l[i].point - res) > distance) {
+			int np = 0;
+#pragma unroll
+			for (int j = 0; j < i; ++j) {
+				if (!R.has(j)) continue;
+				Line line;
+				const float dt = det(R.l[i].dir, R.l[j].dir);
+				if (fabsf(dt) <= kOrcaEps) {
+					if (dot(R.l[i].dir, R.l[j].dir) > 0.0f) continue;
+					line.point = (R.l[i].point + R.l[j].point) * 0.5f;
+				} else {
+					line.point = R.l[i].point + R.l[i].dir * (det(R.l[j].dir, R.l[i].point - R.l[j].point) / dt);
+				}
+				line.dir = normalized(R.l[j].dir - R.l[i].dir);
+				proj[np++] = line;
+			}
+			V2 r = res;
+			if (lp2_mem(proj, np, radius, V2{-R.l[i].dir.y, R.l[i].dir.x}, &r) >= np) res = r;
+			distance = det(R.l[i].dir, R.l[i].point - res);
+		}
+	}
+}
+template <int K>
+UD V2 orca_solve_reg(const RegLines<K>& R, const Self& a, Lines proj) {
+	V2 res;
+	const int fail = lp2_reg<K>(R, a.max_speed, a.pref, false, res);
+	if (fail < K) lp3_reg<K>(R, fail, a.max_speed, res, proj);
 	return res;
 }
 // ORCA::GetCost (ORCA.cpp:49-86)

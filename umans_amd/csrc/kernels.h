@@ -688,12 +688,21 @@ __global__ __launch_bounds__(BS) void k_step(StepArgs A) {
 		Prep<KMAX, CLS> pr;
 		pr.n_lines = 0;
 		if constexpr (CLS == CLS_ORCA) {
-			Line lines_priv[KMAX];
 			Line proj_priv[KMAX];
-			pr.lines = Lines{lines_priv, 1};
 			const Lines proj{proj_priv, 1};
-			pr.n_lines = orca_lines(P.params[0], P.range[0], a, L, pr.lines);
-			const V2 best = orca_solve(pr.lines, pr.n_lines, a, proj);
+			V2 best;
+			if constexpr (KMAX <= 16) {
+				// half-planes in registers; only the rare projected lines of
+				// linearProgram3 go to private memory
+				RegLines<KMAX> R;
+				orca_lines_reg<KMAX>(P.params[0], P.range[0], a, L, R);
+				best = orca_solve_reg<KMAX>(R, a, proj);
+			} else {
+				Line lines_priv[KMAX];
+				pr.lines = Lines{lines_priv, 1};
+				pr.n_lines = orca_lines(P.params[0], P.range[0], a, L, pr.lines);
+				best = orca_solve(pr.lines, pr.n_lines, a, proj);
+			}
 			const V2 acc = clamp_vec((best - a.vel) / smax(a.relaxation, a.dt), a.max_accel);
 			next = clamp_vec(a.vel + acc * a.dt, a.max_speed);
 		} else {
