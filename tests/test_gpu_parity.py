@@ -112,6 +112,32 @@ def test_orca_capped_lists_dense(world_type, cap):
     assert st["overflow_agents"] > 0      # the lists are full: the cap is what is tested
 
 
+@pytest.mark.parametrize("world_type", [0, 1])
+@pytest.mark.parametrize("case,cap", [("karamouzas_sampling", 16), ("moussaid_sampling", 16),
+                                      ("paris_sampling", 12), ("vantoll_sampling", 16),
+                                      ("ttcadca_sampling", 12)])
+def test_cooperative_sampling_capped_lists(world_type, case, cap):
+    """Regular sampling over per-candidate neighbour costs with lists of at most 12 or 16
+    entries runs four waves per 64 agents with the neighbours staged in LDS (longer lists
+    use one wave and read them from memory): one step of a dense cloud against the FP32
+    and FP64 oracles."""
+    from cases import CASES
+    from umans_amd.registry import pack_policy
+    m, costs, samp, relax = CASES[case]
+    pol = pack_policy(m, costs, sampling=samp, relaxation_time=relax)
+    sc = S.random_cloud(3000, 45.0, [pol], world_type=world_type, max_neighbors=cap, seed=90 + cap)
+    a = sc.agents
+    out, _, st = gpu_step(world_type, sc.width, sc.height, sc.dt, [pol], a, cap, record=False)
+    wd = O.make_world(world_type, sc.width, sc.height, sc.dt, cap)
+    o32 = O.step(wd, [pol], a, precision=0)
+    o64 = O.step(wd, [pol], a, precision=1)
+    e64 = rel_err(out["vel"], o64["vel64"])
+    e32 = rel_err(out["vel"], o32["vel"])
+    ok = (e64 <= REL_TOL) | (e32 <= 1e-6)
+    assert ok.all(), f"{(~ok).sum()} agents off"
+    assert st["overflow_agents"] > 0
+
+
 def test_mixed_crowd_100k_one_step():
     """The mixed crowd (PowerLaw sampling / SocialForces gradient / ORCA) at 100k agents
     and the same density: one step against the FP32 and FP64 oracles."""
