@@ -179,8 +179,9 @@ def main():
     ap.add_argument("--cpu-threads", type=int, default=min(16, os.cpu_count() or 1))
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-autotune", action="store_true")
-    ap.add_argument("--tuning", default="", help="CELL_FACTOR,BLOCK: fix the launch tuning instead of "
-                    "autotuning (profiling runs use the tuning the timed run chose)")
+    ap.add_argument("--tuning", default="", help="CELL_FACTOR,BLOCK[,B0:B1:B2:B3:B4]: fix the launch tuning "
+                    "(per-class block sizes optional, 0 = BLOCK) instead of autotuning (profiling runs use "
+                    "the tuning the timed run chose)")
     ap.add_argument("--no-extras", action="store_true",
                     help="skip the batched-sweep and adjoint legs added to the default line")
     ap.add_argument("--no-graph", action="store_true", help="time eagerly launched steps instead of "
@@ -210,9 +211,11 @@ def main():
         w = gpu.World(sc.world_type, sc.width, sc.height, sc.dt, sc.policies, sc.max_neighbors)
     w.upload(sc.agents)
     if args.tuning:
-        cf, bs = args.tuning.split(",")
-        w.set_tuning(float(cf), int(bs))
-        tuning = {"cell_factor": float(cf), "block_size": int(bs), "fixed": True}
+        parts = args.tuning.split(",")
+        cf, bs = float(parts[0]), int(parts[1])
+        cb = [int(x) for x in parts[2].split(":")] if len(parts) > 2 else [0] * 5
+        w.set_tuning(cf, bs, class_block=cb)
+        tuning = {"cell_factor": cf, "block_size": bs, "class_block": cb, "fixed": True}
     else:
         tuning = None if args.no_autotune else w.autotune(trial_steps=3)
     graph = not args.no_graph

@@ -164,6 +164,11 @@ typedef struct {
 	                              longer being redone at the next capacity; 128: a single
 	                              launch at full capacity; 0 in umans_set_tuning: unchanged.
 	                              Results do not depend on it. */
+	int32_t class_block[5];    /* block size of each policy class's step kernel (force +
+	                              gradient, force + sampling, ORCA, generic, cooperative
+	                              sampling), 0 = block_size; the autotuner refines them one
+	                              class at a time after choosing the rest. The cooperative
+	                              class always runs 64. Results do not depend on them. */
 } umans_tuning;
 int umans_autotune(umans_ctx* ctx, int trial_steps, umans_tuning* out);
 int umans_set_tuning(umans_ctx* ctx, const umans_tuning* tuning);

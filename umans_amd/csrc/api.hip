@@ -131,8 +131,10 @@ void launch_step(int cls, int kmax, int bs, const StepArgs& a, hipStream_t st) {
 }
 
 // The cooperative class works on one wave per block.
-int class_block(int cls, int kmax, int bs) {
+// over: the per-class block sizes (0 = bs)
+int class_block(int cls, int kmax, int bs, const int* over) {
 	if (cls == CLS_COOP) return 64;                 // one wave per block (k_step_coop)
+	if (over && over[cls] > 0 && block_supported(kmax, over[cls])) return over[cls];
 	return bs;
 }
 
@@ -193,6 +195,7 @@ struct umans_ctx {
 	int* d_cls_slot = nullptr;
 	float cell_factor = 1.0f;    // cell width = factor x max interaction range (autotuned)
 	int block = 0;               // step-kernel block size (0 = default; autotuned)
+	int cls_block[NUM_CLS] = {}; // per-class block sizes (0 = block; autotuned)
 
 	int record = 0;
 	int* rec_count = nullptr;
@@ -760,7 +763,7 @@ int enqueue_step(umans_ctx* c, bool events, const State* ysave = nullptr, unsign
 			hipStreamWaitEvent(sq, c->fork_ev, 0);
 		}
 		if (q == CLS_COOP || kmax <= 32 || c->tier_first >= 2) {
-			launch_step(q, kmax, class_block(q, kmax, bs), aq, sq);
+			launch_step(q, kmax, class_block(q, kmax, bs, c->cls_block), aq, sq);
 		} else {
 			// Long lists (no cap, or a cap above 32): a first launch with 32-entry lists
 			// (4x the occupancy of 128) hands the agents whose list is longer to a 64-entry
@@ -783,7 +786,7 @@ int enqueue_step(umans_ctx* c, bool events, const State* ysave = nullptr, unsign
 					at.ovf_base = c->cls_off[q];
 				}
 				const int tk = tiers[ti];
-				launch_step(q, tk, class_block(q, tk, tk <= 32 ? bs : 64), at, sq);
+				launch_step(q, tk, class_block(q, tk, tk <= 32 ? bs : 64, nullptr), at, sq);
 			}
 		}
 		if (!first) hipEventRecord(c->join_ev[q], sq);
@@ -1386,11 +1389,31 @@ int umans_autotune(umans_ctx* c, int trial_steps, umans_tuning* out) {
 	int best_t = tier0;
 	const int rec0 = c->record;
 	c->record = 0;
+	int cb0[NUM_CLS], best_cb[NUM_CLS] = {};
+	for (int q = 0; q < NUM_CLS; ++q) cb0[q] = c->cls_block[q];
 	float best_ms = 3.4e38f, best_f = f0;
 	int best_b = b0 > 0 ? b0 : default_block(kmax);
 	hipEvent_t t0 = nullptr, t1 = nullptr;
 	hipEventCreate(&t0);
 	hipEventCreate(&t1);
+	// time trial_steps steps of the configuration in c (after a warm-up step)
+	auto trial = [&](float& ms) -> int {
+		c->drop_graph();
+		int r;
+		if ((r = setup_worlds(c, per_world))) return r;
+		if ((r = c->hip(copy_state(c->X.s, c->B.s, c->n, c->stream), "restore"))) return r;
+		if ((r = enqueue_step(c, false))) return r;   // warm-up
+		hipEventRecord(t0, c->stream);
+		for (int s = 0; s < trial_steps && !r; ++s) r = enqueue_step(c, false);
+		hipEventRecord(t1, c->stream);
+		if (r) return r;
+		if ((r = c->hip(hipEventSynchronize(t1), "autotune"))) return r;
+		ms = 0.0f;
+		hipEventElapsedTime(&ms, t0, t1);
+		return UMANS_OK;
+	};
+	// 1. cell width, block size and first list capacity for every class together
+	for (int q = 0; q < NUM_CLS; ++q) c->cls_block[q] = 0;
 	for (float f : factors) {
 		for (int b : blocks)
 		for (int ti = 0; ti < n_tiers; ++ti) {
@@ -1398,20 +1421,29 @@ int umans_autotune(umans_ctx* c, int trial_steps, umans_tuning* out) {
 			c->cell_factor = f;
 			c->block = b;
 			c->tier_first = n_tiers > 1 ? ti : tier0;
-			c->drop_graph();
-			if ((rc = setup_worlds(c, per_world))) break;
-			if ((rc = c->hip(copy_state(c->X.s, c->B.s, c->n, c->stream), "restore"))) break;
-			if ((rc = enqueue_step(c, false))) break;   // warm-up
-			hipEventRecord(t0, c->stream);
-			for (int s = 0; s < trial_steps && !rc; ++s) rc = enqueue_step(c, false);
-			hipEventRecord(t1, c->stream);
-			if (rc) break;
-			if ((rc = c->hip(hipEventSynchronize(t1), "autotune"))) break;
-			float ms = 0.0f;
-			hipEventElapsedTime(&ms, t0, t1);
+			float ms;
+			if ((rc = trial(ms))) break;
 			if (ms < best_ms) { best_ms = ms; best_f = f; best_b = b; best_t = c->tier_first; }
 		}
 		if (rc) break;
+	}
+	// 2. then each class's block size in turn, the others kept at their best (the class
+	// kernels share the GPU, so one class's best depends on the others')
+	if (!rc) {
+		c->cell_factor = best_f;
+		c->block = best_b;
+		c->tier_first = best_t;
+		for (int q = 0; q < NUM_CLS && !rc; ++q) {
+			if (q == CLS_COOP || c->cls_count[q] == 0) continue;
+			for (int b : blocks) {
+				if (b == best_b || !block_supported(kmax, b)) continue;
+				for (int p = 0; p < NUM_CLS; ++p) c->cls_block[p] = best_cb[p];
+				c->cls_block[q] = b;
+				float ms;
+				if ((rc = trial(ms))) break;
+				if (ms < best_ms) { best_ms = ms; for (int p = 0; p < NUM_CLS; ++p) best_cb[p] = c->cls_block[p]; }
+			}
+		}
 	}
 	hipEventDestroy(t0);
 	hipEventDestroy(t1);
@@ -1419,6 +1451,7 @@ int umans_autotune(umans_ctx* c, int trial_steps, umans_tuning* out) {
 	c->cell_factor = rc ? f0 : best_f;
 	c->block = rc ? b0 : best_b;
 	c->tier_first = rc ? tier0 : best_t;
+	for (int q = 0; q < NUM_CLS; ++q) c->cls_block[q] = rc ? cb0[q] : best_cb[q];
 	c->record = rec0;
 	c->drop_graph();
 	int rc2 = setup_worlds(c, per_world);
@@ -1431,6 +1464,7 @@ int umans_autotune(umans_ctx* c, int trial_steps, umans_tuning* out) {
 		out->block_size = c->block;
 		out->ms_per_step = best_ms / trial_steps;
 		out->list_capacity = c->tier_first == 0 ? 32 : (c->tier_first == 1 ? 64 : 128);
+		for (int q = 0; q < NUM_CLS; ++q) out->class_block[q] = c->cls_block[q];
 	}
 	return UMANS_OK;
 }
@@ -1441,11 +1475,15 @@ int umans_set_tuning(umans_ctx* c, const umans_tuning* t) {
 		return c->fail(UMANS_EINVAL, "unsupported tuning");
 	if (t->list_capacity != 0 && t->list_capacity != 32 && t->list_capacity != 64 && t->list_capacity != 128)
 		return c->fail(UMANS_EINVAL, "list_capacity must be 0, 32, 64 or 128");
+	for (int q = 0; q < NUM_CLS; ++q)
+		if (t->class_block[q] != 0 && !block_supported(kmax_bucket(c->kcap), t->class_block[q]))
+			return c->fail(UMANS_EINVAL, "unsupported class block size");
 	hipSetDevice(c->device);
 	int rc = c->hip(hipStreamSynchronize(c->stream), "tuning");
 	if (rc) return rc;
 	c->cell_factor = t->cell_factor;
 	c->block = t->block_size;
+	for (int q = 0; q < NUM_CLS; ++q) c->cls_block[q] = t->class_block[q];
 	if (t->list_capacity) c->tier_first = t->list_capacity == 32 ? 0 : (t->list_capacity == 64 ? 1 : 2);
 	c->drop_graph();
 	if (c->n > 0) {

@@ -70,7 +70,7 @@ class Profile(C.Structure):
 
 class Tuning(C.Structure):
     _fields_ = [("cell_factor", C.c_float), ("block_size", C.c_int32), ("ms_per_step", C.c_float),
-                ("list_capacity", C.c_int32)]
+                ("list_capacity", C.c_int32), ("class_block", C.c_int32 * 5)]
 
 
 class AdjointDesc(C.Structure):
@@ -380,10 +380,13 @@ class World:
         t = Tuning()
         self._check(lib().umans_autotune(self._h, int(trial_steps), C.byref(t)))
         return {"cell_factor": t.cell_factor, "block_size": t.block_size, "ms_per_step": t.ms_per_step,
-                "list_capacity": t.list_capacity}
+                "list_capacity": t.list_capacity, "class_block": list(t.class_block)}
 
-    def set_tuning(self, cell_factor, block_size=0, list_capacity=0):
-        t = Tuning(float(cell_factor), int(block_size), 0.0, int(list_capacity))
+    def set_tuning(self, cell_factor, block_size=0, list_capacity=0, class_block=(0, 0, 0, 0, 0)):
+        """class_block: block size per policy class (force + gradient, force + sampling,
+        ORCA, generic, cooperative sampling); 0 = block_size."""
+        t = Tuning(float(cell_factor), int(block_size), 0.0, int(list_capacity),
+                   (C.c_int32 * 5)(*[int(b) for b in class_block]))
         self._check(lib().umans_set_tuning(self._h, C.byref(t)))
 
     def set_policies(self, policies):
