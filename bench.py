@@ -217,7 +217,7 @@ def main():
         w.set_tuning(cf, bs, class_block=cb)
         tuning = {"cell_factor": cf, "block_size": bs, "class_block": cb, "fixed": True}
     else:
-        tuning = None if args.no_autotune else w.autotune(trial_steps=3)
+        tuning = None if args.no_autotune else w.autotune(trial_steps=10)
     graph = not args.no_graph
     # the timed run: replays of the captured step (or eager launches with --no-graph)
     w.set_graph(graph)

@@ -1380,7 +1380,7 @@ int umans_autotune(umans_ctx* c, int trial_steps, umans_tuning* out) {
 	if (!c->B.s.pos && (rc = c->hip(c->B.alloc(c->cap), "alloc backup"))) return rc;
 	if ((rc = c->hip(copy_state(c->B.s, c->X.s, c->n, c->stream), "backup"))) return rc;
 	const int kmax = kmax_bucket(c->kcap);
-	const float factors[] = {0.5f, 0.75f, 1.0f, 1.5f};
+	const float factors[] = {0.5f, 0.625f, 0.75f, 0.875f, 1.0f, 1.5f};
 	const int blocks[] = {64, 128, 256};
 	const float f0 = c->cell_factor;
 	const int b0 = c->block;
