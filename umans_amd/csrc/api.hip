@@ -41,8 +41,8 @@ struct DeviceState {
 		return dalloc(&s.meta, n);
 	}
 	void release() {
-		hipFree(s.pos); hipFree(s.vel); hipFree(s.goal); hipFree(s.viewdir);
-		hipFree(s.pref); hipFree(s.prm); hipFree(s.meta);
+		(void)hipFree(s.pos); (void)hipFree(s.vel); (void)hipFree(s.goal); (void)hipFree(s.viewdir);
+		(void)hipFree(s.pref); (void)hipFree(s.prm); (void)hipFree(s.meta);
 		s = State{};
 	}
 };
@@ -292,8 +292,8 @@ struct umans_ctx {
 		return UMANS_EHIP;
 	}
 	void drop_graph() {
-		if (graph_exec) hipGraphExecDestroy(graph_exec);
-		if (graph) hipGraphDestroy(graph);
+		if (graph_exec) (void)hipGraphExecDestroy(graph_exec);
+		if (graph) (void)hipGraphDestroy(graph);
 		graph_exec = nullptr;
 		graph = nullptr;
 	}
@@ -302,8 +302,8 @@ struct umans_ctx {
 		for (DeviceState& d : seg_states) d.release();
 		ckpts.clear();
 		seg_states.clear();
-		hipFree(targets); hipFree(seg_nb); hipFree(seg_cnt); hipFree(seg_lg); hipFree(adj_buf);
-		hipFree(adj_params); hipFree(dloss);
+		(void)hipFree(targets); (void)hipFree(seg_nb); (void)hipFree(seg_cnt); (void)hipFree(seg_lg); (void)hipFree(adj_buf);
+		(void)hipFree(adj_params); (void)hipFree(dloss);
 		targets = nullptr; seg_nb = nullptr; seg_cnt = nullptr; seg_lg = nullptr; adj_buf = nullptr;
 		adj_params = nullptr; dloss = nullptr;
 		adj_ready = false;
@@ -311,17 +311,17 @@ struct umans_ctx {
 	}
 	void free_csv_buffers() {
 		for (int i = 0; i < 2; ++i) {
-			if (csv_pos[i]) hipHostFree(csv_pos[i]);
-			if (csv_meta[i]) hipHostFree(csv_meta[i]);
-			if (csv_ev[i]) hipEventDestroy(csv_ev[i]);
+			if (csv_pos[i]) (void)hipHostFree(csv_pos[i]);
+			if (csv_meta[i]) (void)hipHostFree(csv_meta[i]);
+			if (csv_ev[i]) (void)hipEventDestroy(csv_ev[i]);
 			csv_pos[i] = nullptr; csv_meta[i] = nullptr; csv_ev[i] = nullptr;
 			csv_pending[i] = false;
 		}
 		csv_cap = 0;
 	}
 	void free_pop_buffers() {
-		hipFree(d_remove); hipFree(d_rows); hipFree(d_inv); hipFree(d_pop_count);
-		if (h_pop_count) hipHostFree(h_pop_count);
+		(void)hipFree(d_remove); (void)hipFree(d_rows); (void)hipFree(d_inv); (void)hipFree(d_pop_count);
+		if (h_pop_count) (void)hipHostFree(h_pop_count);
 		d_remove = nullptr; d_rows = nullptr; d_inv = nullptr; d_pop_count = nullptr; h_pop_count = nullptr;
 		pop_alloc = 0;
 	}
@@ -329,10 +329,10 @@ struct umans_ctx {
 		drop_graph();
 		free_adjoint();
 		X.release(); Y.release(); B.release();
-		hipFree(cell_of); hipFree(ovf[0]); hipFree(ovf[1]); hipFree(class_list);
+		(void)hipFree(cell_of); (void)hipFree(ovf[0]); (void)hipFree(ovf[1]); (void)hipFree(class_list);
 		ovf[0] = ovf[1] = nullptr;
 		class_list = nullptr;
-		hipFree(rec_count); hipFree(rec_id); hipFree(rec_d2); hipFree(rec_seg);
+		(void)hipFree(rec_count); (void)hipFree(rec_id); (void)hipFree(rec_d2); (void)hipFree(rec_seg);
 		cell_of = nullptr;
 		rec_count = rec_id = rec_seg = nullptr;
 		rec_d2 = nullptr;
@@ -355,13 +355,13 @@ struct umans_ctx {
 	hipEvent_t next_event() {
 		if (ev_used == (int)ev.size()) {
 			hipEvent_t e = nullptr;
-			hipEventCreate(&e);
+			if (hipEventCreate(&e) != hipSuccess) e = nullptr;   // a null event fails its record, which is checked
 			ev.push_back(e);
 		}
 		return ev[ev_used++];
 	}
 	void free_events() {
-		for (hipEvent_t e : ev) hipEventDestroy(e);
+		for (hipEvent_t e : ev) (void)hipEventDestroy(e);
 		ev.clear();
 		ev_used = 0;
 	}
@@ -530,7 +530,7 @@ int setup_worlds(umans_ctx* c, const std::vector<int>& per_world) {
 	c->total_cells = (int)off;
 	if ((long long)NUM_CLS * off + 1 > (1LL << 31) - 1) return c->fail(UMANS_EINVAL, "grid too large");
 	if (NUM_CLS * c->total_cells + 1 > c->cell_alloc) {
-		hipFree(c->cell_count); hipFree(c->cell_start); hipFree(c->cell_first); hipFree(c->scan_sums);
+		(void)hipFree(c->cell_count); (void)hipFree(c->cell_start); (void)hipFree(c->cell_first); (void)hipFree(c->scan_sums);
 		c->cell_count = c->cell_start = c->cell_first = c->scan_sums = nullptr;
 		c->cell_alloc = 0;
 		const int cells = NUM_CLS * c->total_cells + 1;
@@ -684,7 +684,7 @@ int enqueue_step(umans_ctx* c, bool events, const State* ysave = nullptr, unsign
 			if (rc) return rc;
 		}
 		e0 = c->next_event(); e1 = c->next_event(); e2 = c->next_event();
-		hipEventRecord(e0, st);
+		if (int rc = c->hip(hipEventRecord(e0, st), "profile event")) return rc;
 	}
 	const bool have_keys = c->keys_ready && !c->any_infinite;
 	const bool fuse = fuse_next && !c->any_infinite;
@@ -745,10 +745,14 @@ int enqueue_step(umans_ctx* c, bool events, const State* ysave = nullptr, unsign
 	}
 	const int kmax = kmax_bucket(c->kcap);
 	const int bs = c->block > 0 ? c->block : default_block(kmax);
-	if (e1) hipEventRecord(e1, st);
+	if (e1) {
+		if (int rc = c->hip(hipEventRecord(e1, st), "profile event")) return rc;
+	}
 	int active = 0;
 	for (int q = 0; q < NUM_CLS; ++q) active += c->cls_count[q] > 0;
-	if (active > 1) hipEventRecord(c->fork_ev, st);
+	if (active > 1) {
+		if (int rc = c->hip(hipEventRecord(c->fork_ev, st), "fork")) return rc;
+	}
 	bool first = true;
 	for (int qi = 0; qi < NUM_CLS; ++qi) {
 		const int q = kLaunchOrder[qi];
@@ -760,7 +764,7 @@ int enqueue_step(umans_ctx* c, bool events, const State* ysave = nullptr, unsign
 		hipStream_t sq = st;
 		if (!first) {
 			sq = c->cls_stream[q];
-			hipStreamWaitEvent(sq, c->fork_ev, 0);
+			if (int rc = c->hip(hipStreamWaitEvent(sq, c->fork_ev, 0), "fork")) return rc;
 		}
 		if (q == CLS_COOP || kmax <= 32 || c->tier_first >= 2) {
 			launch_step(q, kmax, class_block(q, kmax, bs, c->cls_block), aq, sq);
@@ -789,7 +793,9 @@ int enqueue_step(umans_ctx* c, bool events, const State* ysave = nullptr, unsign
 				launch_step(q, tk, class_block(q, tk, tk <= 32 ? bs : 64, nullptr), at, sq);
 			}
 		}
-		if (!first) hipEventRecord(c->join_ev[q], sq);
+		if (!first) {
+			if (int rc = c->hip(hipEventRecord(c->join_ev[q], sq), "join")) return rc;
+		}
 		first = false;
 	}
 	if (active > 1) {
@@ -797,11 +803,15 @@ int enqueue_step(umans_ctx* c, bool events, const State* ysave = nullptr, unsign
 		for (int qi = 0; qi < NUM_CLS; ++qi) {
 			const int q = kLaunchOrder[qi];
 			if (!c->cls_count[q]) continue;
-			if (!first) hipStreamWaitEvent(st, c->join_ev[q], 0);
+			if (!first) {
+				if (int rc = c->hip(hipStreamWaitEvent(st, c->join_ev[q], 0), "join")) return rc;
+			}
 			first = false;
 		}
 	}
-	if (e2) hipEventRecord(e2, st);
+	if (e2) {
+		if (int rc = c->hip(hipEventRecord(e2, st), "profile event")) return rc;
+	}
 	const int rc = c->hip(hipGetLastError(), "step launch");
 	if (!rc) {
 		c->counts_zero_n = fuse ? -1 : scan_n;
@@ -818,7 +828,7 @@ int capture_graph(umans_ctx* c, bool fuse_next) {
 	hipGraph_t g = nullptr;
 	const hipError_t e = hipStreamEndCapture(c->stream, &g);
 	if (erc) {
-		if (g) hipGraphDestroy(g);
+		if (g) (void)hipGraphDestroy(g);
 		return erc;
 	}
 	if ((rc = c->hip(e, "graph capture"))) return rc;
@@ -850,7 +860,7 @@ void open_edits(umans_ctx* c) {
 int ensure_pop_buffers(umans_ctx* c, int n) {
 	if (c->pop_alloc >= n && c->d_pop_count) return UMANS_OK;
 	const int cap = std::max(n, 64);
-	hipFree(c->d_remove); hipFree(c->d_rows); hipFree(c->d_inv);
+	(void)hipFree(c->d_remove); (void)hipFree(c->d_rows); (void)hipFree(c->d_inv);
 	c->d_remove = nullptr; c->d_rows = nullptr; c->d_inv = nullptr;
 	c->pop_alloc = 0;
 	int rc;
@@ -924,7 +934,7 @@ int apply_device_edits(umans_ctx* c) {
 		                   c->d_inv, d_ed, (int)ed.size());
 		rc = c->hip(hipStreamSynchronize(c->stream), "edits");
 	}
-	hipFree(d_ed);
+	(void)hipFree(d_ed);
 	return rc;
 }
 
@@ -1190,23 +1200,23 @@ int umans_num_worlds(const umans_ctx* c) { return c ? (int)c->worlds.size() : 0;
 
 void umans_destroy(umans_ctx* c) {
 	if (!c) return;
-	hipSetDevice(c->device);
-	if (c->stream) hipStreamSynchronize(c->stream);
+	(void)hipSetDevice(c->device);
+	if (c->stream) (void)hipStreamSynchronize(c->stream);
 	if (c->csv_on) csv_flush(c);
 	c->free_csv_buffers();
 	c->free_pop_buffers();
 	c->free_agents();
-	hipFree(c->dgrids); hipFree(c->dworlds); hipFree(c->dbbox); hipFree(c->dstats); hipFree(c->dpol);
-	hipFree(c->d_edges); hipFree(c->ovf_count);
-	hipFree(c->d_cls_slot);
-	hipFree(c->cell_count); hipFree(c->cell_start); hipFree(c->cell_first); hipFree(c->scan_sums);
+	(void)hipFree(c->dgrids); (void)hipFree(c->dworlds); (void)hipFree(c->dbbox); (void)hipFree(c->dstats); (void)hipFree(c->dpol);
+	(void)hipFree(c->d_edges); (void)hipFree(c->ovf_count);
+	(void)hipFree(c->d_cls_slot);
+	(void)hipFree(c->cell_count); (void)hipFree(c->cell_start); (void)hipFree(c->cell_first); (void)hipFree(c->scan_sums);
 	c->free_events();
 	for (int q = 0; q < NUM_CLS; ++q) {
-		if (c->cls_stream[q]) hipStreamDestroy(c->cls_stream[q]);
-		if (c->join_ev[q]) hipEventDestroy(c->join_ev[q]);
+		if (c->cls_stream[q]) (void)hipStreamDestroy(c->cls_stream[q]);
+		if (c->join_ev[q]) (void)hipEventDestroy(c->join_ev[q]);
 	}
-	if (c->fork_ev) hipEventDestroy(c->fork_ev);
-	if (c->stream) hipStreamDestroy(c->stream);
+	if (c->fork_ev) (void)hipEventDestroy(c->fork_ev);
+	if (c->stream) (void)hipStreamDestroy(c->stream);
 	delete c;
 }
 
@@ -1226,7 +1236,7 @@ int umans_set_policies(umans_ctx* c, const umans_policy* p, int n) {
 	if (c->n > 0 && mr > c->max_range) return c->fail(UMANS_ESTATE, "a larger interaction range needs a new upload");
 	if (c->n > 0 && n < (int)c->hpol.size())
 		return c->fail(UMANS_ESTATE, "uploaded agents may use every policy: the table cannot shrink before a new upload");
-	hipSetDevice(c->device);
+	if (const int rc_dev = c->hip(hipSetDevice(c->device), "set device")) return rc_dev;
 	const int rc = c->hip(hipMemcpyAsync(c->dpol, v.data(), n * sizeof(DevPolicy), hipMemcpyHostToDevice, c->stream),
 	                      "policy upload");
 	if (rc) return rc;
@@ -1278,7 +1288,7 @@ int umans_upload(umans_ctx* c, const umans_agents* a, int n) {
 		}
 		meta[i] = make_int4(a->id[i], a->policy[i] | (w << kPolicyBits), i, (int)r);
 	}
-	hipSetDevice(c->device);
+	if (const int rc_dev = c->hip(hipSetDevice(c->device), "set device")) return rc_dev;
 	if (c->csv_on) {                                  // the logged rows belong to the old agent set
 		const int rc0 = csv_flush(c);
 		c->csv_on = false;
@@ -1334,7 +1344,7 @@ int umans_set_record_neighbors(umans_ctx* c, int enable) {
 	if (!c) return UMANS_EINVAL;
 	if (enable && !c->rec_count) {
 		if (c->cap <= 0) return c->fail(UMANS_ESTATE, "upload agents before enabling the neighbour record");
-		hipSetDevice(c->device);
+		if (const int rc_dev = c->hip(hipSetDevice(c->device), "set device")) return rc_dev;
 		const int stride = kmax_bucket(c->kcap);
 		int rc;
 		if ((rc = c->hip(dalloc(&c->rec_count, c->cap), "alloc record"))) return rc;
@@ -1350,7 +1360,7 @@ int umans_set_record_neighbors(umans_ctx* c, int enable) {
 
 int umans_set_graph(umans_ctx* c, int enable) {
 	if (!c) return UMANS_EINVAL;
-	hipSetDevice(c->device);
+	if (const int rc_dev = c->hip(hipSetDevice(c->device), "set device")) return rc_dev;
 	const int rc = c->hip(hipStreamSynchronize(c->stream), "graph");
 	c->use_graph = enable ? 1 : 0;
 	c->drop_graph();
@@ -1361,7 +1371,7 @@ int umans_step(umans_ctx* c, int nsteps) {
 	if (!c) return UMANS_EINVAL;
 	if (nsteps < 0) return c->fail(UMANS_EINVAL, "negative step count");
 	if (c->hpol.empty()) return c->fail(UMANS_ESTATE, "no policies");
-	hipSetDevice(c->device);
+	if (const int rc_dev = c->hip(hipSetDevice(c->device), "set device")) return rc_dev;
 	const int rc = run_steps(c, nsteps);
 	if (rc) return rc;
 	if (c->record && nsteps > 0 && c->rec_count) c->have_record = true;
@@ -1371,7 +1381,7 @@ int umans_step(umans_ctx* c, int nsteps) {
 int umans_autotune(umans_ctx* c, int trial_steps, umans_tuning* out) {
 	if (!c) return UMANS_EINVAL;
 	if (trial_steps < 1) trial_steps = 3;
-	hipSetDevice(c->device);
+	if (const int rc_dev = c->hip(hipSetDevice(c->device), "set device")) return rc_dev;
 	int rc;
 	if ((rc = sync_population(c))) return rc;
 	if (c->n == 0) return c->fail(UMANS_ESTATE, "upload agents before autotuning");
@@ -1394,8 +1404,10 @@ int umans_autotune(umans_ctx* c, int trial_steps, umans_tuning* out) {
 	float best_ms = 3.4e38f, best_f = f0;
 	int best_b = b0 > 0 ? b0 : default_block(kmax);
 	hipEvent_t t0 = nullptr, t1 = nullptr;
-	hipEventCreate(&t0);
-	hipEventCreate(&t1);
+	if ((rc = c->hip(hipEventCreate(&t0), "autotune")) || (rc = c->hip(hipEventCreate(&t1), "autotune"))) {
+		if (t0) (void)hipEventDestroy(t0);
+		return rc;
+	}
 	// time trial_steps steps of the configuration in c (after a warm-up step)
 	auto trial = [&](float& ms) -> int {
 		c->drop_graph();
@@ -1403,14 +1415,13 @@ int umans_autotune(umans_ctx* c, int trial_steps, umans_tuning* out) {
 		if ((r = setup_worlds(c, per_world))) return r;
 		if ((r = c->hip(copy_state(c->X.s, c->B.s, c->n, c->stream), "restore"))) return r;
 		if ((r = enqueue_step(c, false))) return r;   // warm-up
-		hipEventRecord(t0, c->stream);
+		if ((r = c->hip(hipEventRecord(t0, c->stream), "autotune"))) return r;
 		for (int s = 0; s < trial_steps && !r; ++s) r = enqueue_step(c, false);
-		hipEventRecord(t1, c->stream);
 		if (r) return r;
+		if ((r = c->hip(hipEventRecord(t1, c->stream), "autotune"))) return r;
 		if ((r = c->hip(hipEventSynchronize(t1), "autotune"))) return r;
 		ms = 0.0f;
-		hipEventElapsedTime(&ms, t0, t1);
-		return UMANS_OK;
+		return c->hip(hipEventElapsedTime(&ms, t0, t1), "autotune");
 	};
 	// 1. cell width, block size and first list capacity for every class together
 	for (int q = 0; q < NUM_CLS; ++q) c->cls_block[q] = 0;
@@ -1445,8 +1456,8 @@ int umans_autotune(umans_ctx* c, int trial_steps, umans_tuning* out) {
 			}
 		}
 	}
-	hipEventDestroy(t0);
-	hipEventDestroy(t1);
+	(void)hipEventDestroy(t0);
+	(void)hipEventDestroy(t1);
 	// restore the configuration chosen (or the previous one on failure) and the state
 	c->cell_factor = rc ? f0 : best_f;
 	c->block = rc ? b0 : best_b;
@@ -1478,7 +1489,7 @@ int umans_set_tuning(umans_ctx* c, const umans_tuning* t) {
 	for (int q = 0; q < NUM_CLS; ++q)
 		if (t->class_block[q] != 0 && !block_supported(kmax_bucket(c->kcap), t->class_block[q]))
 			return c->fail(UMANS_EINVAL, "unsupported class block size");
-	hipSetDevice(c->device);
+	if (const int rc_dev = c->hip(hipSetDevice(c->device), "set device")) return rc_dev;
 	int rc = c->hip(hipStreamSynchronize(c->stream), "tuning");
 	if (rc) return rc;
 	c->cell_factor = t->cell_factor;
@@ -1499,7 +1510,7 @@ int umans_csv_start(umans_ctx* c, const char* dirname, int flush_immediately) {
 	if (!c || !dirname || !*dirname) return UMANS_EINVAL;
 	if (c->worlds.size() != 1) return c->fail(UMANS_EINVAL, "CSV output is per world: use a single-world context");
 	if (c->csv_on) {
-		hipSetDevice(c->device);
+		if (const int rc_dev = c->hip(hipSetDevice(c->device), "set device")) return rc_dev;
 		const int rc0 = csv_flush(c);
 		if (rc0) return rc0;
 	}
@@ -1510,7 +1521,7 @@ int umans_csv_start(umans_ctx* c, const char* dirname, int flush_immediately) {
 	struct stat sb;
 	if (stat(dir.c_str(), &sb) != 0 || !S_ISDIR(sb.st_mode))
 		return c->fail(UMANS_EINVAL, "could not set CSV output directory to " + dir);
-	hipSetDevice(c->device);
+	if (const int rc_dev = c->hip(hipSetDevice(c->device), "set device")) return rc_dev;
 	int rc = sync_population(c);
 	if (!rc) rc = c->hip(hipStreamSynchronize(c->stream), "csv");
 	if (rc) return rc;
@@ -1528,14 +1539,14 @@ int umans_csv_start(umans_ctx* c, const char* dirname, int flush_immediately) {
 int umans_csv_flush(umans_ctx* c) {
 	if (!c) return UMANS_EINVAL;
 	if (!c->csv_on) return UMANS_OK;
-	hipSetDevice(c->device);
+	if (const int rc_dev = c->hip(hipSetDevice(c->device), "set device")) return rc_dev;
 	return csv_flush(c);
 }
 
 int umans_csv_stop(umans_ctx* c) {
 	if (!c) return UMANS_EINVAL;
 	if (!c->csv_on) return UMANS_OK;
-	hipSetDevice(c->device);
+	if (const int rc_dev = c->hip(hipSetDevice(c->device), "set device")) return rc_dev;
 	const int rc = csv_flush(c);
 	c->csv_on = false;
 	c->csv_log.clear();
@@ -1544,7 +1555,7 @@ int umans_csv_stop(umans_ctx* c) {
 
 int umans_set_profiling(umans_ctx* c, int enable) {
 	if (!c) return UMANS_EINVAL;
-	hipSetDevice(c->device);
+	if (const int rc_dev = c->hip(hipSetDevice(c->device), "set device")) return rc_dev;
 	int rc = c->hip(hipStreamSynchronize(c->stream), "profiling");
 	if (rc) return rc;
 	c->ev_used = 0;
@@ -1556,7 +1567,7 @@ int umans_set_profiling(umans_ctx* c, int enable) {
 
 int umans_get_profile(umans_ctx* c, umans_profile* out) {
 	if (!c || !out) return UMANS_EINVAL;
-	hipSetDevice(c->device);
+	if (const int rc_dev = c->hip(hipSetDevice(c->device), "set device")) return rc_dev;
 	const int rc = c->drain_profile();
 	if (rc) return rc;
 	out->steps = c->prof_steps;
@@ -1567,13 +1578,13 @@ int umans_get_profile(umans_ctx* c, umans_profile* out) {
 
 int umans_synchronize(umans_ctx* c) {
 	if (!c) return UMANS_EINVAL;
-	hipSetDevice(c->device);
+	if (const int rc_dev = c->hip(hipSetDevice(c->device), "set device")) return rc_dev;
 	return c->hip(hipStreamSynchronize(c->stream), "synchronize");
 }
 
 int umans_get_stats(umans_ctx* c, umans_stats* out) {
 	if (!c || !out) return UMANS_EINVAL;
-	hipSetDevice(c->device);
+	if (const int rc_dev = c->hip(hipSetDevice(c->device), "set device")) return rc_dev;
 	unsigned long long h[3] = {0, 0, 0};
 	int rc = c->hip(hipMemcpyAsync(h, c->dstats, sizeof(h), hipMemcpyDeviceToHost, c->stream), "stats");
 	if (rc) return rc;
@@ -1588,7 +1599,7 @@ int umans_get_stats(umans_ctx* c, umans_stats* out) {
 
 int umans_download(umans_ctx* c, umans_agents* a) {
 	if (!c || !a) return UMANS_EINVAL;
-	hipSetDevice(c->device);
+	if (const int rc_dev = c->hip(hipSetDevice(c->device), "set device")) return rc_dev;
 	if (int rc0 = sync_population(c)) return rc0;
 	const int n = c->n;
 	std::vector<float2> pos(n), vel(n), view(n), goal(n), pref(n);
@@ -1627,7 +1638,7 @@ int umans_download(umans_ctx* c, umans_agents* a) {
 int umans_dump_neighbors(umans_ctx* c, int stride, int32_t* count, int32_t* ids, double* d2, int32_t* seg) {
 	if (!c || !count || stride < 0 || (stride > 0 && (!ids || !d2 || !seg))) return UMANS_EINVAL;
 	if (!c->have_record) return c->fail(UMANS_ESTATE, "no neighbour record: enable it and run a step");
-	hipSetDevice(c->device);
+	if (const int rc_dev = c->hip(hipSetDevice(c->device), "set device")) return rc_dev;
 	const int n = c->n, rs = c->rec_stride;
 	std::vector<int> hc(n), hid((size_t)n * rs), hseg((size_t)n * rs);
 	std::vector<double> hd((size_t)n * rs);
@@ -1654,7 +1665,7 @@ int umans_dump_neighbors(umans_ctx* c, int stride, int32_t* count, int32_t* ids,
 
 int umans_adjoint_forward(umans_ctx* c, const umans_adjoint_desc* d, const float* targets, double* loss) {
 	if (!c || !d || !targets) return UMANS_EINVAL;
-	hipSetDevice(c->device);
+	if (const int rc_dev = c->hip(hipSetDevice(c->device), "set device")) return rc_dev;
 	if (int rc0 = sync_population(c)) return rc0;
 	if (c->n == 0) return c->fail(UMANS_ESTATE, "upload agents first");
 	if (!c->pending.empty() || c->n_remove > 0)
@@ -1667,7 +1678,7 @@ int umans_adjoint_forward(umans_ctx* c, const umans_adjoint_desc* d, const float
 		if (!adjoint_policy_ok(c->hpol[i]))
 			return c->fail(UMANS_EINVAL, "the adjoint needs gradient-method policies of force models (policy " +
 			                                 std::to_string(i) + ")");
-	hipSetDevice(c->device);
+	if (const int rc_dev = c->hip(hipSetDevice(c->device), "set device")) return rc_dev;
 	c->drop_graph();
 	c->free_adjoint();
 	const int n = c->n, T = d->nsteps, ck = d->checkpoint_every;
@@ -1714,7 +1725,7 @@ int umans_adjoint_forward(umans_ctx* c, const umans_adjoint_desc* d, const float
 int umans_adjoint_backward(umans_ctx* c, float* d_params, float* d_pos0, float* d_vel0) {
 	if (!c) return UMANS_EINVAL;
 	if (!c->adj_ready || c->adj_n != c->n) return c->fail(UMANS_ESTATE, "run umans_adjoint_forward first");
-	hipSetDevice(c->device);
+	if (const int rc_dev = c->hip(hipSetDevice(c->device), "set device")) return rc_dev;
 	const int n = c->n, T = c->adj_steps, ck = c->adj_ckpt;
 	const int nseg = ceil_div(T, ck);
 	const int kmax = kmax_bucket(c->kcap);
@@ -1976,10 +1987,10 @@ int umans_set_obstacles(umans_ctx* c, int n_polygons, const int32_t* counts, con
 		o += m;
 	}
 	if (edges.size() > (size_t)(1 << 24)) return c->fail(UMANS_EINVAL, "too many obstacle edges");
-	hipSetDevice(c->device);
+	if (const int rc_dev = c->hip(hipSetDevice(c->device), "set device")) return rc_dev;
 	int rc = c->hip(hipStreamSynchronize(c->stream), "obstacles");
 	if (rc) return rc;
-	hipFree(c->d_edges);
+	(void)hipFree(c->d_edges);
 	c->d_edges = nullptr;
 	c->n_edges = 0;
 	if (!edges.empty()) {
@@ -1996,7 +2007,7 @@ int umans_set_obstacles(umans_ctx* c, int n_polygons, const int32_t* counts, con
 int umans_download_records(umans_ctx* c, umans_agent_record* out, int capacity) {
 	static_assert(sizeof(umans_agent_record) == sizeof(AgentRecord), "record layout");
 	if (!c || (!out && capacity > 0)) return UMANS_EINVAL;
-	hipSetDevice(c->device);
+	if (const int rc_dev = c->hip(hipSetDevice(c->device), "set device")) return rc_dev;
 	int rc = sync_population(c);
 	if (rc) return rc;
 	const int n = c->n;
@@ -2008,7 +2019,7 @@ int umans_download_records(umans_ctx* c, umans_agent_record* out, int capacity) 
 	hipLaunchKernelGGL(k_pack_records, dim3(ceil_div(n, 256)), dim3(256), 0, c->stream, c->X.s, n, d);
 	rc = c->hip(hipMemcpyAsync(out, d, n * sizeof(AgentRecord), hipMemcpyDeviceToHost, c->stream), "records");
 	if (!rc) rc = c->hip(hipStreamSynchronize(c->stream), "records");
-	hipFree(d);
+	(void)hipFree(d);
 	return rc;
 }
 

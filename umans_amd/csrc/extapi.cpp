@@ -22,7 +22,7 @@ struct Simulation {
 	int record_cap = 0;
 
 	~Simulation() {
-		if (records) hipHostFree(records);
+		if (records) (void)hipHostFree(records);
 		umans_destroy(ctx);
 	}
 };
@@ -175,7 +175,7 @@ bool GetAgentPositions(AgentData*& result_agentData, int& result_nrAgents) {
 	Simulation& s = *g_sim;
 	const int n = umans_num_agents(s.ctx);   // the list length, pending changes included
 	if (n > s.record_cap) {
-		if (s.records) hipHostFree(s.records);
+		if (s.records) (void)hipHostFree(s.records);
 		s.records = nullptr;
 		s.record_cap = 0;
 		int cap = 1;
