@@ -159,11 +159,15 @@ typedef struct {
 	float cell_factor;
 	int32_t block_size;
 	float ms_per_step;         /* autotune: time of the chosen configuration */
-	int32_t list_capacity;     /* lists longer than 32 (no cap, or a cap above 32): capacity
-	                              of the first launch, 32 or 64, the agents whose list is
-	                              longer being redone at the next capacity; 128: a single
-	                              launch at full capacity; 0 in umans_set_tuning: unchanged.
-	                              Results do not depend on it. */
+	int32_t list_capacity;     /* lists longer than 32 (no cap, or a cap above 32): 1 (the
+	                              default): every agent's neighbours collected unsorted in
+	                              device memory and ranked by its wave, agents with more than
+	                              128 in range redone by a 128-entry sorted insertion; 32 or
+	                              64: capacity of a first sorted-insertion launch, the agents
+	                              whose list is longer being redone at the next capacity;
+	                              128: a single sorted-insertion launch at full capacity;
+	                              0 in umans_set_tuning: unchanged. Results do not depend on
+	                              it. */
 	int32_t class_block[5];    /* block size of each policy class's step kernel (force +
 	                              gradient, force + sampling, ORCA, generic, cooperative
 	                              sampling), 0 = block_size; the autotuner refines them one

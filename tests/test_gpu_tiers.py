@@ -1,7 +1,9 @@
-"""Long neighbour lists (no cap) are computed in tiers: a launch with 32- or 64-entry
-lists hands the agents whose list is longer to a launch with longer lists. The tier a
-list is finished in must not show: every list capacity gives the same lists and the
-same new state, bit for bit, as one launch at full capacity."""
+"""Long neighbour lists (no cap) have several launch plans: by default every agent's
+neighbours are collected unsorted and ranked by its wave (list_capacity 1), agents with
+more than 128 in range being redone by a 128-entry sorted insertion; or tiers, a launch
+with 32- or 64-entry lists handing the agents whose list is longer to a launch with
+longer lists. The plan must not show: every one gives the same lists and the same new
+state, bit for bit, as one sorted-insertion launch at full capacity."""
 import numpy as np
 import pytest
 
@@ -36,7 +38,7 @@ def dense_world(case, world_type, list_capacity, size=50.0, steps=3):
 def test_list_capacity_tiers_do_not_change_results(case, world_type):
     ref, (rc, rid, rd2, rseg), rst = dense_world(case, world_type, 128)
     assert rst["max_list"] > 64, "the scenario must reach the last tier"
-    for cap in (32, 64):
+    for cap in (1, 32, 64):
         out, (cnt, ids, d2, seg), st = dense_world(case, world_type, cap)
         np.testing.assert_array_equal(cnt, rc)
         np.testing.assert_array_equal(ids, rid)
@@ -45,3 +47,33 @@ def test_list_capacity_tiers_do_not_change_results(case, world_type):
             np.testing.assert_array_equal(out[k], ref[k], err_msg=f"{case} capacity {cap} {k}")
         assert st["overflow_agents"] == rst["overflow_agents"]
         assert st["max_list"] == rst["max_list"]
+
+
+@pytest.mark.parametrize("world_type", [0, 1])
+def test_ranked_lists_beyond_their_capacity(world_type):
+    """Agents with more than 128 neighbours in range leave the ranked lists for the
+    128-entry sorted insertion, which keeps the nearest 128: the same lists and state as
+    a single sorted-insertion launch, and the overflow counted once."""
+    def run(list_capacity):
+        method, costs, samp, relax = CASES["powerlaw_sampling"]
+        sc = S.random_cloud(2200, 25.0, [None], world_type=world_type, seed=32)
+        w = gpu.World(world_type, 25.0, 25.0, sc.dt, [pack_policy(method, costs, sampling=samp,
+                                                                   relaxation_time=relax)])
+        w.upload(sc.agents)
+        w.set_tuning(1.0, 0, list_capacity)
+        w.record_neighbors(True)
+        w.step(2)
+        out, nb, st = w.download(), w.neighbors(), w.stats()
+        w.close()
+        return out, nb, st
+    ref, (rc, rid, rd2, rseg), rst = run(128)
+    assert rst["overflow_agents"] > 0, "some lists must exceed 128"
+    out, (cnt, ids, d2, seg), st = run(1)
+    np.testing.assert_array_equal(cnt, rc)
+    np.testing.assert_array_equal(ids, rid)
+    np.testing.assert_array_equal(d2, rd2)
+    np.testing.assert_array_equal(seg, rseg)
+    for k in ("pos", "vel", "viewdir", "rng"):
+        np.testing.assert_array_equal(out[k], ref[k], err_msg=k)
+    assert st["overflow_agents"] == rst["overflow_agents"]
+    assert st["max_list"] == rst["max_list"]

@@ -101,6 +101,7 @@ void launch_step_c(int kmax, int bs, const StepArgs& a, hipStream_t st) {
 		else launch_step_t<32, 128, CLS>(a, st);
 		break;
 	case 64: launch_step_t<64, 64, CLS>(a, st); break;
+	case 0: launch_step_t<0, 64, CLS>(a, st); break;     // long lists, ranked per wave
 	default: launch_step_t<128, 64, CLS>(a, st); break;
 	}
 }
@@ -167,11 +168,15 @@ struct umans_ctx {
 	unsigned* scan_sums = nullptr;
 	unsigned* cell_of = nullptr;
 	int* class_list = nullptr;                // sorted index of each class-list entry
+	double* long_d2 = nullptr;                // long lists: kLongCap x cap collection buffers
+	unsigned* long_u = nullptr;
 	// list-capacity tiers (uncapped and large caps): agents whose list outgrew a tier,
 	// per tier, in their class's range; and the counts per class and tier
 	int* ovf[2] = {nullptr, nullptr};
 	unsigned* ovf_count = nullptr;            // [NUM_CLS][2]
-	int tier_first = 2;                       // first tier: 0 (32 entries), 1 (64), 2 (single launch; autotuned)
+	int tier_first = 3;                       // 0: tiers of 32, 64, 128 entries; 1: of 64, 128; 2: a single
+	                                          // 128-entry launch; 3: long lists collected and ranked per
+	                                          // wave (autotuned; lists longer than 32 only)
 	int cell_alloc = 0;          // allocated cell counters (>= total_cells + 1)
 	int counts_zero_n = -1;      // the counters are zero over this key space (-1: unknown)
 	// the previous step of this run_steps call counted X for this step (cell_count,
@@ -330,8 +335,11 @@ struct umans_ctx {
 		free_adjoint();
 		X.release(); Y.release(); B.release();
 		(void)hipFree(cell_of); (void)hipFree(ovf[0]); (void)hipFree(ovf[1]); (void)hipFree(class_list);
+		(void)hipFree(long_d2); (void)hipFree(long_u);
 		ovf[0] = ovf[1] = nullptr;
 		class_list = nullptr;
+		long_d2 = nullptr;
+		long_u = nullptr;
 		(void)hipFree(rec_count); (void)hipFree(rec_id); (void)hipFree(rec_d2); (void)hipFree(rec_seg);
 		cell_of = nullptr;
 		rec_count = rec_id = rec_seg = nullptr;
@@ -766,8 +774,27 @@ int enqueue_step(umans_ctx* c, bool events, const State* ysave = nullptr, unsign
 			sq = c->cls_stream[q];
 			if (int rc = c->hip(hipStreamWaitEvent(sq, c->fork_ev, 0), "fork")) return rc;
 		}
-		if (q == CLS_COOP || kmax <= 32 || c->tier_first >= 2) {
+		if (q == CLS_COOP || kmax <= 32 || c->tier_first == 2) {
 			launch_step(q, kmax, class_block(q, kmax, bs, c->cls_block), aq, sq);
+		} else if (c->tier_first == 3) {
+			// Long lists: every lane collects its neighbours unsorted into its row of the
+			// list buffers and the wave ranks them; the agents with more than kLongCap in
+			// range are redone by a 128-entry sorted insertion, which keeps the nearest.
+			// Results are those of a single launch at full capacity.
+			StepArgs al = aq;
+			al.long_d2 = c->long_d2;
+			al.long_u = c->long_u;
+			al.long_stride = c->cap;
+			al.tier_final = 0;
+			al.ovf_list = c->ovf[0];
+			al.ovf_count = c->ovf_count + 2 * q;
+			al.ovf_base = c->cls_off[q];
+			launch_step(q, 0, 64, al, sq);
+			StepArgs af = aq;
+			af.agent_list = c->ovf[0];
+			af.agent_count = c->ovf_count + 2 * q;
+			af.tier_final = 1;
+			launch_step(q, 128, 64, af, sq);
 		} else {
 			// Long lists (no cap, or a cap above 32): a first launch with 32-entry lists
 			// (4x the occupancy of 128) hands the agents whose list is longer to a 64-entry
@@ -888,6 +915,10 @@ int grow_agents(umans_ctx* c, int n, int keep) {
 	if ((rc = c->hip(c->Y.alloc(cap), "alloc state"))) return rc;
 	if ((rc = c->hip(dalloc(&c->cell_of, cap), "alloc grid"))) return rc;
 	if ((rc = c->hip(dalloc(&c->class_list, cap), "alloc grid"))) return rc;
+	if (kmax_bucket(c->kcap) > 32) {
+		if ((rc = c->hip(dalloc(&c->long_d2, (size_t)kLongCap * cap), "alloc lists"))) return rc;
+		if ((rc = c->hip(dalloc(&c->long_u, (size_t)kLongCap * cap), "alloc lists"))) return rc;
+	}
 	if ((rc = c->hip(dalloc(&c->ovf[0], cap), "alloc grid"))) return rc;
 	if ((rc = c->hip(dalloc(&c->ovf[1], cap), "alloc grid"))) return rc;
 	c->cap = cap;
@@ -1303,6 +1334,10 @@ int umans_upload(umans_ctx* c, const umans_agents* a, int n) {
 		if ((rc = c->hip(c->Y.alloc(n), "alloc state"))) return rc;
 		if ((rc = c->hip(dalloc(&c->cell_of, n), "alloc grid"))) return rc;
 		if ((rc = c->hip(dalloc(&c->class_list, n), "alloc grid"))) return rc;
+		if (kmax_bucket(c->kcap) > 32) {
+			if ((rc = c->hip(dalloc(&c->long_d2, (size_t)kLongCap * n), "alloc lists"))) return rc;
+			if ((rc = c->hip(dalloc(&c->long_u, (size_t)kLongCap * n), "alloc lists"))) return rc;
+		}
 		if ((rc = c->hip(dalloc(&c->ovf[0], n), "alloc grid"))) return rc;
 		if ((rc = c->hip(dalloc(&c->ovf[1], n), "alloc grid"))) return rc;
 		c->cap = n;
@@ -1395,7 +1430,7 @@ int umans_autotune(umans_ctx* c, int trial_steps, umans_tuning* out) {
 	const float f0 = c->cell_factor;
 	const int b0 = c->block;
 	const int tier0 = c->tier_first;
-	const int n_tiers = kmax > 32 ? 3 : 1;   // tier choices only matter for long lists
+	const int n_tiers = kmax > 32 ? 4 : 1;   // list choices only matter for long lists
 	int best_t = tier0;
 	const int rec0 = c->record;
 	c->record = 0;
@@ -1474,7 +1509,7 @@ int umans_autotune(umans_ctx* c, int trial_steps, umans_tuning* out) {
 		out->cell_factor = c->cell_factor;
 		out->block_size = c->block;
 		out->ms_per_step = best_ms / trial_steps;
-		out->list_capacity = c->tier_first == 0 ? 32 : (c->tier_first == 1 ? 64 : 128);
+		out->list_capacity = c->tier_first == 0 ? 32 : (c->tier_first == 1 ? 64 : (c->tier_first == 2 ? 128 : 1));
 		for (int q = 0; q < NUM_CLS; ++q) out->class_block[q] = c->cls_block[q];
 	}
 	return UMANS_OK;
@@ -1484,8 +1519,9 @@ int umans_set_tuning(umans_ctx* c, const umans_tuning* t) {
 	if (!c || !t) return UMANS_EINVAL;
 	if (!(t->cell_factor > 0.0f) || (t->block_size != 0 && !block_supported(kmax_bucket(c->kcap), t->block_size)))
 		return c->fail(UMANS_EINVAL, "unsupported tuning");
-	if (t->list_capacity != 0 && t->list_capacity != 32 && t->list_capacity != 64 && t->list_capacity != 128)
-		return c->fail(UMANS_EINVAL, "list_capacity must be 0, 32, 64 or 128");
+	if (t->list_capacity != 0 && t->list_capacity != 1 && t->list_capacity != 32 && t->list_capacity != 64 &&
+	    t->list_capacity != 128)
+		return c->fail(UMANS_EINVAL, "list_capacity must be 0, 1, 32, 64 or 128");
 	for (int q = 0; q < NUM_CLS; ++q)
 		if (t->class_block[q] != 0 && !block_supported(kmax_bucket(c->kcap), t->class_block[q]))
 			return c->fail(UMANS_EINVAL, "unsupported class block size");
@@ -1495,7 +1531,8 @@ int umans_set_tuning(umans_ctx* c, const umans_tuning* t) {
 	c->cell_factor = t->cell_factor;
 	c->block = t->block_size;
 	for (int q = 0; q < NUM_CLS; ++q) c->cls_block[q] = t->class_block[q];
-	if (t->list_capacity) c->tier_first = t->list_capacity == 32 ? 0 : (t->list_capacity == 64 ? 1 : 2);
+	if (t->list_capacity)
+		c->tier_first = t->list_capacity == 1 ? 3 : (t->list_capacity == 32 ? 0 : (t->list_capacity == 64 ? 1 : 2));
 	c->drop_graph();
 	if (c->n > 0) {
 		std::vector<int> per_world;

@@ -380,6 +380,106 @@ UD int find_neighbors(const StepArgs& A, const WorldInfo& wi, const GridInfo& g,
 	return filled;
 }
 
+// Long lists, step 1 (one lane per agent): the same queries and exact test as
+// find_neighbors, but every candidate in range is appended, unsorted, to the lane's row of
+// the collection buffers (double d2, sorted index | segment), at most kLongCap of them.
+// Returns the number in range (all of them, as find_neighbors' total).
+UD int collect_neighbors(const StepArgs& A, const WorldInfo& wi, const GridInfo& g, int self, V2 p, float range,
+                         int row) {
+	const State& S = A.in;
+	const double r2 = (double)(range * range);
+	const float rs = range * 1.0001f + 1e-3f;
+	const size_t st = (size_t)A.long_stride;
+	int n = 0;
+	for (int s = 0; s < 5; ++s) {
+		float dx = 0.0f, dy = 0.0f;
+		if (s > 0) {
+			if (wi.world_type != 1) break;
+			if (s == 1) { if (!((double)(p.x - range) < -0.5 * (double)wi.W)) continue; dx = wi.W; }
+			else if (s == 2) { if (!((double)(p.x + range) > 0.5 * (double)wi.W)) continue; dx = -wi.W; }
+			else if (s == 3) { if (!((double)(p.y - range) < -0.5 * (double)wi.H)) continue; dy = wi.H; }
+			else { if (!((double)(p.y + range) > 0.5 * (double)wi.H)) continue; dy = -wi.H; }
+		}
+		const V2 q{p.x + dx, p.y + dy};
+		const float m = rs + (fabsf(q.x) + fabsf(q.y)) * 1e-6f;
+		const int cx0 = bin_axis(q.x - m, g.ox, g.invx, g.ncx), cx1 = bin_axis(q.x + m, g.ox, g.invx, g.ncx);
+		const int cy0 = bin_axis(q.y - m, g.oy, g.invy, g.ncy), cy1 = bin_axis(q.y + m, g.oy, g.invy, g.ncy);
+		const double qx = (double)q.x, qy = (double)q.y;
+		for (int cy = cy0; cy <= cy1; ++cy) {
+			const int crow = wi.cell_offset + cy * g.ncx;
+			const unsigned b = A.cell_start[crow + cx0], e = A.cell_start[crow + cx1 + 1];
+			for (unsigned j = b; j < e; ++j) {
+				const float2 pj = S.pos[j];
+				const double ddx = qx - (double)pj.x;
+				const double ddy = qy - (double)pj.y;
+				const double d2 = ddx * ddx + ddy * ddy;
+				if (!(d2 < r2) || (int)j == self) continue;
+				if (n < kLongCap) {
+					A.long_d2[(size_t)n * st + row] = d2;
+					A.long_u[(size_t)n * st + row] = j | ((unsigned)s << kSegShift);
+				}
+				++n;
+			}
+		}
+	}
+	return n;
+}
+
+// Long lists, step 2 (the whole wave, in uniform control flow): each of the wave's rows
+// in turn is ranked by its lanes, each holding entries lane and lane + 64, against every
+// entry of the row broadcast by shuffles, in the canonical order (segment, d2, agent ID;
+// the IDs are fetched only for a row that holds equal distances), and its first kcap
+// entries are written back into long_u in that order. n_row: the row's entry count
+// (0 for rows the wave does not sort).
+UD void rank_long_lists(const StepArgs& A, int row, int n_row, int kcap) {
+	const int lane = threadIdx.x & 63;
+	const size_t st = (size_t)A.long_stride;
+	// the rows were written by their own lanes: make those stores visible to the wave
+	// (one wave per workgroup, so workgroup scope is the wave)
+	__builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup");
+	for (int r = 0; r < 64; ++r) {
+		const int n = __shfl(n_row, r);
+		if (n == 0) continue;
+		const int rw = __shfl(row, r);
+		double d0 = 0.0, d1 = 0.0;
+		unsigned u0 = 0u, u1 = 0u;
+		const bool h0 = lane < n, h1 = lane + 64 < n;
+		if (h0) { d0 = A.long_d2[(size_t)lane * st + rw]; u0 = A.long_u[(size_t)lane * st + rw]; }
+		if (h1) { d1 = A.long_d2[(size_t)(lane + 64) * st + rw]; u1 = A.long_u[(size_t)(lane + 64) * st + rw]; }
+		// key order: segment, then d2; equal keys are counted and settled by the IDs
+		int rk0 = 0, rk1 = 0, eq = 0;
+		for (int j = 0; j < n; ++j) {
+			const int src = j & 63;
+			const bool hi = j >= 64;
+			const double dj = __shfl(hi ? d1 : d0, src);
+			const unsigned sj = __shfl(hi ? u1 : u0, src) >> kSegShift;
+			const unsigned s0 = u0 >> kSegShift, s1 = u1 >> kSegShift;
+			rk0 += (sj < s0 || (sj == s0 && dj < d0)) ? 1 : 0;
+			rk1 += (sj < s1 || (sj == s1 && dj < d1)) ? 1 : 0;
+			eq += (h0 && j != lane && sj == s0 && dj == d0) ? 1 : 0;
+			eq += (h1 && j != lane + 64 && sj == s1 && dj == d1) ? 1 : 0;
+		}
+		if (__ballot(eq > 0)) {
+			const State& S = A.in;
+			const int id0 = h0 ? S.meta[u0 & kIndexMask].x : 0;
+			const int id1 = h1 ? S.meta[u1 & kIndexMask].x : 0;
+			for (int j = 0; j < n; ++j) {
+				const int src = j & 63;
+				const bool hi = j >= 64;
+				const double dj = __shfl(hi ? d1 : d0, src);
+				const unsigned sj = __shfl(hi ? u1 : u0, src) >> kSegShift;
+				const int ij = __shfl(hi ? id1 : id0, src);
+				rk0 += (h0 && j != lane && sj == (u0 >> kSegShift) && dj == d0 && ij < id0) ? 1 : 0;
+				rk1 += (h1 && j != lane + 64 && sj == (u1 >> kSegShift) && dj == d1 && ij < id1) ? 1 : 0;
+			}
+		}
+		if (h0 && rk0 < kcap) A.long_u[(size_t)rk0 * st + rw] = u0;
+		if (h1 && rk1 < kcap) A.long_u[(size_t)rk1 * st + rw] = u1;
+	}
+	// the sorted rows are read back by their own lanes
+	__builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup");
+}
+
 // WorldBase::computeNeighboringObstacles_Flat (worldBase.cpp:62-77) for each query of
 // WorldInfinite / WorldToric::ComputeNeighbors (worldToric.cpp:32-79): the edges whose
 // distance to float(position + displacement) is within the range, in edge order, moved
@@ -573,8 +673,12 @@ UD int xcd_tile(int b, int nb) {
 // One lane per agent of the class's list entries [A.base, A.base + A.n).
 template <int KMAX, int BS, int CLS>
 __global__ __launch_bounds__(BS) void k_step(StepArgs A) {
-	__shared__ float s_d2[KMAX * BS];
-	__shared__ unsigned s_nb[KMAX * BS];
+	// KMAX = 0: long lists, collected in global buffers and ranked by the wave (BS = 64)
+	constexpr bool kLong = KMAX == 0;
+	constexpr int KL = kLong ? kLongCap : KMAX;    // the longest list a lane can hold
+	static_assert(!kLong || BS == 64, "long lists are ranked per wave: one wave per block");
+	__shared__ float s_d2[kLong ? 1 : KMAX * BS];
+	__shared__ unsigned s_nb[kLong ? 1 : KMAX * BS];
 	// The ORCA half-planes live in private memory: kept in LDS they bound the class to
 	// ~2 waves per SIMD, and the private copy (L1-resident) measured faster at 6.
 	const int tid = threadIdx.x;
@@ -601,7 +705,19 @@ __global__ __launch_bounds__(BS) void k_step(StepArgs A) {
 		a.radius = prm.x; a.pref_speed = prm.y; a.max_speed = prm.z; a.max_accel = prm.w;
 		a.dt = wi.dt;
 		a.relaxation = A.pol[pi].relaxation;
-		cnt = find_neighbors<KMAX, BS>(A, wi, g, k, a.pos, A.pol[pi].query_range, s_d2 + tid, s_nb + tid, total);
+		if constexpr (kLong) {
+			total = collect_neighbors(A, wi, g, k, a.pos, A.pol[pi].query_range, A.base + t);
+		} else {
+			cnt = find_neighbors<KMAX, BS>(A, wi, g, k, a.pos, A.pol[pi].query_range, s_d2 + tid, s_nb + tid, total);
+		}
+	}
+	// long lists: a lane with more than kLongCap in range is handed on; the others are
+	// ranked by the wave (every lane takes part, active or not)
+	const int kcap_eff = A.kcap < KL ? A.kcap : KL;
+	if constexpr (kLong) {
+		const bool keep = active && total <= kLongCap;
+		rank_long_lists(A, A.base + t, keep ? total : 0, kcap_eff);
+		cnt = keep ? (total < kcap_eff ? total : kcap_eff) : 0;
 	}
 	// obstacle edges: only the generic class runs when a world has obstacles
 	float4 obs_priv[CLS == CLS_GENERIC ? kMaxObstacleEdges : 1];
@@ -613,9 +729,13 @@ __global__ __launch_bounds__(BS) void k_step(StepArgs A) {
 			if (obs_total > n_obs) atomicAdd(&A.stats[2], 1ull);
 		}
 	}
+	// a list that outgrew this launch's capacity is redone by the next launch, which
+	// counts it in the stats
+	const bool handover = kLong ? total > kLongCap : (!A.tier_final && total > KMAX);
 	// stats: one atomic per wave
 	{
-		const unsigned long long ov = __ballot(active && A.tier_final && total > A.kcap);
+		const bool final_here = kLong ? !handover : A.tier_final != 0;
+		const unsigned long long ov = __ballot(active && final_here && total > A.kcap);
 		const int mx = wave_max_i(total);
 		if ((tid & 63) == 0) {
 			const int c = __popcll(ov);
@@ -624,17 +744,19 @@ __global__ __launch_bounds__(BS) void k_step(StepArgs A) {
 		}
 	}
 	if (!active) return;
-	// a list that outgrew this launch's capacity is redone by the next tier
-	if (!A.tier_final && total > KMAX) {
+	if (handover) {
 		A.ovf_list[A.ovf_base + atomicAdd(A.ovf_count, 1u)] = k;
 		return;
 	}
 	const DevPolicy& P = A.pol[pi];
+	// the lane's sorted list: its LDS column, or its row of the long-list buffer
+	const unsigned* lst = kLong ? A.long_u + (A.base + t) : s_nb + tid;
+	const int lstride = kLong ? A.long_stride : BS;
 	if (A.rec_count) {
 		A.rec_count[k] = cnt;
 		const int m = cnt < A.rec_stride ? cnt : A.rec_stride;
 		for (int e = 0; e < m; ++e) {
-			const unsigned u = s_nb[e * BS + tid];
+			const unsigned u = lst[e * lstride];
 			const long long o = (long long)k * A.rec_stride + e;
 			A.rec_id[o] = S.meta[u & kIndexMask].x;
 			A.rec_d2[o] = entry_d2(S, a.pos, wi, u);
@@ -643,9 +765,9 @@ __global__ __launch_bounds__(BS) void k_step(StepArgs A) {
 	}
 	if (A.raw_nb) {
 		A.raw_count[k] = cnt;
-		for (int e = 0; e < cnt; ++e) A.raw_nb[(size_t)k * A.raw_stride + e] = s_nb[e * BS + tid];
+		for (int e = 0; e < cnt; ++e) A.raw_nb[(size_t)k * A.raw_stride + e] = lst[e * lstride];
 	}
-	const NbList L{s_nb + tid, BS, cnt, S.pos, S.vel, S.prm, S.pref, wi.W, wi.H, a.pos, obs_priv, n_obs};
+	const NbList L{lst, lstride, cnt, S.pos, S.vel, S.prm, S.pref, wi.W, wi.H, a.pos, obs_priv, n_obs};
 	uint32_t rng = (uint32_t)meta.w;
 
 	// Force classes: one pass over the neighbours gathers each neighbour once for the
@@ -688,17 +810,17 @@ __global__ __launch_bounds__(BS) void k_step(StepArgs A) {
 		Prep<KMAX, CLS> pr;
 		pr.n_lines = 0;
 		if constexpr (CLS == CLS_ORCA) {
-			Line proj_priv[KMAX];
+			Line proj_priv[KL];
 			const Lines proj{proj_priv, 1};
 			V2 best;
-			if constexpr (KMAX <= 16) {
+			if constexpr (!kLong && KMAX <= 16) {
 				// half-planes in registers; only the rare projected lines of
 				// linearProgram3 go to private memory
 				RegLines<KMAX> R;
 				orca_lines_reg<KMAX>(P.params[0], P.range[0], a, L, R);
 				best = orca_solve_reg<KMAX>(R, a, proj);
 			} else {
-				Line lines_priv[KMAX];
+				Line lines_priv[KL];
 				pr.lines = Lines{lines_priv, 1};
 				pr.n_lines = orca_lines(P.params[0], P.range[0], a, L, pr.lines);
 				best = orca_solve(pr.lines, pr.n_lines, a, proj);
@@ -706,8 +828,8 @@ __global__ __launch_bounds__(BS) void k_step(StepArgs A) {
 			const V2 acc = clamp_vec((best - a.vel) / smax(a.relaxation, a.dt), a.max_accel);
 			next = clamp_vec(a.vel + acc * a.dt, a.max_speed);
 		} else {
-			Line lines_priv[CLS == CLS_GENERIC ? KMAX : 1];
-			Line proj_priv[CLS == CLS_GENERIC ? KMAX : 1];
+			Line lines_priv[CLS == CLS_GENERIC ? KL : 1];
+			Line proj_priv[CLS == CLS_GENERIC ? KL : 1];
 			pr.lines = Lines{lines_priv, 1};
 			for_costs<CLS>(P, [&](int c) {
 				const int kind = P.kind[c];

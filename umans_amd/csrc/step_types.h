@@ -122,7 +122,18 @@ struct StepArgs {
 	unsigned* next_count;
 	unsigned* next_cell_of;
 	int next_key_base;
+	// long lists (the step kernel instance with KMAX = 0): lane t collects its in-range
+	// candidates unsorted into row base + t of these buffers (entry e at
+	// [e * long_stride + row]), at most kLongCap of them; the wave then ranks each row
+	// and writes the sorted entries back into long_u
+	double* long_d2;
+	unsigned* long_u;
+	int long_stride;
 };
+
+// Capacity of a collected long list; an agent with more candidates in range is handed
+// to a launch with the 128-entry sorted insertion, which keeps the nearest.
+constexpr int kLongCap = 128;
 
 // Obstacle edges kept per agent (the generic class only); more are counted in stats[2].
 constexpr int kMaxObstacleEdges = 32;
