@@ -427,10 +427,35 @@ UD int collect_neighbors(const StepArgs& A, const WorldInfo& wi, const GridInfo&
 
 // Long lists, step 2 (the whole wave, in uniform control flow): each of the wave's rows
 // in turn is ranked by its lanes, each holding entries lane and lane + 64, against every
-// entry of the row broadcast by shuffles, in the canonical order (segment, d2, agent ID;
+// entry of the row read out of its lane into scalar registers, in the canonical order
+// (segment, d2, agent ID;
 // the IDs are fetched only for a row that holds equal distances), and its first kcap
 // entries are written back into long_u in that order. n_row: the row's entry count
 // (0 for rows the wave does not sort).
+UD double readlane_d(double v, int l) {
+	const unsigned long long b = (unsigned long long)__double_as_longlong(v);
+	const unsigned lo = __builtin_amdgcn_readlane((unsigned)b, l);
+	const unsigned hi = __builtin_amdgcn_readlane((unsigned)(b >> 32), l);
+	return __longlong_as_double((long long)(((unsigned long long)hi << 32) | lo));
+}
+// Rank of the entries (d_i, u_i) held by the lanes among the first n of them (entry j in
+// lane j & 63, slot j >> 6 of P parts), by segment then d2; eq counts equal keys.
+template <int P>
+UD void rank_parts(int n, const double (&d)[2], const unsigned (&u)[2], int (&rk)[2], int& eq) {
+	const int lane = threadIdx.x & 63;
+	for (int j = 0; j < n; ++j) {
+		const int src = j & 63;
+		const int part = P == 1 ? 0 : (j >> 6);
+		const double dj = readlane_d(part ? d[1] : d[0], src);
+		const unsigned sj = __builtin_amdgcn_readlane(part ? u[1] : u[0], src) >> kSegShift;
+#pragma unroll
+		for (int q = 0; q < P; ++q) {
+			const unsigned sq = u[q] >> kSegShift;
+			rk[q] += (sj < sq || (sj == sq && dj < d[q])) ? 1 : 0;
+			eq += (j != lane + 64 * q && sj == sq && dj == d[q]) ? 1 : 0;
+		}
+	}
+}
 UD void rank_long_lists(const StepArgs& A, int row, int n_row, int kcap) {
 	const int lane = threadIdx.x & 63;
 	const size_t st = (size_t)A.long_stride;
@@ -438,43 +463,35 @@ UD void rank_long_lists(const StepArgs& A, int row, int n_row, int kcap) {
 	// (one wave per workgroup, so workgroup scope is the wave)
 	__builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup");
 	for (int r = 0; r < 64; ++r) {
-		const int n = __shfl(n_row, r);
+		const int n = __builtin_amdgcn_readlane(n_row, r);
 		if (n == 0) continue;
-		const int rw = __shfl(row, r);
-		double d0 = 0.0, d1 = 0.0;
-		unsigned u0 = 0u, u1 = 0u;
+		const int rw = __builtin_amdgcn_readlane(row, r);
+		double d[2] = {0.0, 0.0};
+		unsigned u[2] = {0u, 0u};
 		const bool h0 = lane < n, h1 = lane + 64 < n;
-		if (h0) { d0 = A.long_d2[(size_t)lane * st + rw]; u0 = A.long_u[(size_t)lane * st + rw]; }
-		if (h1) { d1 = A.long_d2[(size_t)(lane + 64) * st + rw]; u1 = A.long_u[(size_t)(lane + 64) * st + rw]; }
+		if (h0) { d[0] = A.long_d2[(size_t)lane * st + rw]; u[0] = A.long_u[(size_t)lane * st + rw]; }
+		if (h1) { d[1] = A.long_d2[(size_t)(lane + 64) * st + rw]; u[1] = A.long_u[(size_t)(lane + 64) * st + rw]; }
 		// key order: segment, then d2; equal keys are counted and settled by the IDs
-		int rk0 = 0, rk1 = 0, eq = 0;
-		for (int j = 0; j < n; ++j) {
-			const int src = j & 63;
-			const bool hi = j >= 64;
-			const double dj = __shfl(hi ? d1 : d0, src);
-			const unsigned sj = __shfl(hi ? u1 : u0, src) >> kSegShift;
-			const unsigned s0 = u0 >> kSegShift, s1 = u1 >> kSegShift;
-			rk0 += (sj < s0 || (sj == s0 && dj < d0)) ? 1 : 0;
-			rk1 += (sj < s1 || (sj == s1 && dj < d1)) ? 1 : 0;
-			eq += (h0 && j != lane && sj == s0 && dj == d0) ? 1 : 0;
-			eq += (h1 && j != lane + 64 && sj == s1 && dj == d1) ? 1 : 0;
-		}
+		int rk[2] = {0, 0}, eq = 0;
+		if (n <= 64) rank_parts<1>(n, d, u, rk, eq);
+		else rank_parts<2>(n, d, u, rk, eq);
+		if (!h0) eq = 0;
 		if (__ballot(eq > 0)) {
 			const State& S = A.in;
-			const int id0 = h0 ? S.meta[u0 & kIndexMask].x : 0;
-			const int id1 = h1 ? S.meta[u1 & kIndexMask].x : 0;
+			const int id0 = h0 ? S.meta[u[0] & kIndexMask].x : 0;
+			const int id1 = h1 ? S.meta[u[1] & kIndexMask].x : 0;
 			for (int j = 0; j < n; ++j) {
 				const int src = j & 63;
 				const bool hi = j >= 64;
-				const double dj = __shfl(hi ? d1 : d0, src);
-				const unsigned sj = __shfl(hi ? u1 : u0, src) >> kSegShift;
-				const int ij = __shfl(hi ? id1 : id0, src);
-				rk0 += (h0 && j != lane && sj == (u0 >> kSegShift) && dj == d0 && ij < id0) ? 1 : 0;
-				rk1 += (h1 && j != lane + 64 && sj == (u1 >> kSegShift) && dj == d1 && ij < id1) ? 1 : 0;
+				const double dj = readlane_d(hi ? d[1] : d[0], src);
+				const unsigned sj = __builtin_amdgcn_readlane(hi ? u[1] : u[0], src) >> kSegShift;
+				const int ij = __builtin_amdgcn_readlane(hi ? id1 : id0, src);
+				rk[0] += (h0 && j != lane && sj == (u[0] >> kSegShift) && dj == d[0] && ij < id0) ? 1 : 0;
+				rk[1] += (h1 && j != lane + 64 && sj == (u[1] >> kSegShift) && dj == d[1] && ij < id1) ? 1 : 0;
 			}
 		}
-		if (h0 && rk0 < kcap) A.long_u[(size_t)rk0 * st + rw] = u0;
-		if (h1 && rk1 < kcap) A.long_u[(size_t)rk1 * st + rw] = u1;
+		if (h0 && rk[0] < kcap) A.long_u[(size_t)rk[0] * st + rw] = u[0];
+		if (h1 && rk[1] < kcap) A.long_u[(size_t)rk[1] * st + rw] = u[1];
 	}
 	// the sorted rows are read back by their own lanes
 	__builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup");
