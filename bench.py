@@ -284,7 +284,7 @@ def main():
         sw, _ = workload("sweep")
         wb = gpu.World(0, 0, 0, 0.1, sw.policies, max_neighbors=sw.max_neighbors, batch=sw.worlds)
         wb.upload(sw.agents)
-        tb = wb.autotune(trial_steps=3)
+        tb = wb.autotune(trial_steps=6)
         wb.set_graph(True)
         wb.step(3)
         wb.synchronize()
