@@ -77,3 +77,37 @@ def test_ranked_lists_beyond_their_capacity(world_type):
         np.testing.assert_array_equal(out[k], ref[k], err_msg=k)
     assert st["overflow_agents"] == rst["overflow_agents"]
     assert st["max_list"] == rst["max_list"]
+
+
+@pytest.mark.parametrize("world_type", [0, 1])
+def test_ranked_lists_on_a_lattice(world_type):
+    """Agents on a square lattice: every list holds many equal distances, ordered by agent
+    ID. The ranked lists (sorted without the IDs, then again with them where distances
+    tie) equal the sorted insertion's, bit for bit."""
+    def run(list_capacity):
+        method, costs, samp, relax = CASES["powerlaw_sampling"]
+        side = 36
+        g = (np.arange(side, dtype=np.float32) - side / 2 + 0.5) * np.float32(1.0)
+        xx, yy = np.meshgrid(g, g)
+        sc = S.random_cloud(side * side, float(side), [None], world_type=world_type, seed=33)
+        a = dict(sc.agents)
+        a["pos"] = np.stack([xx.ravel(), yy.ravel()], 1).astype(np.float32)
+        rng = np.random.default_rng(5)
+        a["id"] = rng.permutation(side * side).astype(np.int32)   # IDs unrelated to positions
+        w = gpu.World(world_type, float(side), float(side), sc.dt,
+                      [pack_policy(method, costs, sampling=samp, relaxation_time=relax)])
+        w.upload(a)
+        w.set_tuning(1.0, 0, list_capacity)
+        w.record_neighbors(True)
+        w.step(1)
+        out, nb, st = w.download(), w.neighbors(), w.stats()
+        w.close()
+        return out, nb, st
+    ref, (rc, rid, rd2, rseg), rst = run(128)
+    out, (cnt, ids, d2, seg), st = run(1)
+    np.testing.assert_array_equal(cnt, rc)
+    np.testing.assert_array_equal(ids, rid)
+    np.testing.assert_array_equal(d2, rd2)
+    np.testing.assert_array_equal(seg, rseg)
+    for k in ("pos", "vel", "viewdir", "rng"):
+        np.testing.assert_array_equal(out[k], ref[k], err_msg=k)

@@ -425,73 +425,139 @@ UD int collect_neighbors(const StepArgs& A, const WorldInfo& wi, const GridInfo&
 	return n;
 }
 
-// Long lists, step 2 (the whole wave, in uniform control flow): each of the wave's rows
-// in turn is ranked by its lanes, each holding entries lane and lane + 64, against every
-// entry of the row read out of its lane into scalar registers, in the canonical order
-// (segment, d2, agent ID;
-// the IDs are fetched only for a row that holds equal distances), and its first kcap
-// entries are written back into long_u in that order. n_row: the row's entry count
-// (0 for rows the wave does not sort).
-UD double readlane_d(double v, int l) {
-	const unsigned long long b = (unsigned long long)__double_as_longlong(v);
-	const unsigned lo = __builtin_amdgcn_readlane((unsigned)b, l);
-	const unsigned hi = __builtin_amdgcn_readlane((unsigned)(b >> 32), l);
-	return __longlong_as_double((long long)(((unsigned long long)hi << 32) | lo));
+// Long lists, step 2 (the whole wave, in uniform control flow): the wave sorts its rows
+// with bitonic networks across lanes, in the canonical order (segment, d2, agent ID),
+// and writes each row's first kcap entries back into long_u in that order. Rows of up to
+// 16, 32 and 64 entries are sorted four, two and one at a time (one entry per lane);
+// rows of up to 128 one at a time with two entries per lane. The agent IDs are fetched
+// only for a group of rows in which two entries have the same segment and distance; it
+// is then sorted again with them. n_row: the row's entry count (0 for rows the wave does
+// not sort).
+UD double shfl_xor_d(double v, int m) {
+	const long long b = __double_as_longlong(v);
+	const int lo = __shfl_xor((int)b, m), hi = __shfl_xor((int)(b >> 32), m);
+	return __longlong_as_double(((long long)hi << 32) | (unsigned)lo);
 }
-// Rank of the entries (d_i, u_i) held by the lanes among the first n of them (entry j in
-// lane j & 63, slot j >> 6 of P parts), by segment then d2; eq counts equal keys.
+UD bool key_less(double da, unsigned ua, int ia, double db, unsigned ub, int ib, bool with_id) {
+	const unsigned sa = ua >> kSegShift, sb = ub >> kSegShift;
+	if (sa != sb) return sa < sb;
+	if (da != db) return da < db;
+	return with_id && ia < ib;
+}
+// One bitonic sort over blocks of S lanes (P = 1, S <= 64) or over 128 entries, two per
+// lane (P = 2: entry lane and lane + 64). Returns whether two different entries compared
+// equal (only possible without the IDs); padding entries (u = ~0) compare after all.
 template <int P>
-UD void rank_parts(int n, const double (&d)[2], const unsigned (&u)[2], int (&rk)[2], int& eq) {
+UD bool bitonic(double (&d)[2], unsigned (&u)[2], int (&id)[2], int S, bool with_id) {
 	const int lane = threadIdx.x & 63;
-	for (int j = 0; j < n; ++j) {
-		const int src = j & 63;
-		const int part = P == 1 ? 0 : (j >> 6);
-		const double dj = readlane_d(part ? d[1] : d[0], src);
-		const unsigned sj = __builtin_amdgcn_readlane(part ? u[1] : u[0], src) >> kSegShift;
+	const int N = P == 1 ? S : 128;
+	bool tie = false;
+	for (int k = 2; k <= N; k <<= 1) {
+		for (int j = k >> 1; j > 0; j >>= 1) {
+			if (P == 2 && j == 64) {
+				// entries lane and lane + 64 of one lane; k == 128: ascending
+				const bool lt = key_less(d[1], u[1], id[1], d[0], u[0], id[0], with_id);
+				const bool gt = key_less(d[0], u[0], id[0], d[1], u[1], id[1], with_id);
+				if (!lt && !gt && u[0] != ~0u) tie = true;
+				if (lt) {
+					const double td = d[0]; d[0] = d[1]; d[1] = td;
+					const unsigned tu = u[0]; u[0] = u[1]; u[1] = tu;
+					const int ti = id[0]; id[0] = id[1]; id[1] = ti;
+				}
+				continue;
+			}
 #pragma unroll
-		for (int q = 0; q < P; ++q) {
-			const unsigned sq = u[q] >> kSegShift;
-			rk[q] += (sj < sq || (sj == sq && dj < d[q])) ? 1 : 0;
-			eq += (j != lane + 64 * q && sj == sq && dj == d[q]) ? 1 : 0;
+			for (int q = 0; q < P; ++q) {
+				const double dp = shfl_xor_d(d[q], j);
+				const unsigned up = (unsigned)__shfl_xor((int)u[q], j);
+				const int ip = with_id ? __shfl_xor(id[q], j) : 0;
+				const int idx = P == 1 ? (lane & (S - 1)) : lane + 64 * q;
+				const bool asc = (idx & k) == 0;
+				const bool lower = (lane & j) == 0;
+				const bool p_less = key_less(dp, up, ip, d[q], u[q], id[q], with_id);
+				const bool m_less = key_less(d[q], u[q], id[q], dp, up, ip, with_id);
+				if (!p_less && !m_less && u[q] != ~0u) tie = true;
+				if (lower == asc ? p_less : m_less) { d[q] = dp; u[q] = up; id[q] = ip; }
+			}
 		}
 	}
+	return tie;
 }
 UD void rank_long_lists(const StepArgs& A, int row, int n_row, int kcap) {
 	const int lane = threadIdx.x & 63;
 	const size_t st = (size_t)A.long_stride;
+	const State& S = A.in;
 	// the rows were written by their own lanes: make those stores visible to the wave
 	// (one wave per workgroup, so workgroup scope is the wave)
 	__builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup");
-	for (int r = 0; r < 64; ++r) {
-		const int n = __builtin_amdgcn_readlane(n_row, r);
-		if (n == 0) continue;
-		const int rw = __builtin_amdgcn_readlane(row, r);
-		double d[2] = {0.0, 0.0};
-		unsigned u[2] = {0u, 0u};
-		const bool h0 = lane < n, h1 = lane + 64 < n;
-		if (h0) { d[0] = A.long_d2[(size_t)lane * st + rw]; u[0] = A.long_u[(size_t)lane * st + rw]; }
-		if (h1) { d[1] = A.long_d2[(size_t)(lane + 64) * st + rw]; u[1] = A.long_u[(size_t)(lane + 64) * st + rw]; }
-		// key order: segment, then d2; equal keys are counted and settled by the IDs
-		int rk[2] = {0, 0}, eq = 0;
-		if (n <= 64) rank_parts<1>(n, d, u, rk, eq);
-		else rank_parts<2>(n, d, u, rk, eq);
-		if (!h0) eq = 0;
-		if (__ballot(eq > 0)) {
-			const State& S = A.in;
-			const int id0 = h0 ? S.meta[u[0] & kIndexMask].x : 0;
-			const int id1 = h1 ? S.meta[u[1] & kIndexMask].x : 0;
-			for (int j = 0; j < n; ++j) {
-				const int src = j & 63;
-				const bool hi = j >= 64;
-				const double dj = readlane_d(hi ? d[1] : d[0], src);
-				const unsigned sj = __builtin_amdgcn_readlane(hi ? u[1] : u[0], src) >> kSegShift;
-				const int ij = __builtin_amdgcn_readlane(hi ? id1 : id0, src);
-				rk[0] += (h0 && j != lane && sj == (u[0] >> kSegShift) && dj == d[0] && ij < id0) ? 1 : 0;
-				rk[1] += (h1 && j != lane + 64 && sj == (u[1] >> kSegShift) && dj == d[1] && ij < id1) ? 1 : 0;
+	// rows of up to 16, 32 and 64 entries: 4, 2 and 1 per pass, one entry per lane
+	for (int lg = 4; lg <= 6; ++lg) {
+		const int SZ = 1 << lg, G = 64 >> lg;
+		unsigned long long m = __ballot(n_row > (lg == 4 ? 0 : SZ / 2) && n_row <= SZ);
+		while (m) {
+			// the pass's rows: the next G set bits of m (uniform), lane group g = lane >> lg
+			int rsel = -1;
+			for (int g = 0; g < G; ++g) {
+				const int r = m ? __builtin_ctzll(m) : -1;
+				if (m) m &= m - 1;
+				if ((lane >> lg) == g) rsel = r;
 			}
+			const int src = rsel < 0 ? 0 : rsel;          // every lane shuffles
+			const int n_src = __shfl(n_row, src), rw = __shfl(row, src);
+			const int n = rsel >= 0 ? n_src : 0;
+			const int i = lane & (SZ - 1);
+			double d[2] = {__longlong_as_double(0x7ff0000000000000ll), 0.0};
+			unsigned u[2] = {~0u, ~0u};
+			int id[2] = {0, 0};
+			if (i < n) {
+				d[0] = A.long_d2[(size_t)i * st + rw];
+				u[0] = A.long_u[(size_t)i * st + rw];
+			}
+			if (__ballot(bitonic<1>(d, u, id, SZ, false))) {
+				// equal distances in this pass: sort its rows again with the IDs
+				d[0] = __longlong_as_double(0x7ff0000000000000ll);
+				u[0] = ~0u;
+				if (i < n) {
+					d[0] = A.long_d2[(size_t)i * st + rw];
+					u[0] = A.long_u[(size_t)i * st + rw];
+					id[0] = S.meta[u[0] & kIndexMask].x;
+				}
+				bitonic<1>(d, u, id, SZ, true);
+			}
+			const int keep = n < kcap ? n : kcap;
+			if (i < keep) A.long_u[(size_t)i * st + rw] = u[0];
 		}
-		if (h0 && rk[0] < kcap) A.long_u[(size_t)rk[0] * st + rw] = u[0];
-		if (h1 && rk[1] < kcap) A.long_u[(size_t)rk[1] * st + rw] = u[1];
+	}
+	// rows of up to 128 entries: one per pass, two entries per lane
+	unsigned long long m = __ballot(n_row > 64);
+	while (m) {
+		const int r = __builtin_ctzll(m);
+		m &= m - 1;
+		const int n = __shfl(n_row, r), rw = __shfl(row, r);
+		double d[2];
+		unsigned u[2];
+		int id[2] = {0, 0};
+		auto load = [&](bool ids) {
+#pragma unroll
+			for (int q = 0; q < 2; ++q) {
+				const int e = lane + 64 * q;
+				d[q] = __longlong_as_double(0x7ff0000000000000ll);
+				u[q] = ~0u;
+				if (e < n) {
+					d[q] = A.long_d2[(size_t)e * st + rw];
+					u[q] = A.long_u[(size_t)e * st + rw];
+					if (ids) id[q] = S.meta[u[q] & kIndexMask].x;
+				}
+			}
+		};
+		load(false);
+		if (__ballot(bitonic<2>(d, u, id, 128, false))) {
+			load(true);
+			bitonic<2>(d, u, id, 128, true);
+		}
+		const int keep = n < kcap ? n : kcap;
+		if (lane < keep) A.long_u[(size_t)lane * st + rw] = u[0];
+		if (lane + 64 < keep) A.long_u[(size_t)(lane + 64) * st + rw] = u[1];
 	}
 	// the sorted rows are read back by their own lanes
 	__builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup");
