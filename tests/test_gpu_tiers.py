@@ -79,22 +79,55 @@ def test_ranked_lists_beyond_their_capacity(world_type):
     assert st["max_list"] == rst["max_list"]
 
 
+def lattice(side, spacing, jitter, seed=5):
+    """Agents on a square lattice; with jitter, each coordinate moved by 0 or +-2^-13 m,
+    so that many distances in a list round to the same float but differ as doubles."""
+    g = (np.arange(side, dtype=np.float32) - side / 2 + 0.5) * np.float32(spacing)
+    xx, yy = np.meshgrid(g, g)
+    pos = np.stack([xx.ravel(), yy.ravel()], 1).astype(np.float32)
+    if jitter:
+        rng = np.random.default_rng(seed)
+        pos += rng.choice(np.float32([0.0, 2.0 ** -13, -(2.0 ** -13)]), pos.shape).astype(np.float32)
+    return pos
+
+
+def float_only_ties(pos, rng_m):
+    """Number of agents whose in-range list holds two distances equal as floats and
+    unequal as doubles (infinite world)."""
+    p = pos.astype(np.float64)
+    n = 0
+    for i in range(len(p)):
+        d2 = ((p - p[i]) ** 2).sum(1)
+        d2 = d2[(d2 < np.float64(np.float32(rng_m * rng_m))) & (np.arange(len(p)) != i)]
+        f = d2.astype(np.float32)
+        o = np.argsort(d2, kind="stable")
+        d2, f = d2[o], f[o]
+        n += bool(np.any((f[1:] == f[:-1]) & (d2[1:] != d2[:-1])))
+    return n
+
+
 @pytest.mark.parametrize("world_type", [0, 1])
-def test_ranked_lists_on_a_lattice(world_type):
-    """Agents on a square lattice: every list holds many equal distances, ordered by agent
-    ID. The ranked lists (sorted without the IDs, then again with them where distances
-    tie) equal the sorted insertion's, bit for bit."""
+@pytest.mark.parametrize("spacing", [0.75, 1.5, 2.5])
+@pytest.mark.parametrize("jitter", [False, True])
+def test_ranked_lists_on_a_lattice(world_type, spacing, jitter):
+    """Agents on a square lattice (range 4 m; spacings giving lists of ~89, ~22 and ~8
+    entries, so rows of every size the wave sorts): every list holds many equal distances, ordered by
+    agent ID; with jitter, also distances equal as floats and unequal as doubles, ordered
+    by the doubles. The ranked lists (sorted by packed keys, then again by exact doubles
+    and IDs where floats tie) equal the sorted insertion's, bit for bit."""
+    side = 36
+    pos = lattice(side, spacing, jitter)
+    if jitter:
+        assert float_only_ties(pos[:100], 4.0) > 0, "the jitter must make float-only ties"
+
     def run(list_capacity):
         method, costs, samp, relax = CASES["powerlaw_sampling"]
-        side = 36
-        g = (np.arange(side, dtype=np.float32) - side / 2 + 0.5) * np.float32(1.0)
-        xx, yy = np.meshgrid(g, g)
-        sc = S.random_cloud(side * side, float(side), [None], world_type=world_type, seed=33)
+        sc = S.random_cloud(side * side, float(side * spacing), [None], world_type=world_type, seed=33)
         a = dict(sc.agents)
-        a["pos"] = np.stack([xx.ravel(), yy.ravel()], 1).astype(np.float32)
+        a["pos"] = pos
         rng = np.random.default_rng(5)
         a["id"] = rng.permutation(side * side).astype(np.int32)   # IDs unrelated to positions
-        w = gpu.World(world_type, float(side), float(side), sc.dt,
+        w = gpu.World(world_type, float(side * spacing), float(side * spacing), sc.dt,
                       [pack_policy(method, costs, sampling=samp, relaxation_time=relax)])
         w.upload(a)
         w.set_tuning(1.0, 0, list_capacity)

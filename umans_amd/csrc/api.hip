@@ -168,7 +168,7 @@ struct umans_ctx {
 	unsigned* scan_sums = nullptr;
 	unsigned* cell_of = nullptr;
 	int* class_list = nullptr;                // sorted index of each class-list entry
-	double* long_d2 = nullptr;                // long lists: kLongCap x cap collection buffers
+	unsigned long long* long_key = nullptr;   // long lists: kLongCap x cap collection buffers
 	unsigned* long_u = nullptr;
 	// list-capacity tiers (uncapped and large caps): agents whose list outgrew a tier,
 	// per tier, in their class's range; and the counts per class and tier
@@ -335,10 +335,10 @@ struct umans_ctx {
 		free_adjoint();
 		X.release(); Y.release(); B.release();
 		(void)hipFree(cell_of); (void)hipFree(ovf[0]); (void)hipFree(ovf[1]); (void)hipFree(class_list);
-		(void)hipFree(long_d2); (void)hipFree(long_u);
+		(void)hipFree(long_key); (void)hipFree(long_u);
 		ovf[0] = ovf[1] = nullptr;
 		class_list = nullptr;
-		long_d2 = nullptr;
+		long_key = nullptr;
 		long_u = nullptr;
 		(void)hipFree(rec_count); (void)hipFree(rec_id); (void)hipFree(rec_d2); (void)hipFree(rec_seg);
 		cell_of = nullptr;
@@ -782,7 +782,7 @@ int enqueue_step(umans_ctx* c, bool events, const State* ysave = nullptr, unsign
 			// range are redone by a 128-entry sorted insertion, which keeps the nearest.
 			// Results are those of a single launch at full capacity.
 			StepArgs al = aq;
-			al.long_d2 = c->long_d2;
+			al.long_key = c->long_key;
 			al.long_u = c->long_u;
 			al.long_stride = c->cap;
 			al.tier_final = 0;
@@ -916,7 +916,7 @@ int grow_agents(umans_ctx* c, int n, int keep) {
 	if ((rc = c->hip(dalloc(&c->cell_of, cap), "alloc grid"))) return rc;
 	if ((rc = c->hip(dalloc(&c->class_list, cap), "alloc grid"))) return rc;
 	if (kmax_bucket(c->kcap) > 32) {
-		if ((rc = c->hip(dalloc(&c->long_d2, (size_t)kLongCap * cap), "alloc lists"))) return rc;
+		if ((rc = c->hip(dalloc(&c->long_key, (size_t)kLongCap * cap), "alloc lists"))) return rc;
 		if ((rc = c->hip(dalloc(&c->long_u, (size_t)kLongCap * cap), "alloc lists"))) return rc;
 	}
 	if ((rc = c->hip(dalloc(&c->ovf[0], cap), "alloc grid"))) return rc;
@@ -1335,7 +1335,7 @@ int umans_upload(umans_ctx* c, const umans_agents* a, int n) {
 		if ((rc = c->hip(dalloc(&c->cell_of, n), "alloc grid"))) return rc;
 		if ((rc = c->hip(dalloc(&c->class_list, n), "alloc grid"))) return rc;
 		if (kmax_bucket(c->kcap) > 32) {
-			if ((rc = c->hip(dalloc(&c->long_d2, (size_t)kLongCap * n), "alloc lists"))) return rc;
+			if ((rc = c->hip(dalloc(&c->long_key, (size_t)kLongCap * n), "alloc lists"))) return rc;
 			if ((rc = c->hip(dalloc(&c->long_u, (size_t)kLongCap * n), "alloc lists"))) return rc;
 		}
 		if ((rc = c->hip(dalloc(&c->ovf[0], n), "alloc grid"))) return rc;

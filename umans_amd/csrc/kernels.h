@@ -380,9 +380,22 @@ UD int find_neighbors(const StepArgs& A, const WorldInfo& wi, const GridInfo& g,
 	return filled;
 }
 
+// A long-list entry packed into one 64-bit key: segment (bits 61-59), the bits of
+// float(d2) (58-28; d2 >= 0, so the sign bit is clear) and the sorted index (27-0).
+// Rounding to float is monotonic, so the keys order entries as (segment, double d2)
+// does wherever their floats differ; two entries of one row with equal segment and
+// float are resolved by their exact doubles and agent IDs.
+constexpr unsigned long long kKeyPad = ~0ull;   // padding: orders after every entry
+UD unsigned long long long_key(int s, double d2, unsigned j) {
+	return ((unsigned long long)s << 59) | ((unsigned long long)__float_as_uint((float)d2) << 28) | j;
+}
+UD unsigned key_entry(unsigned long long k) {   // sorted index | segment << kSegShift
+	return (unsigned)(k & kIndexMask) | ((unsigned)(k >> 59) << kSegShift);
+}
+
 // Long lists, step 1 (one lane per agent): the same queries and exact test as
 // find_neighbors, but every candidate in range is appended, unsorted, to the lane's row of
-// the collection buffers (double d2, sorted index | segment), at most kLongCap of them.
+// the collection buffer (packed keys, below), at most kLongCap of them.
 // Returns the number in range (all of them, as find_neighbors' total).
 UD int collect_neighbors(const StepArgs& A, const WorldInfo& wi, const GridInfo& g, int self, V2 p, float range,
                          int row) {
@@ -414,10 +427,7 @@ UD int collect_neighbors(const StepArgs& A, const WorldInfo& wi, const GridInfo&
 				const double ddy = qy - (double)pj.y;
 				const double d2 = ddx * ddx + ddy * ddy;
 				if (!(d2 < r2) || (int)j == self) continue;
-				if (n < kLongCap) {
-					A.long_d2[(size_t)n * st + row] = d2;
-					A.long_u[(size_t)n * st + row] = j | ((unsigned)s << kSegShift);
-				}
+				if (n < kLongCap) A.long_key[(size_t)n * st + row] = long_key(s, d2, j);
 				++n;
 			}
 		}
@@ -427,39 +437,69 @@ UD int collect_neighbors(const StepArgs& A, const WorldInfo& wi, const GridInfo&
 
 // Long lists, step 2 (the whole wave, in uniform control flow): the wave sorts its rows
 // with bitonic networks across lanes, in the canonical order (segment, d2, agent ID),
-// and writes each row's first kcap entries back into long_u in that order. Rows of up to
-// 16, 32 and 64 entries are sorted four, two and one at a time (one entry per lane);
-// rows of up to 128 one at a time with two entries per lane. The agent IDs are fetched
-// only for a group of rows in which two entries have the same segment and distance; it
-// is then sorted again with them. n_row: the row's entry count (0 for rows the wave does
-// not sort).
-UD double shfl_xor_d(double v, int m) {
-	const long long b = __double_as_longlong(v);
-	const int lo = __shfl_xor((int)b, m), hi = __shfl_xor((int)(b >> 32), m);
-	return __longlong_as_double(((long long)hi << 32) | (unsigned)lo);
+// and writes each row's first kcap entries into long_u in that order. Rows of up to 16,
+// 32 and 64 entries are sorted four, two and one at a time (one entry per lane); rows of
+// up to 128 one at a time with two entries per lane. The networks sort the packed keys;
+// a pass in which two neighbouring entries of a row share segment and float(d2) sorts
+// its rows again by (segment, exact d2, agent ID), the doubles recomputed from the
+// positions as the search computed them. n_row: the row's entry count (0 for rows the
+// wave does not sort); p, wi: the lane's own agent position and world.
+UD unsigned long long shfl_xor_u64(unsigned long long v, int m) {
+	const int lo = __shfl_xor((int)v, m), hi = __shfl_xor((int)(v >> 32), m);
+	return ((unsigned long long)(unsigned)hi << 32) | (unsigned)lo;
 }
-UD bool key_less(double da, unsigned ua, int ia, double db, unsigned ub, int ib, bool with_id) {
+UD unsigned long long shfl_u64(unsigned long long v, int src) {
+	const int lo = __shfl((int)v, src), hi = __shfl((int)(v >> 32), src);
+	return ((unsigned long long)(unsigned)hi << 32) | (unsigned)lo;
+}
+UD double shfl_xor_d(double v, int m) {
+	return __longlong_as_double((long long)shfl_xor_u64((unsigned long long)__double_as_longlong(v), m));
+}
+UD bool key_less(double da, unsigned ua, int ia, double db, unsigned ub, int ib) {
 	const unsigned sa = ua >> kSegShift, sb = ub >> kSegShift;
 	if (sa != sb) return sa < sb;
 	if (da != db) return da < db;
-	return with_id && ia < ib;
+	return ia < ib;
 }
-// One bitonic sort over blocks of S lanes (P = 1, S <= 64) or over 128 entries, two per
-// lane (P = 2: entry lane and lane + 64). Returns whether two different entries compared
-// equal (only possible without the IDs); padding entries (u = ~0) compare after all.
+UD bool same_float_key(unsigned long long a, unsigned long long b) {
+	return a != kKeyPad && ((a ^ b) >> 28) == 0;
+}
+// One bitonic sort of packed keys over blocks of S lanes (P = 1, S <= 64) or over 128
+// entries, two per lane (P = 2: entry lane and lane + 64).
 template <int P>
-UD bool bitonic(double (&d)[2], unsigned (&u)[2], int (&id)[2], int S, bool with_id) {
+UD void bitonic_keys(unsigned long long (&k)[2], int S) {
 	const int lane = threadIdx.x & 63;
 	const int N = P == 1 ? S : 128;
-	bool tie = false;
-	for (int k = 2; k <= N; k <<= 1) {
-		for (int j = k >> 1; j > 0; j >>= 1) {
+	for (int w = 2; w <= N; w <<= 1) {
+		for (int j = w >> 1; j > 0; j >>= 1) {
 			if (P == 2 && j == 64) {
-				// entries lane and lane + 64 of one lane; k == 128: ascending
-				const bool lt = key_less(d[1], u[1], id[1], d[0], u[0], id[0], with_id);
-				const bool gt = key_less(d[0], u[0], id[0], d[1], u[1], id[1], with_id);
-				if (!lt && !gt && u[0] != ~0u) tie = true;
-				if (lt) {
+				// entries lane and lane + 64 of one lane; w == 128: ascending
+				const unsigned long long lo = k[0] < k[1] ? k[0] : k[1];
+				const unsigned long long hi = k[0] < k[1] ? k[1] : k[0];
+				k[0] = lo; k[1] = hi;
+				continue;
+			}
+#pragma unroll
+			for (int q = 0; q < P; ++q) {
+				const unsigned long long kp = shfl_xor_u64(k[q], j);
+				const int idx = P == 1 ? (lane & (S - 1)) : lane + 64 * q;
+				const bool take_min = ((idx & w) == 0) == ((lane & j) == 0);
+				const bool p_less = kp < k[q];
+				if (take_min ? p_less : !p_less) k[q] = kp;
+			}
+		}
+	}
+}
+// The same network over full entries (segment, exact d2, agent ID): the rare second
+// sort of a pass whose keys tie. Padding entries (u = ~0) compare after all.
+template <int P>
+UD void bitonic_exact(double (&d)[2], unsigned (&u)[2], int (&id)[2], int S) {
+	const int lane = threadIdx.x & 63;
+	const int N = P == 1 ? S : 128;
+	for (int w = 2; w <= N; w <<= 1) {
+		for (int j = w >> 1; j > 0; j >>= 1) {
+			if (P == 2 && j == 64) {
+				if (key_less(d[1], u[1], id[1], d[0], u[0], id[0])) {
 					const double td = d[0]; d[0] = d[1]; d[1] = td;
 					const unsigned tu = u[0]; u[0] = u[1]; u[1] = tu;
 					const int ti = id[0]; id[0] = id[1]; id[1] = ti;
@@ -470,20 +510,36 @@ UD bool bitonic(double (&d)[2], unsigned (&u)[2], int (&id)[2], int S, bool with
 			for (int q = 0; q < P; ++q) {
 				const double dp = shfl_xor_d(d[q], j);
 				const unsigned up = (unsigned)__shfl_xor((int)u[q], j);
-				const int ip = with_id ? __shfl_xor(id[q], j) : 0;
+				const int ip = __shfl_xor(id[q], j);
 				const int idx = P == 1 ? (lane & (S - 1)) : lane + 64 * q;
-				const bool asc = (idx & k) == 0;
+				const bool asc = (idx & w) == 0;
 				const bool lower = (lane & j) == 0;
-				const bool p_less = key_less(dp, up, ip, d[q], u[q], id[q], with_id);
-				const bool m_less = key_less(d[q], u[q], id[q], dp, up, ip, with_id);
-				if (!p_less && !m_less && u[q] != ~0u) tie = true;
+				const bool p_less = key_less(dp, up, ip, d[q], u[q], id[q]);
+				const bool m_less = key_less(d[q], u[q], id[q], dp, up, ip);
 				if (lower == asc ? p_less : m_less) { d[q] = dp; u[q] = up; id[q] = ip; }
 			}
 		}
 	}
-	return tie;
 }
-UD void rank_long_lists(const StepArgs& A, int row, int n_row, int kcap) {
+// Full entry of key k of the row whose agent stands at (px, py) in a world of width W
+// and height H: exact d2 (as entry_d2 computes it), index | segment, agent ID.
+UD void exact_entry(const State& S, float px, float py, float W, float H, unsigned long long k,
+                    double& d, unsigned& u, int& id) {
+	d = __longlong_as_double(0x7ff0000000000000ll);
+	u = ~0u;
+	id = 0;
+	if (k == kKeyPad) return;
+	u = key_entry(k);
+	const unsigned sg = u >> kSegShift;
+	const float qx = px + (sg == 1 ? W : (sg == 2 ? -W : 0.0f));
+	const float qy = py + (sg == 3 ? H : (sg == 4 ? -H : 0.0f));
+	const float2 pj = S.pos[u & kIndexMask];
+	const double ddx = (double)qx - (double)pj.x;
+	const double ddy = (double)qy - (double)pj.y;
+	d = ddx * ddx + ddy * ddy;
+	id = S.meta[u & kIndexMask].x;
+}
+UD void rank_long_lists(const StepArgs& A, int row, int n_row, int kcap, V2 p, const WorldInfo& wi) {
 	const int lane = threadIdx.x & 63;
 	const size_t st = (size_t)A.long_stride;
 	const State& S = A.in;
@@ -506,26 +562,24 @@ UD void rank_long_lists(const StepArgs& A, int row, int n_row, int kcap) {
 			const int n_src = __shfl(n_row, src), rw = __shfl(row, src);
 			const int n = rsel >= 0 ? n_src : 0;
 			const int i = lane & (SZ - 1);
-			double d[2] = {__longlong_as_double(0x7ff0000000000000ll), 0.0};
-			unsigned u[2] = {~0u, ~0u};
-			int id[2] = {0, 0};
-			if (i < n) {
-				d[0] = A.long_d2[(size_t)i * st + rw];
-				u[0] = A.long_u[(size_t)i * st + rw];
-			}
-			if (__ballot(bitonic<1>(d, u, id, SZ, false))) {
-				// equal distances in this pass: sort its rows again with the IDs
-				d[0] = __longlong_as_double(0x7ff0000000000000ll);
-				u[0] = ~0u;
-				if (i < n) {
-					d[0] = A.long_d2[(size_t)i * st + rw];
-					u[0] = A.long_u[(size_t)i * st + rw];
-					id[0] = S.meta[u[0] & kIndexMask].x;
-				}
-				bitonic<1>(d, u, id, SZ, true);
-			}
+			unsigned long long k[2] = {kKeyPad, kKeyPad};
+			if (i < n) k[0] = A.long_key[(size_t)i * st + rw];
+			bitonic_keys<1>(k, SZ);
+			// ties sit next to each other once sorted
+			const unsigned long long nx = shfl_u64(k[0], (lane + 1) & 63);
 			const int keep = n < kcap ? n : kcap;
-			if (i < keep) A.long_u[(size_t)i * st + rw] = u[0];
+			if (__ballot(i + 1 < SZ && same_float_key(k[0], nx))) {
+				const float px = __shfl(p.x, src), py = __shfl(p.y, src);
+				const float W = __shfl(wi.W, src), H = __shfl(wi.H, src);
+				double d[2];
+				unsigned u[2];
+				int id[2];
+				exact_entry(S, px, py, W, H, k[0], d[0], u[0], id[0]);
+				bitonic_exact<1>(d, u, id, SZ);
+				if (i < keep) A.long_u[(size_t)i * st + rw] = u[0];
+			} else if (i < keep) {
+				A.long_u[(size_t)i * st + rw] = key_entry(k[0]);
+			}
 		}
 	}
 	// rows of up to 128 entries: one per pass, two entries per lane
@@ -534,30 +588,35 @@ UD void rank_long_lists(const StepArgs& A, int row, int n_row, int kcap) {
 		const int r = __builtin_ctzll(m);
 		m &= m - 1;
 		const int n = __shfl(n_row, r), rw = __shfl(row, r);
-		double d[2];
-		unsigned u[2];
-		int id[2] = {0, 0};
-		auto load = [&](bool ids) {
+		unsigned long long k[2];
 #pragma unroll
-			for (int q = 0; q < 2; ++q) {
-				const int e = lane + 64 * q;
-				d[q] = __longlong_as_double(0x7ff0000000000000ll);
-				u[q] = ~0u;
-				if (e < n) {
-					d[q] = A.long_d2[(size_t)e * st + rw];
-					u[q] = A.long_u[(size_t)e * st + rw];
-					if (ids) id[q] = S.meta[u[q] & kIndexMask].x;
-				}
-			}
-		};
-		load(false);
-		if (__ballot(bitonic<2>(d, u, id, 128, false))) {
-			load(true);
-			bitonic<2>(d, u, id, 128, true);
+		for (int q = 0; q < 2; ++q) {
+			const int e = lane + 64 * q;
+			k[q] = e < n ? A.long_key[(size_t)e * st + rw] : kKeyPad;
 		}
+		bitonic_keys<2>(k, 128);
+		// sorted position of k[q] is lane + 64 q: the next entry of k[0] on lane 63 is k[1]
+		// of lane 0
+		const unsigned long long n0 = shfl_u64(k[0], (lane + 1) & 63);
+		const unsigned long long n1 = shfl_u64(k[1], (lane + 1) & 63);
+		const unsigned long long f1 = shfl_u64(k[1], 0);
+		const bool tie = same_float_key(k[0], lane < 63 ? n0 : f1) || (lane < 63 && same_float_key(k[1], n1));
 		const int keep = n < kcap ? n : kcap;
-		if (lane < keep) A.long_u[(size_t)lane * st + rw] = u[0];
-		if (lane + 64 < keep) A.long_u[(size_t)(lane + 64) * st + rw] = u[1];
+		if (__ballot(tie)) {
+			const float px = __shfl(p.x, r), py = __shfl(p.y, r);
+			const float W = __shfl(wi.W, r), H = __shfl(wi.H, r);
+			double d[2];
+			unsigned u[2];
+			int id[2];
+			exact_entry(S, px, py, W, H, k[0], d[0], u[0], id[0]);
+			exact_entry(S, px, py, W, H, k[1], d[1], u[1], id[1]);
+			bitonic_exact<2>(d, u, id, 128);
+			if (lane < keep) A.long_u[(size_t)lane * st + rw] = u[0];
+			if (lane + 64 < keep) A.long_u[(size_t)(lane + 64) * st + rw] = u[1];
+		} else {
+			if (lane < keep) A.long_u[(size_t)lane * st + rw] = key_entry(k[0]);
+			if (lane + 64 < keep) A.long_u[(size_t)(lane + 64) * st + rw] = key_entry(k[1]);
+		}
 	}
 	// the sorted rows are read back by their own lanes
 	__builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup");
@@ -799,7 +858,7 @@ __global__ __launch_bounds__(BS) void k_step(StepArgs A) {
 	const int kcap_eff = A.kcap < KL ? A.kcap : KL;
 	if constexpr (kLong) {
 		const bool keep = active && total <= kLongCap;
-		rank_long_lists(A, A.base + t, keep ? total : 0, kcap_eff);
+		rank_long_lists(A, A.base + t, keep ? total : 0, kcap_eff, a.pos, wi);
 		cnt = keep ? (total < kcap_eff ? total : kcap_eff) : 0;
 	}
 	// obstacle edges: only the generic class runs when a world has obstacles

@@ -123,10 +123,11 @@ struct StepArgs {
 	unsigned* next_cell_of;
 	int next_key_base;
 	// long lists (the step kernel instance with KMAX = 0): lane t collects its in-range
-	// candidates unsorted into row base + t of these buffers (entry e at
-	// [e * long_stride + row]), at most kLongCap of them; the wave then ranks each row
-	// and writes the sorted entries back into long_u
-	double* long_d2;
+	// candidates unsorted into row base + t of long_key (entry e at
+	// [e * long_stride + row], packed as segment | float(d2) | index), at most kLongCap
+	// of them; the wave then sorts each row and writes its entries, in order, into
+	// long_u (index | segment, same layout)
+	unsigned long long* long_key;
 	unsigned* long_u;
 	int long_stride;
 };
