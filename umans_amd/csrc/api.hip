@@ -187,7 +187,7 @@ struct umans_ctx {
 	GridInfo* dgrids = nullptr;
 	int* dbbox = nullptr;
 	bool any_infinite = false;
-	unsigned long long* dstats = nullptr;   // 4 counters, see StepArgs::stats
+	unsigned long long* dstats = nullptr;   // kStatCopies x kStatLine, see StepArgs::stats
 	float4* d_edges = nullptr;                // obstacle edges (single world)
 	int n_edges = 0;
 	int kcap = 0;
@@ -703,12 +703,13 @@ int enqueue_step(umans_ctx* c, bool events, const State* ysave = nullptr, unsign
 		hipLaunchKernelGGL(k_fill, dim3(ceil_div(c->cell_alloc, 256)), dim3(256), 0, st, c->cell_count, c->cell_alloc, 0u);
 	c->counts_zero_n = -1;
 	if (c->any_infinite) {
-		hipLaunchKernelGGL(k_bbox_init, dim3(ceil_div(nw, 256)), dim3(256), 0, st, c->dbbox, nw);
+		hipLaunchKernelGGL(k_bbox_init, dim3(ceil_div(nw * kBoxCopies, 256)), dim3(256), 0, st, c->dbbox, nw);
 		hipLaunchKernelGGL(k_bbox, dim3(ceil_div(n, 256)), dim3(256), 0, st, c->X.s.pos, c->X.s.meta, c->dworlds, n,
 		                   c->dbbox);
 	}
-	hipLaunchKernelGGL(k_grid_setup, dim3(ceil_div(std::max(nw, 2 * NUM_CLS), 64)), dim3(64), 0, st, c->dworlds, c->dbbox,
-	                   c->dgrids, nw, (unsigned*)c->dstats, 8, c->ovf_count, 2 * NUM_CLS);
+	constexpr int stat_words = 2 * kStatCopies * kStatLine;
+	hipLaunchKernelGGL(k_grid_setup, dim3(ceil_div(std::max(std::max(nw, stat_words), 2 * NUM_CLS), 64)), dim3(64), 0, st,
+	                   c->dworlds, c->dbbox, c->dgrids, nw, (unsigned*)c->dstats, stat_words, c->ovf_count, 2 * NUM_CLS);
 	if (!have_keys)
 		hipLaunchKernelGGL(k_count, dim3(ceil_div(n, 256)), dim3(256), 0, st, c->X.s.pos, c->X.s.meta, n, c->dworlds,
 		                   c->dgrids, c->dpol, c->d_cls_slot, c->total_cells, c->cell_count, c->cell_of);
@@ -1208,8 +1209,12 @@ int create_impl(const umans_world_desc* descs, int nworlds, umans_ctx** out) {
 		             hipEventCreateWithFlags(&c->join_ev[q], hipEventDisableTiming) == hipSuccess;
 	if (!streams_ok || hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess ||
 	    dalloc(&c->dgrids, nworlds) != hipSuccess || dalloc(&c->dworlds, nworlds) != hipSuccess ||
-	    dalloc(&c->dbbox, 4 * nworlds) != hipSuccess || dalloc(&c->dstats, 4) != hipSuccess || dalloc(&c->ovf_count, 2 * NUM_CLS) != hipSuccess ||
-	    dalloc(&c->dpol, kMaxPolicies) != hipSuccess || dalloc(&c->d_cls_slot, NUM_CLS) != hipSuccess) {
+	    dalloc(&c->dbbox, (size_t)kBoxCopies * kBoxLine * nworlds) != hipSuccess ||
+	    dalloc(&c->dstats, (size_t)kStatCopies * kStatLine) != hipSuccess ||
+	    dalloc(&c->ovf_count, 2 * NUM_CLS) != hipSuccess || dalloc(&c->dpol, kMaxPolicies) != hipSuccess ||
+	    dalloc(&c->d_cls_slot, NUM_CLS) != hipSuccess ||
+	    // the counters read before the first step: zero
+	    hipMemsetAsync(c->dstats, 0, (size_t)kStatCopies * kStatLine * sizeof(unsigned long long), c->stream) != hipSuccess) {
 		umans_destroy(c);
 		return UMANS_EHIP;
 	}
@@ -1622,10 +1627,18 @@ int umans_synchronize(umans_ctx* c) {
 int umans_get_stats(umans_ctx* c, umans_stats* out) {
 	if (!c || !out) return UMANS_EINVAL;
 	if (const int rc_dev = c->hip(hipSetDevice(c->device), "set device")) return rc_dev;
-	unsigned long long h[3] = {0, 0, 0};
-	int rc = c->hip(hipMemcpyAsync(h, c->dstats, sizeof(h), hipMemcpyDeviceToHost, c->stream), "stats");
+	std::vector<unsigned long long> hc((size_t)kStatCopies * kStatLine);
+	int rc = c->hip(hipMemcpyAsync(hc.data(), c->dstats, hc.size() * sizeof(unsigned long long), hipMemcpyDeviceToHost,
+	                               c->stream), "stats");
 	if (rc) return rc;
 	if ((rc = c->hip(hipStreamSynchronize(c->stream), "stats"))) return rc;
+	unsigned long long h[3] = {0, 0, 0};
+	for (int cp = 0; cp < kStatCopies; ++cp) {
+		const unsigned long long* s = hc.data() + (size_t)cp * kStatLine;
+		h[0] += s[0];
+		h[1] = std::max(h[1], s[1]);
+		h[2] += s[2];
+	}
 	out->steps = c->steps;
 	out->overflow_agents = (int64_t)h[0];
 	out->max_list = (int64_t)h[1];

@@ -39,17 +39,27 @@ UD int wave_min_i(int v) {
 	for (int o = 32; o > 0; o >>= 1) v = min(v, __shfl_xor(v, o));
 	return v;
 }
+// The copy of the step counters a wave updates (its wave index in the launch).
+UD int stat_copy() {
+	return (int)((blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6)) & (kStatCopies - 1));
+}
 UD int wave_max_i(int v) {
 	for (int o = 32; o > 0; o >>= 1) v = max(v, __shfl_xor(v, o));
 	return v;
 }
 
-// Per world w, bbox[4w..4w+3] = ordered keys of min x, min y, max x, max y.
+// Per world w, kBoxCopies copies of (ordered keys of min x, min y, max x, max y), each
+// in a 128-byte line of its own, at bbox + (w * kBoxCopies + copy) * kBoxLine: atomics on
+// one line are serialised, so the waves of a world spread theirs over the copies, and
+// the grid set-up reduces them.
+constexpr int kBoxCopies = 8;
+constexpr int kBoxLine = 32;
 __global__ void k_bbox_init(int* bbox, int nworlds) {
 	const int i = blockIdx.x * blockDim.x + threadIdx.x;
-	if (i < nworlds) {
-		bbox[4 * i + 0] = 0x7fffffff; bbox[4 * i + 1] = 0x7fffffff;
-		bbox[4 * i + 2] = (int)0x80000000; bbox[4 * i + 3] = (int)0x80000000;
+	if (i < nworlds * kBoxCopies) {
+		int* b = bbox + (size_t)i * kBoxLine;
+		b[0] = 0x7fffffff; b[1] = 0x7fffffff;
+		b[2] = (int)0x80000000; b[3] = (int)0x80000000;
 	}
 }
 // Bounding boxes of the infinite worlds. Agents of a batch are grouped by world, so a
@@ -71,16 +81,19 @@ __global__ __launch_bounds__(256) void k_bbox(const float2* pos, const int4* met
 	}
 	const int w0 = __shfl(w, 0);
 	const bool uniform = __all(w == w0 || w == -1);
+	const int copy = (int)((blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6)) & (kBoxCopies - 1));
 	if (uniform) {
 		mnx = wave_min_i(mnx); mny = wave_min_i(mny); mxx = wave_max_i(mxx); mxy = wave_max_i(mxy);
 		const int wl = wave_max_i(w);
 		if ((threadIdx.x & 63) == 0 && wl >= 0) {
-			atomicMin(&bbox[4 * wl + 0], mnx); atomicMin(&bbox[4 * wl + 1], mny);
-			atomicMax(&bbox[4 * wl + 2], mxx); atomicMax(&bbox[4 * wl + 3], mxy);
+			int* b = bbox + ((size_t)wl * kBoxCopies + copy) * kBoxLine;
+			atomicMin(&b[0], mnx); atomicMin(&b[1], mny);
+			atomicMax(&b[2], mxx); atomicMax(&b[3], mxy);
 		}
 	} else if (w >= 0) {
-		atomicMin(&bbox[4 * w + 0], mnx); atomicMin(&bbox[4 * w + 1], mny);
-		atomicMax(&bbox[4 * w + 2], mxx); atomicMax(&bbox[4 * w + 3], mxy);
+		int* b = bbox + ((size_t)w * kBoxCopies + copy) * kBoxLine;
+		atomicMin(&b[0], mnx); atomicMin(&b[1], mny);
+		atomicMax(&b[2], mxx); atomicMax(&b[3], mxy);
 	}
 }
 // One thread per world. Infinite worlds: a grid over the bounding box with cells of
@@ -106,7 +119,12 @@ __global__ void k_grid_setup(const WorldInfo* worlds, const int* bbox, GridInfo*
 		r.ncx = (int)nx; r.ncy = (int)ny;
 		r.invx = (float)(nx / (double)W.W); r.invy = (float)(ny / (double)W.H);
 	} else {
-		const int* b = bbox + 4 * wi;
+		int b[4] = {0x7fffffff, 0x7fffffff, (int)0x80000000, (int)0x80000000};
+		for (int cp = 0; cp < kBoxCopies; ++cp) {
+			const int* bc = bbox + ((size_t)wi * kBoxCopies + cp) * kBoxLine;
+			b[0] = min(b[0], bc[0]); b[1] = min(b[1], bc[1]);
+			b[2] = max(b[2], bc[2]); b[3] = max(b[3], bc[3]);
+		}
 		const bool empty = b[0] > b[2];
 		const float x0 = empty ? 0.0f : from_ordered_key(b[0]);
 		const float y0 = empty ? 0.0f : from_ordered_key(b[1]);
@@ -827,6 +845,9 @@ __global__ __launch_bounds__(BS) void k_step(StepArgs A) {
 	const int t = xcd_tile(blockIdx.x, gridDim.x) * BS + tid;
 	const int n_here = A.agent_count ? (int)*A.agent_count : A.n;
 	const bool active = t < n_here;
+	// a wave without agents (most of a launch over a list counted on the device) leaves
+	// at once: it takes part in no wave-level step and adds nothing to the stats
+	if (__ballot(active) == 0ull) return;
 	const int k = active ? (A.agent_list ? A.agent_list[A.base + t] : A.class_list[A.base + t]) : 0;
 	int total = 0, cnt = 0;
 	const State& S = A.in;
@@ -868,7 +889,7 @@ __global__ __launch_bounds__(BS) void k_step(StepArgs A) {
 		if (active && A.n_edges > 0) {
 			int obs_total = 0;
 			n_obs = find_obstacles(A, wi, a.pos, A.pol[pi].query_range, obs_priv, obs_total);
-			if (obs_total > n_obs) atomicAdd(&A.stats[2], 1ull);
+			if (obs_total > n_obs) atomicAdd(&A.stats[stat_copy() * kStatLine + 2], 1ull);
 		}
 	}
 	// a list that outgrew this launch's capacity is redone by the next launch, which
@@ -881,8 +902,9 @@ __global__ __launch_bounds__(BS) void k_step(StepArgs A) {
 		const int mx = wave_max_i(total);
 		if ((tid & 63) == 0) {
 			const int c = __popcll(ov);
-			if (c) atomicAdd(&A.stats[0], (unsigned long long)c);
-			atomicMax(&A.stats[1], (unsigned long long)mx);
+			unsigned long long* sc = A.stats + stat_copy() * kStatLine;
+			if (c) atomicAdd(&sc[0], (unsigned long long)c);
+			atomicMax(&sc[1], (unsigned long long)mx);
 		}
 	}
 	if (!active) return;
@@ -1119,8 +1141,9 @@ __global__ __launch_bounds__(64) void k_step_coop(StepArgs A) {
 		const int mx = wave_max_i(total);
 		if (tid == 0) {
 			const int c = __popcll(ov);
-			if (c) atomicAdd(&A.stats[0], (unsigned long long)c);
-			atomicMax(&A.stats[1], (unsigned long long)mx);
+			unsigned long long* sc = A.stats + stat_copy() * kStatLine;
+			if (c) atomicAdd(&sc[0], (unsigned long long)c);
+			atomicMax(&sc[1], (unsigned long long)mx);
 		}
 	}
 	const DevPolicy& P = A.pol[pi];

@@ -26,6 +26,8 @@ constexpr int kPolicyMask = (1 << kPolicyBits) - 1;
 
 constexpr unsigned kSegShift = 28;
 constexpr unsigned kIndexMask = (1u << kSegShift) - 1u;
+constexpr int kStatCopies = 64;
+constexpr int kStatLine = 16;
 constexpr int kMaxAngleSamples = 64;
 
 // A policy as the kernels read it. The sampling fields are the effective ones (for a
@@ -103,7 +105,11 @@ struct StepArgs {
 	unsigned* raw_nb;
 	int* raw_count;
 	int raw_stride;
-	unsigned long long* stats;    // [0] overflow agents, [1] max list length, [2] obstacle overflow agents
+	// step counters, kStatCopies copies of kStatLine (one 128-byte line each): [0] overflow
+	// agents, [1] max list length, [2] obstacle overflow agents. Atomics on one line are
+	// serialised, so each wave updates the copy of its wave index (stat_copy); the host
+	// sums (0, 2) and maxes (1) the copies.
+	unsigned long long* stats;
 	// obstacle edges of a single-world context (Polygon2D::GetEdges order), or none
 	const float4* edges;
 	int n_edges;
