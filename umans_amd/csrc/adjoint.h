@@ -34,7 +34,11 @@ struct AdjArgs {
 	float2* adj_pos_in;           // accumulated adjoint of the input position / velocity / preferred
 	float2* adj_vel_in;           // velocity (row space; cleared before the launch)
 	float2* adj_pref;
-	float* adj_params;            // [policy][2]: (V0, sigma) of SocialForces or (k, tau0) of PowerLaw
+	// [copy][policy][2]: (V0, sigma) of SocialForces or (k, tau0) of PowerLaw; kStatCopies
+	// copies param_stride floats apart (whole 128-byte lines), each wave adding into the
+	// copy of its wave index (float atomics on one line are serialised); the host sums them
+	float* adj_params;
+	int param_stride;
 	int param_kind;
 };
 
@@ -300,8 +304,9 @@ __global__ __launch_bounds__(128) void k_step_adjoint(AdjArgs A) {
 				}
 			}
 		}
-		if (p0bar != 0.0f) atomicAdd(&A.adj_params[2 * pi + 0], p0bar);
-		if (p1bar != 0.0f) atomicAdd(&A.adj_params[2 * pi + 1], p1bar);
+		float* pc = A.adj_params + (size_t)stat_copy() * A.param_stride;
+		if (p0bar != 0.0f) atomicAdd(&pc[2 * pi + 0], p0bar);
+		if (p1bar != 0.0f) atomicAdd(&pc[2 * pi + 1], p1bar);
 	}
 	atomic_add2(&A.adj_pos_in[row], pbar);
 	atomic_add2(&A.adj_vel_in[row], vbar);

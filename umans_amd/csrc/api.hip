@@ -1180,6 +1180,9 @@ void launch_loss(umans_ctx* c, const float2* target, double* loss, float2* grad,
 	                   loss, grad, scale);
 }
 
+// Floats between copies of the adjoint's parameter accumulators: whole 128-byte lines.
+int adj_param_stride(int npol) { return (2 * npol + 31) / 32 * 32; }
+
 void fill_f(umans_ctx* c, void* p, long long nfloats) {
 	hipLaunchKernelGGL(k_fill_f, dim3(ceil_div(nfloats, 256)), dim3(256), 0, c->stream, (float*)p, nfloats, 0.0f);
 }
@@ -1747,7 +1750,8 @@ int umans_adjoint_forward(umans_ctx* c, const umans_adjoint_desc* d, const float
 	if ((rc = c->hip(dalloc(&c->seg_cnt, (size_t)segn * n), "alloc lists"))) return rc;
 	if ((rc = c->hip(dalloc(&c->seg_lg, (size_t)segn * n), "alloc loss grad"))) return rc;
 	if ((rc = c->hip(dalloc(&c->adj_buf, (size_t)5 * n), "alloc adjoint"))) return rc;
-	if ((rc = c->hip(dalloc(&c->adj_params, 2 * c->hpol.size()), "alloc adjoint"))) return rc;
+	if ((rc = c->hip(dalloc(&c->adj_params, (size_t)kStatCopies * adj_param_stride((int)c->hpol.size())), "alloc adjoint")))
+		return rc;
 	if ((rc = c->hip(dalloc(&c->dloss, 1), "alloc loss"))) return rc;
 	if ((rc = c->hip(hipMemcpyAsync(c->targets, targets, (size_t)T * n * sizeof(float2), hipMemcpyHostToDevice,
 	                                c->stream), "targets"))) return rc;
@@ -1790,7 +1794,8 @@ int umans_adjoint_backward(umans_ctx* c, float* d_params, float* d_pos0, float* 
 	float2* av2 = c->adj_buf + 3 * n;
 	float2* apref = c->adj_buf + 4 * n;
 	fill_f(c, c->adj_buf, 2LL * 5 * n);
-	fill_f(c, c->adj_params, 2LL * npol);
+	const int pstride = adj_param_stride(npol);
+	fill_f(c, c->adj_params, (long long)kStatCopies * pstride);
 	for (int seg = nseg - 1; seg >= 0; --seg) {
 		const int t0 = seg * ck, t1 = std::min(t0 + ck, T);
 		if ((rc = c->hip(copy_state(c->X.s, c->ckpts[seg].s, n, c->stream), "restore checkpoint"))) return rc;
@@ -1821,6 +1826,7 @@ int umans_adjoint_backward(umans_ctx* c, float* d_params, float* d_pos0, float* 
 			A.adj_vel_in = av2;
 			A.adj_pref = apref;
 			A.adj_params = c->adj_params;
+			A.param_stride = pstride;
 			A.param_kind = c->adj_param_kind;
 			hipLaunchKernelGGL(k_step_adjoint, dim3(ceil_div(n, 128)), dim3(128), 0, c->stream, A);
 			hipLaunchKernelGGL(k_pref_adjoint, dim3(ceil_div(n, 256)), dim3(256), 0, c->stream, c->seg_states[s].s, n,
@@ -1830,13 +1836,19 @@ int umans_adjoint_backward(umans_ctx* c, float* d_params, float* d_pos0, float* 
 		}
 	}
 	if ((rc = c->hip(hipGetLastError(), "adjoint launch"))) return rc;
-	std::vector<float> hp(2 * npol);
-	if ((rc = c->hip(hipMemcpyAsync(hp.data(), c->adj_params, hp.size() * sizeof(float), hipMemcpyDeviceToHost, c->stream), "adjoint"))) return rc;
+	std::vector<float> hc((size_t)kStatCopies * pstride);
+	if ((rc = c->hip(hipMemcpyAsync(hc.data(), c->adj_params, hc.size() * sizeof(float), hipMemcpyDeviceToHost, c->stream), "adjoint"))) return rc;
 	if (d_pos0 && (rc = c->hip(hipMemcpyAsync(d_pos0, ap, (size_t)n * sizeof(float2), hipMemcpyDeviceToHost, c->stream), "adjoint"))) return rc;
 	if (d_vel0 && (rc = c->hip(hipMemcpyAsync(d_vel0, av, (size_t)n * sizeof(float2), hipMemcpyDeviceToHost, c->stream), "adjoint"))) return rc;
 	if ((rc = c->hip(copy_state(c->X.s, c->B.s, n, c->stream), "restore"))) return rc;
 	if ((rc = c->hip(hipStreamSynchronize(c->stream), "adjoint"))) return rc;
-	if (d_params) std::memcpy(d_params, hp.data(), hp.size() * sizeof(float));
+	if (d_params) {
+		for (int i = 0; i < 2 * npol; ++i) {
+			float sum = 0.0f;
+			for (int cp = 0; cp < kStatCopies; ++cp) sum += hc[(size_t)cp * pstride + i];
+			d_params[i] = sum;
+		}
+	}
 	return UMANS_OK;
 }
 
