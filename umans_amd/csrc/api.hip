@@ -165,7 +165,7 @@ struct umans_ctx {
 	unsigned* cell_count = nullptr;           // per (class slot, cell)
 	unsigned* cell_start = nullptr;           // class-major exclusive scan of cell_count
 	unsigned* cell_first = nullptr;           // per cell over all slots: the search's ranges
-	unsigned* scan_sums = nullptr;
+	unsigned* scan_sums = nullptr;          // single-pass scan: [0] ticket, [1 + t] tile t status
 	unsigned* cell_of = nullptr;
 	int* class_list = nullptr;                // sorted index of each class-list entry
 	unsigned long long* long_key = nullptr;   // long lists: kLongCap x cap collection buffers
@@ -187,6 +187,9 @@ struct umans_ctx {
 	GridInfo* dgrids = nullptr;
 	int* dbbox = nullptr;
 	bool any_infinite = false;
+	// the grids in dgrids match dworlds and the cell reservations (fixed grids only: the
+	// grids of infinite worlds follow their bounding boxes every step)
+	bool grids_valid = false;
 	unsigned long long* dstats = nullptr;   // kStatCopies x kStatLine, see StepArgs::stats
 	float4* d_edges = nullptr;                // obstacle edges (single world)
 	int n_edges = 0;
@@ -502,6 +505,10 @@ int check_world(const umans_world_desc& d) {
 // Per-world cell reservations and WorldInfo records.
 int setup_worlds(umans_ctx* c, const std::vector<int>& per_world) {
 	const int nw = (int)c->worlds.size();
+	// new grids: the next step sets them up, eagerly (the captured step assumes them)
+	c->grids_valid = false;
+	c->keys_ready = false;
+	c->drop_graph();
 	std::vector<WorldInfo> wi(nw);
 	long long off = 0;
 	c->any_infinite = false;
@@ -547,6 +554,9 @@ int setup_worlds(umans_ctx* c, const std::vector<int>& per_world) {
 		if ((rc = c->hip(dalloc(&c->cell_start, cells), "alloc grid"))) return rc;
 		if ((rc = c->hip(dalloc(&c->cell_first, c->total_cells + 1), "alloc grid"))) return rc;
 		if ((rc = c->hip(dalloc(&c->scan_sums, ceil_div(cells, kScanTile) + 1), "alloc grid"))) return rc;
+		// the scan's ticket and statuses start at zero (each step's reorder clears them)
+		if ((rc = c->hip(hipMemsetAsync(c->scan_sums, 0, (ceil_div(cells, kScanTile) + 1) * sizeof(unsigned), c->stream),
+		                 "clear scan"))) return rc;
 		c->cell_alloc = cells;
 		c->counts_zero_n = -1;
 	}
@@ -707,22 +717,29 @@ int enqueue_step(umans_ctx* c, bool events, const State* ysave = nullptr, unsign
 		hipLaunchKernelGGL(k_bbox, dim3(ceil_div(n, 256)), dim3(256), 0, st, c->X.s.pos, c->X.s.meta, c->dworlds, n,
 		                   c->dbbox);
 	}
-	constexpr int stat_words = 2 * kStatCopies * kStatLine;
-	hipLaunchKernelGGL(k_grid_setup, dim3(ceil_div(std::max(std::max(nw, stat_words), 2 * NUM_CLS), 64)), dim3(64), 0, st,
-	                   c->dworlds, c->dbbox, c->dgrids, nw, (unsigned*)c->dstats, stat_words, c->ovf_count, 2 * NUM_CLS);
+	// fixed grids are set up by the first step after a change; a captured step assumes them
+	if (c->any_infinite || !c->grids_valid) {
+		hipLaunchKernelGGL(k_grid_setup, dim3(ceil_div(nw, 64)), dim3(64), 0, st, c->dworlds, c->dbbox, c->dgrids, nw,
+		                   nullptr, 0, nullptr, 0);
+		c->grids_valid = !c->any_infinite;
+	}
 	if (!have_keys)
 		hipLaunchKernelGGL(k_count, dim3(ceil_div(n, 256)), dim3(256), 0, st, c->X.s.pos, c->X.s.meta, n, c->dworlds,
 		                   c->dgrids, c->dpol, c->d_cls_slot, c->total_cells, c->cell_count, c->cell_of);
+	// single-pass scan; it also clears the step's counters. Its ticket and statuses are
+	// cleared by each step's reorder, and here at the first step of a call.
 	const int nb = ceil_div(scan_n, kScanTile);
-	hipLaunchKernelGGL(k_scan_local, dim3(nb), dim3(kScanThreads), 0, st, c->cell_count, c->cell_start, c->scan_sums, scan_n);
-	hipLaunchKernelGGL(k_scan_sums, dim3(1), dim3(kScanThreads), 0, st, c->scan_sums, nb);
-	hipLaunchKernelGGL(k_scan_add, dim3(nb), dim3(kScanThreads), 0, st, c->cell_start, c->scan_sums, scan_n);
+	if (!have_keys)
+		hipLaunchKernelGGL(k_fill, dim3(ceil_div(nb + 1, 256)), dim3(256), 0, st, c->scan_sums, nb + 1, 0u);
+	constexpr int stat_words = 2 * kStatCopies * kStatLine;
+	hipLaunchKernelGGL(k_scan_lookback, dim3(nb), dim3(kScanThreads), 0, st, c->cell_count, c->cell_start, c->scan_sums,
+	                   c->scan_sums + 1, scan_n, nb, (unsigned*)c->dstats, stat_words, c->ovf_count, 2 * NUM_CLS);
 	// one class slot: the per-cell starts are the class-major ones; with several the
 	// reorder also writes them
 	const unsigned* cell_first = c->nslots > 1 ? c->cell_first : c->cell_start;
-	const int nr = c->nslots > 1 ? std::max(n, c->total_cells + 1) : n;
+	const int nr = std::max(c->nslots > 1 ? std::max(n, c->total_cells + 1) : n, nb + 1);
 	hipLaunchKernelGGL(k_reorder, dim3(ceil_div(nr, 256)), dim3(256), 0, st, c->X.s, Ys, n, c->cell_of, c->cell_count,
-	                   c->cell_start, c->nslots, c->total_cells, c->cell_first, c->class_list);
+	                   c->cell_start, c->nslots, c->total_cells, c->cell_first, c->class_list, c->scan_sums, nb + 1);
 
 	StepArgs a{};
 	a.in = Ys;

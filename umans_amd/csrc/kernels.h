@@ -7,9 +7,9 @@
 //   k_bbox/k_grid  infinite worlds only: bounding box -> uniform cell grid
 //   k_count        (class, cell) key of every agent, counted per key (atomic, no
 //                  return)
-//   k_scan_*       class-major exclusive scan of the counts: each class's agents in
-//                  cell order (the class kernels' work lists), and per cell over all
-//                  classes by differences (the ranges the neighbour search walks)
+//   k_scan_lookback class-major exclusive scan of the counts (one pass): each class's
+//                  agents in cell order (the class kernels' work lists), and per cell
+//                  over all classes by differences (the ranges the neighbour search walks)
 //   k_reorder      scatter the state into cell order, classes interleaved within a
 //                  cell (the rank inside a key taken by counting the key back down to
 //                  zero), and fill the class lists; preferred velocity
@@ -183,9 +183,9 @@ UD void count_next(const StepArgs& A, const WorldInfo& wi, int wix, int k, float
 	atomicAdd(&A.next_count[c], 1u);
 }
 
-// ---------------------------------------------------------------- scan (3 passes)
+// ---------------------------------------------------------------- scan (one pass)
 constexpr int kScanThreads = 256;
-constexpr int kScanItems = 4;
+constexpr int kScanItems = 16;   // 4096 counts per tile: few tiles for the look-back to walk
 constexpr int kScanTile = kScanThreads * kScanItems;
 
 UD unsigned block_exclusive_scan(unsigned v, unsigned* sh, unsigned& total) {
@@ -208,41 +208,103 @@ UD unsigned block_exclusive_scan(unsigned v, unsigned* sh, unsigned& total) {
 	return wofs + x - v;
 }
 
-__global__ __launch_bounds__(kScanThreads) void k_scan_local(const unsigned* in, unsigned* out, unsigned* sums, int n) {
+// Single-pass exclusive scan (decoupled look-back, after Merrill & Garland, "Single-pass
+// Parallel Prefix Scan with Decoupled Look-back", 2016). Each block takes a ticket in the
+// order blocks start and scans tile `ticket`; it publishes the tile's total, then adds
+// the totals of the tiles before it, walking back until one has published its inclusive
+// prefix. A tile waits only on tiles with lower tickets, whose blocks have started and
+// publish their totals without waiting on anyone, so the walk always ends (tile 0
+// publishes its inclusive prefix at once). status[t] = flag | value (flag kScanAgg: the
+// tile's total, kScanInc: its inclusive prefix; values below 2^30); `ticket` and
+// status[0..nb) must be zero at launch. Also clears the step's counters
+// (zero0[0..n0), zero1[0..n1)).
+constexpr unsigned kScanAgg = 1u << 30, kScanInc = 2u << 30, kScanValMask = kScanAgg - 1u;
+UD unsigned ld_status(unsigned* p) {
+	return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+UD void st_status(unsigned* p, unsigned v) {
+	__hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__global__ __launch_bounds__(kScanThreads) void k_scan_lookback(const unsigned* in, unsigned* out, unsigned* ticket,
+                                                                unsigned* status, int n, int nb, unsigned* zero0,
+                                                                int n0, unsigned* zero1, int n1) {
 	__shared__ unsigned sh[kScanThreads / 64];
-	const int base = blockIdx.x * kScanTile + threadIdx.x * kScanItems;
+	__shared__ unsigned s_tile, s_prefix;
+	for (int i = blockIdx.x * kScanThreads + threadIdx.x; i < n0; i += gridDim.x * kScanThreads) zero0[i] = 0u;
+	for (int i = blockIdx.x * kScanThreads + threadIdx.x; i < n1; i += gridDim.x * kScanThreads) zero1[i] = 0u;
+	if (threadIdx.x == 0) s_tile = atomicAdd(ticket, 1u);
+	__syncthreads();
+	const int tile = (int)s_tile;
+	if (tile >= nb) return;          // one ticket per block: not reached
+	const int base = tile * kScanTile + threadIdx.x * kScanItems;
 	unsigned v[kScanItems];
-	unsigned s = 0;
-	for (int i = 0; i < kScanItems; ++i) {
-		v[i] = (base + i < n) ? in[base + i] : 0u;
-		s += v[i];
+	const bool whole = base + kScanItems <= n;
+	if (whole) {
+		// 64-byte aligned: four 16-byte loads
+		const uint4* in4 = reinterpret_cast<const uint4*>(in + base);
+#pragma unroll
+		for (int q = 0; q < kScanItems / 4; ++q) {
+			const uint4 x = in4[q];
+			v[4 * q] = x.x; v[4 * q + 1] = x.y; v[4 * q + 2] = x.z; v[4 * q + 3] = x.w;
+		}
+	} else {
+#pragma unroll
+		for (int i = 0; i < kScanItems; ++i) v[i] = (base + i < n) ? in[base + i] : 0u;
 	}
+	unsigned s = 0;
+#pragma unroll
+	for (int i = 0; i < kScanItems; ++i) s += v[i];
 	unsigned total;
 	unsigned run = block_exclusive_scan(s, sh, total);
+	// the first wave walks back over windows of 64 predecessors at once; the wait is
+	// bounded (~2 s) so that the kernel can never hang, whatever the state
+	if (threadIdx.x < 64) {
+		const int lane = threadIdx.x;
+		unsigned prefix = 0;
+		if (tile == 0) {
+			if (lane == 0) st_status(&status[0], kScanInc | total);
+		} else {
+			if (lane == 0) st_status(&status[tile], kScanAgg | total);
+			for (int top = tile - 1, spins = 0;;) {
+				const int j = top - lane;
+				// before tile 0: an inclusive prefix of zero
+				const unsigned w = j >= 0 ? ld_status(&status[j]) : kScanInc;
+				const unsigned flag = w & ~kScanValMask;
+				const unsigned long long inc = __ballot(flag == kScanInc);
+				const unsigned long long unpublished = __ballot(flag == 0u);
+				const int first_inc = inc ? __builtin_ctzll(inc) : 64;
+				const unsigned long long needed = first_inc >= 63 ? ~0ull : ((2ull << first_inc) - 1ull);
+				if ((unpublished & needed) && ++spins < (1 << 26)) {
+					__builtin_amdgcn_s_sleep(1);
+					continue;
+				}
+				// the window's values up to and including the nearest inclusive prefix
+				unsigned val = lane <= first_inc ? (w & kScanValMask) : 0u;
+				for (int o = 32; o > 0; o >>= 1) val += __shfl_xor(val, o);
+				prefix += val;
+				if (first_inc < 64) break;
+				top -= 64;
+			}
+			if (lane == 0) st_status(&status[tile], kScanInc | (prefix + total));
+		}
+		if (lane == 0) s_prefix = prefix;
+	}
+	__syncthreads();
+	run += s_prefix;
+#pragma unroll
 	for (int i = 0; i < kScanItems; ++i) {
-		if (base + i < n) out[base + i] = run;
-		run += v[i];
+		const unsigned x = v[i];
+		v[i] = run;
+		run += x;
 	}
-	if (threadIdx.x == 0) sums[blockIdx.x] = total;
-}
-__global__ __launch_bounds__(kScanThreads) void k_scan_sums(unsigned* sums, int nb) {
-	__shared__ unsigned sh[kScanThreads / 64];
-	unsigned carry = 0;
-	for (int b0 = 0; b0 < nb; b0 += kScanThreads) {
-		const int i = b0 + threadIdx.x;
-		const unsigned v = i < nb ? sums[i] : 0u;
-		unsigned total;
-		const unsigned ex = block_exclusive_scan(v, sh, total);
-		if (i < nb) sums[i] = carry + ex;
-		carry += total;
-		__syncthreads();
+	if (whole) {
+		uint4* out4 = reinterpret_cast<uint4*>(out + base);
+#pragma unroll
+		for (int q = 0; q < kScanItems / 4; ++q) out4[q] = make_uint4(v[4 * q], v[4 * q + 1], v[4 * q + 2], v[4 * q + 3]);
+	} else {
+		for (int i = 0; i < kScanItems; ++i)
+			if (base + i < n) out[base + i] = v[i];
 	}
-}
-__global__ __launch_bounds__(kScanThreads) void k_scan_add(unsigned* out, const unsigned* sums, int n) {
-	const int base = blockIdx.x * kScanTile;
-	const unsigned add = sums[blockIdx.x];
-	for (int i = threadIdx.x; i < kScanTile; i += kScanThreads)
-		if (base + i < n) out[base + i] += add;
 }
 
 // --------------------------------------------------------------------- reorder
@@ -261,10 +323,14 @@ UD unsigned cell_first_of(const unsigned* cs, unsigned c, int nslots, unsigned s
 // canonical). It goes to d = cell_first(cell) + (agents of the lower slots in that cell:
 // differences of class_start) + r and is entry class_start[key] + r of the class lists.
 // Thread i <= slot_cells (with several slots): cell_first[i] for the neighbour search.
+// Threads i <= n_scan_state also clear the single-pass scan's ticket and tile statuses
+// for the next step (the scan has finished: it runs before, on the same stream).
 __global__ __launch_bounds__(256) void k_reorder(State src, State dst, int n, const unsigned* cell_of,
                                                 unsigned* cell_count, const unsigned* class_start, int nslots,
-                                                int slot_cells, unsigned* cell_first, int* class_list) {
+                                                int slot_cells, unsigned* cell_first, int* class_list,
+                                                unsigned* scan_state, int n_scan_state) {
 	const int k = blockIdx.x * blockDim.x + threadIdx.x;
+	if (k < n_scan_state) scan_state[k] = 0u;
 	const unsigned sc = (unsigned)slot_cells;
 	if (nslots > 1 && k <= slot_cells) cell_first[k] = cell_first_of(class_start, (unsigned)k, nslots, sc);
 	if (k >= n) return;
