@@ -185,7 +185,7 @@ UD void count_next(const StepArgs& A, const WorldInfo& wi, int wix, int k, float
 
 // ---------------------------------------------------------------- scan (one pass)
 constexpr int kScanThreads = 256;
-constexpr int kScanItems = 32;   // 8192 counts per tile: few tiles for the look-back to walk
+constexpr int kScanItems = 64;   // 16384 counts per tile: few tiles for the look-back to walk
 constexpr int kScanTile = kScanThreads * kScanItems;
 
 UD unsigned block_exclusive_scan(unsigned v, unsigned* sh, unsigned& total) {
