@@ -169,6 +169,26 @@ def measured_traffic(kernel="k_step"):
     return None
 
 
+# VALU issue peak: 256 CUs x 4 SIMDs, one wave64 VALU instruction per SIMD per 4 clocks
+# (16 lanes wide, packed FP32 not counted) at 2.4 GHz, in G wave-instructions per second
+VALU_ISSUE_PEAK_G = 256 * 4 * 2.4 / 4
+
+
+def measured_valu_per_step():
+    """VALU busy quad-cycles of one mixed-crowd step (sum over the class kernels of waves x
+    SQ_ACTIVE_INST_VALU per wave) from the committed rocprofv3 SQ summary, or None."""
+    path = os.path.join(ROOT, "profiles", "k_step_mixed1M_sq_counters.md")
+    total = 0.0
+    try:
+        for line in open(path):
+            cells = [c.strip() for c in line.split("|")]
+            if len(cells) > 12 and cells[1].startswith("k_step,"):
+                total += float(cells[2]) * float(cells[11])
+    except (OSError, ValueError):
+        return None
+    return total or None
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -243,6 +263,13 @@ def main():
     grid_s = prof["grid_ms"] / 1e3 / max(prof["steps"], 1)
     achieved = RL.STEP_BYTES * n / step_s / 1e9
     valu = flops_per_agent * n / step_s / 1e12
+    # the bound the step kernels meet: VALU issue (quad-cycles per step from the counters)
+    valu_issue = None
+    qc = measured_valu_per_step() if sc.name.startswith("mixed") else None
+    if qc:
+        g = qc / step_s / 1e9
+        valu_issue = {"achieved": g, "peak": VALU_ISSUE_PEAK_G, "unit": "G wave-instructions/s",
+                      "frac": g / VALU_ISSUE_PEAK_G, "quad_cycles_per_step": qc}
     out = {
         "metric": ("agent-steps/s at 1M agents" if sc.name.startswith("mixed") else
                    "batched agent-steps/s" if batched else "agent-steps/s"),
@@ -271,7 +298,8 @@ def main():
                      "traffic": measured_traffic() if sc.name.startswith("mixed") else None,
                      "bytes_per_agent": RL.STEP_BYTES, "avg_launch_ms": step_s * 1e3,
                      "valu": {"achieved": valu, "peak": RL.FP32_PEAK_TFLOPS, "unit": "TFLOP/s",
-                              "frac": valu / RL.FP32_PEAK_TFLOPS, "flops_per_agent": flops_per_agent}},
+                              "frac": valu / RL.FP32_PEAK_TFLOPS, "flops_per_agent": flops_per_agent},
+                     "valu_issue": valu_issue},
         "phases_ms": {"grid": grid_s * 1e3, "step": step_s * 1e3},
         "neighbor_overflow_agents": st["overflow_agents"],
     }
