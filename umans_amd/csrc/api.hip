@@ -679,8 +679,9 @@ int csv_after_step(umans_ctx* c) {
 	return UMANS_OK;
 }
 
-// fuse_next: the step kernels also do the next step's k_count (fixed grids only; the
-// caller guarantees nothing changes X before that step).
+// fuse_next: this step prepares the next one (the caller guarantees nothing changes X
+// before it): on fixed grids the step kernels do its k_count; with infinite worlds the
+// reorder fills the bounding boxes its grids are set up from (no k_bbox pass).
 // Class kernels in launch order (the first runs on the library stream, the others on
 // forked ones). Measured on the 1M-agent mixed crowd: ORCA (the fewest resident waves
 // per SIMD), then PowerLaw sampling, then SocialForces gives the shortest step phase
@@ -704,15 +705,19 @@ int enqueue_step(umans_ctx* c, bool events, const State* ysave = nullptr, unsign
 		e0 = c->next_event(); e1 = c->next_event(); e2 = c->next_event();
 		if (int rc = c->hip(hipEventRecord(e0, st), "profile event")) return rc;
 	}
+	// prepared by the previous step of this call: the counts (fixed grids) or the
+	// bounding boxes (infinite worlds)
 	const bool have_keys = c->keys_ready && !c->any_infinite;
+	const bool have_box = c->keys_ready && c->any_infinite;
 	const bool fuse = fuse_next && !c->any_infinite;
+	const bool fuse_box = fuse_next && c->any_infinite;
 	c->keys_ready = false;
 	// the scan leaves the counters zero; clear them here only when the key space changed
 	// (or a step did not complete, or counted for a step that did not come)
 	if (!have_keys && c->counts_zero_n != scan_n)
 		hipLaunchKernelGGL(k_fill, dim3(ceil_div(c->cell_alloc, 256)), dim3(256), 0, st, c->cell_count, c->cell_alloc, 0u);
 	c->counts_zero_n = -1;
-	if (c->any_infinite) {
+	if (c->any_infinite && !have_box) {
 		hipLaunchKernelGGL(k_bbox_init, dim3(ceil_div(nw * kBoxCopies, 256)), dim3(256), 0, st, c->dbbox, nw);
 		hipLaunchKernelGGL(k_bbox, dim3(ceil_div(n, 256)), dim3(256), 0, st, c->X.s.pos, c->X.s.meta, c->dworlds, n,
 		                   c->dbbox);
@@ -720,7 +725,7 @@ int enqueue_step(umans_ctx* c, bool events, const State* ysave = nullptr, unsign
 	// fixed grids are set up by the first step after a change; a captured step assumes them
 	if (c->any_infinite || !c->grids_valid) {
 		hipLaunchKernelGGL(k_grid_setup, dim3(ceil_div(nw, 64)), dim3(64), 0, st, c->dworlds, c->dbbox, c->dgrids, nw,
-		                   nullptr, 0, nullptr, 0);
+		                   nullptr, 0, nullptr, 0, fuse_box ? 1 : 0);
 		c->grids_valid = !c->any_infinite;
 	}
 	if (!have_keys)
@@ -739,7 +744,8 @@ int enqueue_step(umans_ctx* c, bool events, const State* ysave = nullptr, unsign
 	const unsigned* cell_first = c->nslots > 1 ? c->cell_first : c->cell_start;
 	const int nr = std::max(c->nslots > 1 ? std::max(n, c->total_cells + 1) : n, nb + 1);
 	hipLaunchKernelGGL(k_reorder, dim3(ceil_div(nr, 256)), dim3(256), 0, st, c->X.s, Ys, n, c->cell_of, c->cell_count,
-	                   c->cell_start, c->nslots, c->total_cells, c->cell_first, c->class_list, c->scan_sums, nb + 1);
+	                   c->cell_start, c->nslots, c->total_cells, c->cell_first, c->class_list, c->scan_sums, nb + 1,
+	                   c->dworlds, fuse_box ? c->dbbox : nullptr);
 
 	StepArgs a{};
 	a.in = Ys;
@@ -860,7 +866,7 @@ int enqueue_step(umans_ctx* c, bool events, const State* ysave = nullptr, unsign
 	const int rc = c->hip(hipGetLastError(), "step launch");
 	if (!rc) {
 		c->counts_zero_n = fuse ? -1 : scan_n;
-		c->keys_ready = fuse;
+		c->keys_ready = fuse_next;
 	}
 	return rc;
 }
@@ -1129,10 +1135,11 @@ int remove_reached(umans_ctx* c) {
 
 int run_steps_inner(umans_ctx* c, int nsteps);
 
-// Within one call each step also counts the agents for the next one (fixed grids), so
-// from the second step on k_count is skipped; a population change in between makes the
-// next step count again. A captured graph is the steady-state step (no k_count): it is
-// replayed only on steps whose keys are ready.
+// Within one call each step also counts the agents for the next one (fixed grids) or
+// fills the bounding boxes for its grids (infinite worlds), so from the second step on
+// k_count (k_bbox) is skipped; a population change in between makes the next step count
+// again. A captured graph is the steady-state step: it is replayed only on steps whose
+// keys (boxes) are ready.
 int run_steps(umans_ctx* c, int nsteps) {
 	const int rc0 = run_steps_inner(c, nsteps);
 	c->keys_ready = false;     // keys never carry over to other entry points
@@ -1141,7 +1148,6 @@ int run_steps(umans_ctx* c, int nsteps) {
 
 int run_steps_inner(umans_ctx* c, int nsteps) {
 	const bool single = c->worlds.size() == 1;
-	const bool fuse = !c->any_infinite;
 	c->keys_ready = false;
 	for (int s = 0; s < nsteps; ++s) {
 		if (single) {
@@ -1151,12 +1157,12 @@ int run_steps_inner(umans_ctx* c, int nsteps) {
 		}
 		if (c->n > 0) {
 			int rc;
-			if (c->use_graph && !c->profiling && (!fuse || c->keys_ready)) {
-				if (!c->graph_exec && (rc = capture_graph(c, fuse))) return rc;
+			if (c->use_graph && !c->profiling && c->keys_ready) {
+				if (!c->graph_exec && (rc = capture_graph(c, true))) return rc;
 				if ((rc = c->hip(hipGraphLaunch(c->graph_exec, c->stream), "graph launch"))) return rc;
-				c->keys_ready = fuse;
-				c->counts_zero_n = fuse ? -1 : c->counts_zero_n;
-			} else if ((rc = enqueue_step(c, c->profiling != 0, nullptr, nullptr, nullptr, fuse))) {
+				c->keys_ready = true;
+				if (!c->any_infinite) c->counts_zero_n = -1;
+			} else if ((rc = enqueue_step(c, c->profiling != 0, nullptr, nullptr, nullptr, true))) {
 				return rc;
 			}
 		}

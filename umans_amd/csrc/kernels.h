@@ -62,11 +62,11 @@ __global__ void k_bbox_init(int* bbox, int nworlds) {
 		b[2] = (int)0x80000000; b[3] = (int)0x80000000;
 	}
 }
-// Bounding boxes of the infinite worlds. Agents of a batch are grouped by world, so a
-// wave whose lanes share one world reduces first and issues one atomic per bound.
-__global__ __launch_bounds__(256) void k_bbox(const float2* pos, const int4* meta, const WorldInfo* worlds,
-                                             int n, int* bbox) {
-	const int i = blockIdx.x * blockDim.x + threadIdx.x;
+// Bounding boxes of the infinite worlds: agent i's position added to its world's box
+// (every lane of the wave takes part; lanes with i >= n add nothing). Agents of a batch
+// are grouped by world, so a wave whose lanes share one world reduces first and issues
+// one atomic per bound.
+UD void bbox_add(const float2* pos, const int4* meta, const WorldInfo* worlds, int n, int i, int* bbox) {
 	int w = -1;
 	int mnx = 0x7fffffff, mny = 0x7fffffff, mxx = (int)0x80000000, mxy = (int)0x80000000;
 	if (i < n) {
@@ -96,12 +96,17 @@ __global__ __launch_bounds__(256) void k_bbox(const float2* pos, const int4* met
 		atomicMax(&b[2], mxx); atomicMax(&b[3], mxy);
 	}
 }
+__global__ __launch_bounds__(256) void k_bbox(const float2* pos, const int4* meta, const WorldInfo* worlds,
+                                             int n, int* bbox) {
+	bbox_add(pos, meta, worlds, n, blockIdx.x * blockDim.x + threadIdx.x, bbox);
+}
 // One thread per world. Infinite worlds: a grid over the bounding box with cells of
 // at least cell_size metres and at most cell_cap cells. Toric worlds: the world
 // rectangle centred on the origin with cells at least cell_size wide.
 // Also clears the step's counters (zero0[0..n0), zero1[0..n1)), saving their launches.
-__global__ void k_grid_setup(const WorldInfo* worlds, const int* bbox, GridInfo* grids, int nworlds, unsigned* zero0,
-                             int n0, unsigned* zero1, int n1) {
+// reset_box: the boxes, once read, start again empty (the next reorder fills them).
+__global__ void k_grid_setup(const WorldInfo* worlds, int* bbox, GridInfo* grids, int nworlds, unsigned* zero0,
+                             int n0, unsigned* zero1, int n1, int reset_box) {
 	const int wi = blockIdx.x * blockDim.x + threadIdx.x;
 	if (wi < n0) zero0[wi] = 0u;
 	if (wi < n1) zero1[wi] = 0u;
@@ -121,9 +126,13 @@ __global__ void k_grid_setup(const WorldInfo* worlds, const int* bbox, GridInfo*
 	} else {
 		int b[4] = {0x7fffffff, 0x7fffffff, (int)0x80000000, (int)0x80000000};
 		for (int cp = 0; cp < kBoxCopies; ++cp) {
-			const int* bc = bbox + ((size_t)wi * kBoxCopies + cp) * kBoxLine;
+			int* bc = bbox + ((size_t)wi * kBoxCopies + cp) * kBoxLine;
 			b[0] = min(b[0], bc[0]); b[1] = min(b[1], bc[1]);
 			b[2] = max(b[2], bc[2]); b[3] = max(b[3], bc[3]);
+			if (reset_box) {
+				bc[0] = 0x7fffffff; bc[1] = 0x7fffffff;
+				bc[2] = (int)0x80000000; bc[3] = (int)0x80000000;
+			}
 		}
 		const bool empty = b[0] > b[2];
 		const float x0 = empty ? 0.0f : from_ordered_key(b[0]);
@@ -324,13 +333,19 @@ UD unsigned cell_first_of(const unsigned* cs, unsigned c, int nslots, unsigned s
 // differences of class_start) + r and is entry class_start[key] + r of the class lists.
 // Thread i <= slot_cells (with several slots): cell_first[i] for the neighbour search.
 // Threads i <= n_scan_state also clear the single-pass scan's ticket and tile statuses
-// for the next step (the scan has finished: it runs before, on the same stream).
+// for the next step (the scan has finished: it runs before, on the same stream). With
+// bbox, the agents' positions (those this step starts from) are added to the boxes of
+// their infinite worlds, from which the next step's grids are set up: one step old, they
+// may leave an agent outside its grid, which the clamped binning puts in an edge cell,
+// so lists and results do not depend on it.
 __global__ __launch_bounds__(256) void k_reorder(State src, State dst, int n, const unsigned* cell_of,
                                                 unsigned* cell_count, const unsigned* class_start, int nslots,
                                                 int slot_cells, unsigned* cell_first, int* class_list,
-                                                unsigned* scan_state, int n_scan_state) {
+                                                unsigned* scan_state, int n_scan_state, const WorldInfo* worlds,
+                                                int* bbox) {
 	const int k = blockIdx.x * blockDim.x + threadIdx.x;
 	if (k < n_scan_state) scan_state[k] = 0u;
+	if (bbox) bbox_add(src.pos, src.meta, worlds, n, k, bbox);
 	const unsigned sc = (unsigned)slot_cells;
 	if (nslots > 1 && k <= slot_cells) cell_first[k] = cell_first_of(class_start, (unsigned)k, nslots, sc);
 	if (k >= n) return;
