@@ -126,12 +126,14 @@ def test_autotune_keeps_state_and_results():
     W.close()
 
 
-@pytest.mark.parametrize("kind", ["mixed", "toric_batch", "dynamic"])
+@pytest.mark.parametrize("kind", ["mixed", "toric_batch", "dynamic", "infinite_batch"])
 def test_steps_in_one_call_equal_single_steps(kind):
     """Within one step call every step also counts the agents for the next one (fixed
-    grids), and the next step skips its own count; a call of K steps gives exactly the
-    results of K calls of one step (which all count afresh), with scheduled insertions
-    and remove_at_goal removals in between for "dynamic"."""
+    grids), and the next step skips its own count; with infinite worlds it fills the
+    bounding boxes the next step's grids come from (one step old, where a single-step
+    call measures them afresh). A call of K steps gives exactly the results of K calls of
+    one step, with scheduled insertions and remove_at_goal removals in between for
+    "dynamic"."""
     import os
     from goldens import GOLDEN
     if kind == "dynamic":
@@ -139,6 +141,11 @@ def test_steps_in_one_call_equal_single_steps(kind):
         mk = lambda: gpu.World.from_scenario_file(path)
         agents = None
         steps = 40
+    elif kind == "infinite_batch":
+        b = small_sweep()
+        mk = lambda: gpu.World(0, 0, 0, 0.1, b.policies, max_neighbors=b.max_neighbors, batch=b.worlds)
+        agents = b.agents
+        steps = 12
     elif kind == "mixed":
         sc = S.mixed(n=20_000, size=283.0)
         mk = lambda: gpu.World(sc.world_type, sc.width, sc.height, sc.dt, sc.policies, sc.max_neighbors)
