@@ -185,6 +185,7 @@ struct umans_ctx {
 	int total_cells = 0;         // sum of the worlds' cell_cap
 	WorldInfo* dworlds = nullptr;
 	GridInfo* dgrids = nullptr;
+	GridInfo* dgrids_next = nullptr;     // infinite worlds: the next step's grids, set up after the reorder
 	int* dbbox = nullptr;
 	bool any_infinite = false;
 	// the grids in dgrids match dworlds and the cell reservations (fixed grids only: the
@@ -707,9 +708,8 @@ int enqueue_step(umans_ctx* c, bool events, const State* ysave = nullptr, unsign
 	}
 	// prepared by the previous step of this call: the counts (fixed grids) or the
 	// bounding boxes (infinite worlds)
-	const bool have_keys = c->keys_ready && !c->any_infinite;
-	const bool have_box = c->keys_ready && c->any_infinite;
-	const bool fuse = fuse_next && !c->any_infinite;
+	const bool have_keys = c->keys_ready;
+	const bool fuse = fuse_next;
 	const bool fuse_box = fuse_next && c->any_infinite;
 	c->keys_ready = false;
 	// the scan leaves the counters zero; clear them here only when the key space changed
@@ -717,16 +717,20 @@ int enqueue_step(umans_ctx* c, bool events, const State* ysave = nullptr, unsign
 	if (!have_keys && c->counts_zero_n != scan_n)
 		hipLaunchKernelGGL(k_fill, dim3(ceil_div(c->cell_alloc, 256)), dim3(256), 0, st, c->cell_count, c->cell_alloc, 0u);
 	c->counts_zero_n = -1;
-	if (c->any_infinite && !have_box) {
+	if (c->any_infinite && have_keys) {
+		// the grids the previous step set up and counted this step's agents in
+		hipLaunchKernelGGL(k_copy_grids, dim3(ceil_div(nw, 64)), dim3(64), 0, st, c->dgrids_next, c->dgrids, nw);
+	} else if (c->any_infinite) {
 		hipLaunchKernelGGL(k_bbox_init, dim3(ceil_div(nw * kBoxCopies, 256)), dim3(256), 0, st, c->dbbox, nw);
 		hipLaunchKernelGGL(k_bbox, dim3(ceil_div(n, 256)), dim3(256), 0, st, c->X.s.pos, c->X.s.meta, c->dworlds, n,
 		                   c->dbbox);
-	}
-	// fixed grids are set up by the first step after a change; a captured step assumes them
-	if (c->any_infinite || !c->grids_valid) {
 		hipLaunchKernelGGL(k_grid_setup, dim3(ceil_div(nw, 64)), dim3(64), 0, st, c->dworlds, c->dbbox, c->dgrids, nw,
 		                   nullptr, 0, nullptr, 0, fuse_box ? 1 : 0);
-		c->grids_valid = !c->any_infinite;
+	} else if (!c->grids_valid) {
+		// fixed grids are set up by the first step after a change; a captured step assumes them
+		hipLaunchKernelGGL(k_grid_setup, dim3(ceil_div(nw, 64)), dim3(64), 0, st, c->dworlds, c->dbbox, c->dgrids, nw,
+		                   nullptr, 0, nullptr, 0, 0);
+		c->grids_valid = true;
 	}
 	if (!have_keys)
 		hipLaunchKernelGGL(k_count, dim3(ceil_div(n, 256)), dim3(256), 0, st, c->X.s.pos, c->X.s.meta, n, c->dworlds,
@@ -746,6 +750,11 @@ int enqueue_step(umans_ctx* c, bool events, const State* ysave = nullptr, unsign
 	hipLaunchKernelGGL(k_reorder, dim3(ceil_div(nr, 256)), dim3(256), 0, st, c->X.s, Ys, n, c->cell_of, c->cell_count,
 	                   c->cell_start, c->nslots, c->total_cells, c->cell_first, c->class_list, c->scan_sums, nb + 1,
 	                   c->dworlds, fuse_box ? c->dbbox : nullptr);
+	// infinite worlds: the next step's grids from the boxes the reorder just filled; the
+	// step kernels count their agents in them
+	if (fuse_box)
+		hipLaunchKernelGGL(k_grid_setup, dim3(ceil_div(nw, 64)), dim3(64), 0, st, c->dworlds, c->dbbox, c->dgrids_next,
+		                   nw, nullptr, 0, nullptr, 0, 1);
 
 	StepArgs a{};
 	a.in = Ys;
@@ -753,6 +762,7 @@ int enqueue_step(umans_ctx* c, bool events, const State* ysave = nullptr, unsign
 	a.pol = c->dpol;
 	a.worlds = c->dworlds;
 	a.grids = c->dgrids;
+	a.next_grids = c->any_infinite ? c->dgrids_next : c->dgrids;
 	a.cell_start = cell_first;
 	a.class_list = c->class_list;
 	a.base = 0;
@@ -1161,7 +1171,7 @@ int run_steps_inner(umans_ctx* c, int nsteps) {
 				if (!c->graph_exec && (rc = capture_graph(c, true))) return rc;
 				if ((rc = c->hip(hipGraphLaunch(c->graph_exec, c->stream), "graph launch"))) return rc;
 				c->keys_ready = true;
-				if (!c->any_infinite) c->counts_zero_n = -1;
+				c->counts_zero_n = -1;
 			} else if ((rc = enqueue_step(c, c->profiling != 0, nullptr, nullptr, nullptr, true))) {
 				return rc;
 			}
@@ -1234,7 +1244,7 @@ int create_impl(const umans_world_desc* descs, int nworlds, umans_ctx** out) {
 		streams_ok = hipStreamCreateWithFlags(&c->cls_stream[q], hipStreamNonBlocking) == hipSuccess &&
 		             hipEventCreateWithFlags(&c->join_ev[q], hipEventDisableTiming) == hipSuccess;
 	if (!streams_ok || hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess ||
-	    dalloc(&c->dgrids, nworlds) != hipSuccess || dalloc(&c->dworlds, nworlds) != hipSuccess ||
+	    dalloc(&c->dgrids, nworlds) != hipSuccess || dalloc(&c->dgrids_next, nworlds) != hipSuccess || dalloc(&c->dworlds, nworlds) != hipSuccess ||
 	    dalloc(&c->dbbox, (size_t)kBoxCopies * kBoxLine * nworlds) != hipSuccess ||
 	    dalloc(&c->dstats, (size_t)kStatCopies * kStatLine) != hipSuccess ||
 	    dalloc(&c->ovf_count, 2 * NUM_CLS) != hipSuccess || dalloc(&c->dpol, kMaxPolicies) != hipSuccess ||
@@ -1268,7 +1278,7 @@ void umans_destroy(umans_ctx* c) {
 	c->free_csv_buffers();
 	c->free_pop_buffers();
 	c->free_agents();
-	(void)hipFree(c->dgrids); (void)hipFree(c->dworlds); (void)hipFree(c->dbbox); (void)hipFree(c->dstats); (void)hipFree(c->dpol);
+	(void)hipFree(c->dgrids); (void)hipFree(c->dgrids_next); (void)hipFree(c->dworlds); (void)hipFree(c->dbbox); (void)hipFree(c->dstats); (void)hipFree(c->dpol);
 	(void)hipFree(c->d_edges); (void)hipFree(c->ovf_count);
 	(void)hipFree(c->d_cls_slot);
 	(void)hipFree(c->cell_count); (void)hipFree(c->cell_start); (void)hipFree(c->cell_first); (void)hipFree(c->scan_sums);

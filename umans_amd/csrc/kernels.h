@@ -96,6 +96,10 @@ UD void bbox_add(const float2* pos, const int4* meta, const WorldInfo* worlds, i
 		atomicMax(&b[2], mxx); atomicMax(&b[3], mxy);
 	}
 }
+__global__ void k_copy_grids(const GridInfo* src, GridInfo* dst, int nworlds) {
+	const int i = blockIdx.x * blockDim.x + threadIdx.x;
+	if (i < nworlds) dst[i] = src[i];
+}
 __global__ __launch_bounds__(256) void k_bbox(const float2* pos, const int4* meta, const WorldInfo* worlds,
                                              int n, int* bbox) {
 	bbox_add(pos, meta, worlds, n, blockIdx.x * blockDim.x + threadIdx.x, bbox);
@@ -185,7 +189,7 @@ __global__ __launch_bounds__(256) void k_count(const float2* pos, const int4* me
 // k_count's work for agent k of the next step, done by the step kernel that computed its
 // new position (same key: the class slot of the launch, the world's fixed grid).
 UD void count_next(const StepArgs& A, const WorldInfo& wi, int wix, int k, float2 p) {
-	const GridInfo g = A.grids[wix];
+	const GridInfo g = A.next_grids[wix];
 	const int c = A.next_key_base + wi.cell_offset + bin_axis(p.y, g.oy, g.invy, g.ncy) * g.ncx +
 	              bin_axis(p.x, g.ox, g.invx, g.ncx);
 	A.next_cell_of[k] = (unsigned)c;

@@ -90,6 +90,7 @@ struct StepArgs {
 	const DevPolicy* pol;
 	const WorldInfo* worlds;
 	const GridInfo* grids;
+	const GridInfo* next_grids;   // the next step's grids (count_next); infinite worlds: set up after the reorder
 	const unsigned* cell_start;   // first sorted index of each cell (all worlds), + total
 	const int* class_list;        // sorted index of each class-list entry
 	int base;                     // first sorted index of this launch's agents
