@@ -1520,6 +1520,11 @@ int umans_autotune(umans_ctx* c, int trial_steps, umans_tuning* out) {
 		c->cell_factor = best_f;
 		c->block = best_b;
 		c->tier_first = best_t;
+		// a fresh time of the stage-1 choice (its best trial is the luckiest of many), and a
+		// per-class block kept only if it is faster by more than the trials' noise (1 %)
+		for (int p = 0; p < NUM_CLS; ++p) c->cls_block[p] = best_cb[p];
+		float base_ms;
+		if (!(rc = trial(base_ms))) best_ms = base_ms;
 		for (int q = 0; q < NUM_CLS && !rc; ++q) {
 			if (q == CLS_COOP || c->cls_count[q] == 0) continue;
 			for (int b : blocks) {
@@ -1528,7 +1533,7 @@ int umans_autotune(umans_ctx* c, int trial_steps, umans_tuning* out) {
 				c->cls_block[q] = b;
 				float ms;
 				if ((rc = trial(ms))) break;
-				if (ms < best_ms) { best_ms = ms; for (int p = 0; p < NUM_CLS; ++p) best_cb[p] = c->cls_block[p]; }
+				if (ms < 0.99f * best_ms) { best_ms = ms; for (int p = 0; p < NUM_CLS; ++p) best_cb[p] = c->cls_block[p]; }
 			}
 		}
 	}
