@@ -261,6 +261,29 @@ typedef struct {
 int umans_adjoint_forward(umans_ctx* ctx, const umans_adjoint_desc* desc, const float* targets, double* loss);
 int umans_adjoint_backward(umans_ctx* ctx, float* d_params, float* d_pos0, float* d_vel0);
 
+/* nsteps × WorldBase::DoStep of one world on caller-owned DEVICE arrays, allocating
+ * nothing. No counterpart in the reference: it is for callers that cannot allocate
+ * device memory (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc). It gives the results of umans_create + umans_set_policies +
+ * umans_upload + umans_step(nsteps) + umans_download with the same world, policies and
+ * agents, through the same kernels.
+ *   agents: device pointers, row order. Inputs id, goal, radius, pref_speed,
+ *     max_speed, max_accel, policy; pos, vel, viewdir are read and updated in place;
+ *     rng (optional) is read and updated, null = every call seeds it from the IDs
+ *     (agent.cpp:38); pref_vel (optional) receives the preferred velocity of the last
+ *     step; world must be null. Policy indices outside [0, n_policies) run policy 0.
+ *   Limits: n <= UMANS_NOALLOC_MAX_AGENTS, 1..16 policies, 1 <= max_neighbors <= 128
+ *     (lists keep the first max_neighbors entries in canonical order), no obstacles.
+ *   Scratch: one fixed device array of about 5.4 MB per GPU the library runs on, held
+ *     until the process ends. Every call on a GPU uses that GPU's array and leaves the
+ *     last call's contents in it: calls that overlap on one GPU (other streams, other
+ *     host threads) race and must be serialised by the caller.
+ *   Asynchronous on `stream` (a hipStream_t; null = the null stream): the arrays are
+ *   read and written by kernels on that stream. err (optional) receives a message. */
+#define UMANS_NOALLOC_MAX_AGENTS 16384
+int umans_step_noalloc(const umans_world_desc* world, const umans_policy* policies, int n_policies,
+                       const umans_agents* agents, int n, int nsteps, void* stream, char* err, int err_len);
+
 /* Per-phase GPU timers (hipEvents on the context's stream), accumulated over the
  * steps run while enabled: grid = cell keys + scan + reorder (+ bounding box in
  * infinite worlds), step = the fused per-agent kernel. The reference has only a

@@ -11,6 +11,7 @@
 #include <cstdio>
 #include <cstring>
 #include <fstream>
+#include <mutex>
 #include <queue>
 #include <string>
 #include <unordered_map>
@@ -503,6 +504,34 @@ int check_world(const umans_world_desc& d) {
 	return UMANS_OK;
 }
 
+// The WorldInfo record of a world holding n_agents agents (cell_offset left 0) and the
+// number of cells reserved for its grid: infinite worlds 4 per agent (at least 64), the
+// bounding-box grid coarsened on the device to fit; toric worlds the fixed grid of the
+// world rectangle, halved along its longer side until it has at most max(4 n, 4096) cells.
+long long world_info(const umans_world_desc& d, float cell_size, int n_agents, WorldInfo& r) {
+	r.world_type = d.world_type;
+	r.W = d.width;
+	r.H = d.height;
+	r.dt = d.delta_time;
+	r.cell_size = cell_size;
+	r.cell_offset = 0;
+	r.pad = 0;
+	const long long cap_cells = std::max(4LL * n_agents, 64LL);
+	if (d.world_type != 1) {
+		r.cell_cap = (int)cap_cells;
+		return cap_cells;
+	}
+	// the same reduction k_grid_setup applies on the device
+	double nx = std::max(1.0, std::floor((double)d.width / (double)r.cell_size));
+	double ny = std::max(1.0, std::floor((double)d.height / (double)r.cell_size));
+	const double lim = (double)std::max(cap_cells, 4096LL);
+	while (nx * ny > lim) {
+		if (nx >= ny) nx = std::max(1.0, std::floor(nx / 2.0)); else ny = std::max(1.0, std::floor(ny / 2.0));
+	}
+	r.cell_cap = (int)std::max(cap_cells, 4096LL);
+	return (long long)(nx * ny);
+}
+
 // Per-world cell reservations and WorldInfo records.
 int setup_worlds(umans_ctx* c, const std::vector<int>& per_world) {
 	const int nw = (int)c->worlds.size();
@@ -516,29 +545,8 @@ int setup_worlds(umans_ctx* c, const std::vector<int>& per_world) {
 	for (int w = 0; w < nw; ++w) {
 		const umans_world_desc& d = c->worlds[w];
 		WorldInfo& r = wi[w];
-		r.world_type = d.world_type;
-		r.W = d.width;
-		r.H = d.height;
-		r.dt = d.delta_time;
-		r.cell_size = c->cell_size_of(d);
-		r.pad = 0;
-		const long long cap_cells = std::max(4LL * per_world[w], 64LL);
-		long long cells;
-		if (d.world_type == 1) {
-			// the same reduction k_grid_setup applies on the device
-			double nx = std::max(1.0, std::floor((double)d.width / (double)r.cell_size));
-			double ny = std::max(1.0, std::floor((double)d.height / (double)r.cell_size));
-			const double lim = (double)std::max(cap_cells, 4096LL);
-			while (nx * ny > lim) {
-				if (nx >= ny) nx = std::max(1.0, std::floor(nx / 2.0)); else ny = std::max(1.0, std::floor(ny / 2.0));
-			}
-			cells = (long long)(nx * ny);
-			r.cell_cap = (int)std::max(cap_cells, 4096LL);
-		} else {
-			cells = cap_cells;
-			r.cell_cap = (int)cells;
-			c->any_infinite = true;
-		}
+		const long long cells = world_info(d, c->cell_size_of(d), per_world[w], r);
+		if (d.world_type != 1) c->any_infinite = true;
 		r.cell_offset = (int)off;
 		off += cells;
 		if (off > (1LL << 30)) return c->fail(UMANS_EINVAL, "grid too large");
@@ -2124,3 +2132,266 @@ int umans_download_records(umans_ctx* c, umans_agent_record* out, int capacity) 
 }
 
 } // extern "C"
+
+// ------------------------------------------------- steps without device allocation
+// umans_step_noalloc runs the kernels of umans_step on caller-owned device arrays. Every
+// buffer beyond its inputs and outputs (the packed and the cell-sorted state, the grid's
+// counters, scan and class lists, the policy table, the step counters) lives in one
+// file-scope device array, `scratch`, of fixed size; each GPU the library runs on holds
+// its own copy until the process ends. Host code never passes the array's name to a
+// kernel (on the host it is only a stand-in for the symbol): it uses the device address
+// hipGetSymbolAddress returns, looked up once per device.
+namespace {
+
+constexpr int kNoAllocMaxPolicies = 16;
+constexpr int kNoAllocMaxCells = 4 * UMANS_NOALLOC_MAX_AGENTS > 4096 ? 4 * UMANS_NOALLOC_MAX_AGENTS : 4096;
+constexpr int kNoAllocMaxDevices = 64;
+
+constexpr size_t na_up(size_t x) { return (x + 255) & ~(size_t)255; }
+// Byte offsets of the sections of `scratch`, each on a 256-byte boundary.
+struct NaLayout {
+	static constexpr size_t N = UMANS_NOALLOC_MAX_AGENTS;
+	static constexpr size_t KEYS = (size_t)NUM_CLS * kNoAllocMaxCells + 1;      // (class slot, cell) counters + total
+	static constexpr size_t TILES = (KEYS + kScanTile - 1) / kScanTile;
+	static constexpr size_t pol = 0;
+	static constexpr size_t world = na_up(pol + kNoAllocMaxPolicies * sizeof(DevPolicy));
+	static constexpr size_t grid = na_up(world + sizeof(WorldInfo));
+	static constexpr size_t bbox = na_up(grid + sizeof(GridInfo));
+	static constexpr size_t stats = na_up(bbox + (size_t)kBoxCopies * kBoxLine * sizeof(int));
+	static constexpr size_t cls_slot = na_up(stats + (size_t)kStatCopies * kStatLine * sizeof(unsigned long long));
+	static constexpr size_t ranges = na_up(cls_slot + NUM_CLS * sizeof(int));
+	static constexpr size_t cell_count = na_up(ranges + 2 * NUM_CLS * sizeof(unsigned));
+	static constexpr size_t cell_start = na_up(cell_count + KEYS * sizeof(unsigned));
+	static constexpr size_t cell_first = na_up(cell_start + KEYS * sizeof(unsigned));
+	static constexpr size_t scan = na_up(cell_first + (kNoAllocMaxCells + 1) * sizeof(unsigned));
+	static constexpr size_t cell_of = na_up(scan + (TILES + 1) * sizeof(unsigned));
+	static constexpr size_t class_list = na_up(cell_of + N * sizeof(unsigned));
+	static constexpr size_t state0 = na_up(class_list + N * sizeof(int));
+	// a State: pos, vel, goal, viewdir, pref (float2), prm (float4), meta (int4)
+	static constexpr size_t state_bytes = N * (5 * sizeof(float2) + sizeof(float4) + sizeof(int4));
+	static constexpr size_t state1 = na_up(state0 + state_bytes);
+	static constexpr size_t total = na_up(state1 + state_bytes);
+};
+
+__device__ __align__(256) unsigned char scratch[NaLayout::total];
+
+// scratch's device address on each device (null until looked up)
+void* g_scratch_ptr[kNoAllocMaxDevices] = {};
+std::mutex g_scratch_mu;
+
+State na_state(unsigned char* base) {
+	const size_t N = NaLayout::N;
+	State s;
+	s.pos = (float2*)base;
+	s.vel = s.pos + N;
+	s.goal = s.vel + N;
+	s.viewdir = s.goal + N;
+	s.pref = s.viewdir + N;
+	s.prm = (float4*)(s.pref + N);
+	s.meta = (int4*)(s.prm + N);
+	return s;
+}
+
+// Caller arrays (row order) -> packed state: what umans_upload builds on the host.
+__global__ __launch_bounds__(256) void k_noalloc_pack(umans_agents a, int n, int n_policies, State x) {
+	const int i = blockIdx.x * blockDim.x + threadIdx.x;
+	if (i >= n) return;
+	x.pos[i] = make_float2(a.pos[2 * i], a.pos[2 * i + 1]);
+	x.vel[i] = make_float2(a.vel[2 * i], a.vel[2 * i + 1]);
+	x.goal[i] = make_float2(a.goal[2 * i], a.goal[2 * i + 1]);
+	x.viewdir[i] = make_float2(a.viewdir[2 * i], a.viewdir[2 * i + 1]);
+	x.prm[i] = make_float4(a.radius[i], a.pref_speed[i], a.max_speed[i], a.max_accel[i]);
+	int p = a.policy[i];
+	if (p < 0 || p >= n_policies) p = 0;       // out-of-range indices run policy 0 (documented)
+	uint32_t r;
+	if (a.rng) r = a.rng[i];
+	else {
+		// std::default_random_engine(id) = minstd_rand0 seeded with the ID (agent.cpp:38)
+		r = (uint32_t)a.id[i] % 2147483647u;
+		if (r == 0) r = 1;
+	}
+	x.meta[i] = make_int4(a.id[i], p, i, (int)r);
+}
+
+// Packed state (sorted order) -> caller arrays (row order): pos, vel, viewdir, RNG state and
+// the preferred velocity of the last step.
+__global__ __launch_bounds__(256) void k_noalloc_unpack(State x, int n, umans_agents a) {
+	const int k = blockIdx.x * blockDim.x + threadIdx.x;
+	if (k >= n) return;
+	const int4 m = x.meta[k];
+	const int i = m.z;
+	const float2 p = x.pos[k], v = x.vel[k], w = x.viewdir[k];
+	a.pos[2 * i] = p.x; a.pos[2 * i + 1] = p.y;
+	a.vel[2 * i] = v.x; a.vel[2 * i + 1] = v.y;
+	a.viewdir[2 * i] = w.x; a.viewdir[2 * i + 1] = w.y;
+	if (a.rng) a.rng[i] = (uint32_t)m.w;
+	if (a.pref_vel) {
+		const float2 q = x.pref[k];
+		a.pref_vel[2 * i] = q.x; a.pref_vel[2 * i + 1] = q.y;
+	}
+}
+
+// First list entry and agent count of each class slot: the class-major scan's start of
+// the slot and its difference to the next slot's (the scan has nslots * slot_cells + 1
+// entries, the last one the total).
+__global__ void k_class_ranges(const unsigned* class_start, int slot_cells, int nslots, unsigned* ranges) {
+	const int q = threadIdx.x;
+	if (q >= nslots) return;
+	const unsigned b = class_start[(size_t)q * slot_cells];
+	ranges[2 * q] = b;
+	ranges[2 * q + 1] = class_start[(size_t)(q + 1) * slot_cells] - b;
+}
+
+// The policy table and the world record, passed by value (kernel arguments are copied at
+// launch, so the call needs no host buffer that outlives it and no synchronisation).
+__global__ void k_noalloc_put_policy(DevPolicy p, DevPolicy* dst) {
+	if (threadIdx.x == 0 && blockIdx.x == 0) *dst = p;
+}
+struct NaSlots {
+	int slot[NUM_CLS];
+};
+__global__ void k_noalloc_put_world(WorldInfo w, WorldInfo* dw, NaSlots s, int* dslot) {
+	if (threadIdx.x == 0 && blockIdx.x == 0) {
+		*dw = w;
+		for (int q = 0; q < NUM_CLS; ++q) dslot[q] = s.slot[q];
+	}
+}
+
+int na_fail(char* err, int err_len, int code, const char* msg) {
+	if (err && err_len > 0) std::snprintf(err, (size_t)err_len, "%s", msg);
+	return code;
+}
+
+} // namespace
+
+extern "C" int umans_step_noalloc(const umans_world_desc* world, const umans_policy* policies, int n_policies,
+                                  const umans_agents* agents, int n, int nsteps, void* stream, char* err,
+                                  int err_len) {
+	if (err && err_len > 0) err[0] = '\0';
+	if (!world || !policies || !agents) return na_fail(err, err_len, UMANS_EINVAL, "null argument");
+	if (check_world(*world)) return na_fail(err, err_len, UMANS_EINVAL, "bad world description");
+	if (world->max_neighbors < 1 || world->max_neighbors > kUncappedMax)
+		return na_fail(err, err_len, UMANS_EINVAL, "max_neighbors must be 1..128");
+	if (n_policies < 1 || n_policies > kNoAllocMaxPolicies)
+		return na_fail(err, err_len, UMANS_EINVAL, "need 1..16 policies");
+	if (n < 0 || n > UMANS_NOALLOC_MAX_AGENTS)
+		return na_fail(err, err_len, UMANS_EINVAL, "agent count beyond UMANS_NOALLOC_MAX_AGENTS");
+	if (nsteps < 0) return na_fail(err, err_len, UMANS_EINVAL, "negative step count");
+	if (n > 0 && (!agents->id || !agents->pos || !agents->vel || !agents->goal || !agents->viewdir ||
+	              !agents->radius || !agents->pref_speed || !agents->max_speed || !agents->max_accel ||
+	              !agents->policy))
+		return na_fail(err, err_len, UMANS_EINVAL, "missing agent array");
+	if (agents->world) return na_fail(err, err_len, UMANS_EINVAL, "a single world: the world array must be null");
+	const int kcap = world->max_neighbors;
+	const int kmax = kmax_bucket(kcap);
+	if (world->block_size > 0 && !block_supported(kmax, world->block_size))
+		return na_fail(err, err_len, UMANS_EINVAL, "unsupported block size");
+	// the policy table, with the classes it uses
+	DevPolicy hp[kNoAllocMaxPolicies];
+	float max_range = 0.0f;
+	bool cls_used[NUM_CLS] = {};
+	for (int i = 0; i < n_policies; ++i) {
+		std::string e;
+		if (make_dev_policy(policies[i], hp[i], e)) return na_fail(err, err_len, UMANS_EINVAL, e.c_str());
+		max_range = std::max(max_range, hp[i].query_range);
+		cls_used[hp[i].cls] = true;
+	}
+	int cls_slot[NUM_CLS] = {};
+	int nslots = 0;
+	for (int q = 0; q < NUM_CLS; ++q) cls_slot[q] = cls_used[q] ? nslots++ : 0;
+	WorldInfo wi;
+	const float cell = world->cell_size > 0.0f ? world->cell_size : std::max(max_range, 0.25f);
+	const long long cells = world_info(*world, cell, n, wi);
+	if (cells > kNoAllocMaxCells) return na_fail(err, err_len, UMANS_EINVAL, "grid beyond the fixed maximum");
+	if (n == 0 || nsteps == 0) return UMANS_OK;
+
+	// scratch's address on the current device, looked up once per device
+	int dev = 0;
+	if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= kNoAllocMaxDevices)
+		return na_fail(err, err_len, UMANS_EHIP, "no usable current device");
+	unsigned char* base = nullptr;
+	{
+		std::lock_guard<std::mutex> lock(g_scratch_mu);
+		if (!g_scratch_ptr[dev]) {
+			void* p = nullptr;
+			const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+			if (e != hipSuccess || !p) return na_fail(err, err_len, UMANS_EHIP, hipGetErrorString(e));
+			g_scratch_ptr[dev] = p;
+		}
+		base = (unsigned char*)g_scratch_ptr[dev];
+	}
+	hipStream_t st = (hipStream_t)stream;
+	DevPolicy* dpol = (DevPolicy*)(base + NaLayout::pol);
+	WorldInfo* dworld = (WorldInfo*)(base + NaLayout::world);
+	GridInfo* dgrid = (GridInfo*)(base + NaLayout::grid);
+	int* dbbox = (int*)(base + NaLayout::bbox);
+	unsigned long long* dstats = (unsigned long long*)(base + NaLayout::stats);
+	int* dslot = (int*)(base + NaLayout::cls_slot);
+	unsigned* dranges = (unsigned*)(base + NaLayout::ranges);
+	unsigned* cell_count = (unsigned*)(base + NaLayout::cell_count);
+	unsigned* cell_start = (unsigned*)(base + NaLayout::cell_start);
+	unsigned* cell_first = (unsigned*)(base + NaLayout::cell_first);
+	unsigned* scan = (unsigned*)(base + NaLayout::scan);
+	unsigned* cell_of = (unsigned*)(base + NaLayout::cell_of);
+	int* class_list = (int*)(base + NaLayout::class_list);
+	const State X = na_state(base + NaLayout::state0);
+	const State Y = na_state(base + NaLayout::state1);
+	for (int i = 0; i < n_policies; ++i)
+		hipLaunchKernelGGL(k_noalloc_put_policy, dim3(1), dim3(64), 0, st, hp[i], dpol + i);
+	NaSlots slots;
+	for (int q = 0; q < NUM_CLS; ++q) slots.slot[q] = cls_slot[q];
+	hipLaunchKernelGGL(k_noalloc_put_world, dim3(1), dim3(64), 0, st, wi, dworld, slots, dslot);
+	hipLaunchKernelGGL(k_noalloc_pack, dim3(ceil_div(n, 256)), dim3(256), 0, st, *agents, n, n_policies, X);
+
+	const int total_cells = (int)cells;
+	const int scan_n = nslots * total_cells + 1;
+	const int nb = ceil_div(scan_n, kScanTile);
+	const int bs = world->block_size > 0 ? world->block_size : default_block(kmax);
+	constexpr int stat_words = 2 * kStatCopies * kStatLine;
+	for (int s = 0; s < nsteps; ++s) {
+		// the counters and the scan's ticket and statuses start at zero (every step: a
+		// previous call may have stopped between the scan and the reorder)
+		hipLaunchKernelGGL(k_fill, dim3(ceil_div(scan_n, 256)), dim3(256), 0, st, cell_count, scan_n, 0u);
+		hipLaunchKernelGGL(k_fill, dim3(ceil_div(nb + 1, 256)), dim3(256), 0, st, scan, nb + 1, 0u);
+		if (wi.world_type != 1) {
+			hipLaunchKernelGGL(k_bbox_init, dim3(1), dim3(64), 0, st, dbbox, 1);
+			hipLaunchKernelGGL(k_bbox, dim3(ceil_div(n, 256)), dim3(256), 0, st, X.pos, X.meta, dworld, n, dbbox);
+		}
+		hipLaunchKernelGGL(k_grid_setup, dim3(1), dim3(64), 0, st, dworld, dbbox, dgrid, 1, nullptr, 0, nullptr, 0, 0);
+		hipLaunchKernelGGL(k_count, dim3(ceil_div(n, 256)), dim3(256), 0, st, X.pos, X.meta, n, dworld, dgrid, dpol,
+		                   dslot, total_cells, cell_count, cell_of);
+		hipLaunchKernelGGL(k_scan_lookback, dim3(nb), dim3(kScanThreads), 0, st, cell_count, cell_start, scan, scan + 1,
+		                   scan_n, nb, (unsigned*)dstats, stat_words, nullptr, 0);
+		const unsigned* cf = nslots > 1 ? cell_first : cell_start;
+		const int nr = std::max(nslots > 1 ? std::max(n, total_cells + 1) : n, nb + 1);
+		hipLaunchKernelGGL(k_reorder, dim3(ceil_div(nr, 256)), dim3(256), 0, st, X, Y, n, cell_of, cell_count, cell_start,
+		                   nslots, total_cells, cell_first, class_list, scan, nb + 1, dworld, nullptr);
+		hipLaunchKernelGGL(k_class_ranges, dim3(1), dim3(64), 0, st, cell_start, total_cells, nslots, dranges);
+		StepArgs a{};
+		a.in = Y;
+		a.out = X;
+		a.pol = dpol;
+		a.worlds = dworld;
+		a.grids = dgrid;
+		a.next_grids = dgrid;
+		a.cell_start = cf;
+		a.class_list = class_list;
+		a.n = n;
+		a.kcap = kcap;
+		a.stats = dstats;
+		a.tier_final = 1;
+		for (int qi = 0; qi < NUM_CLS; ++qi) {
+			const int q = kLaunchOrder[qi];
+			if (!cls_used[q]) continue;
+			StepArgs aq = a;
+			aq.dev_base = dranges + 2 * cls_slot[q];
+			aq.agent_count = dranges + 2 * cls_slot[q] + 1;
+			// lists longer than 32: one launch at full capacity (sorted insertion)
+			launch_step(q, kmax, kmax <= 32 ? class_block(q, kmax, bs, nullptr) : 64, aq, st);
+		}
+	}
+	hipLaunchKernelGGL(k_noalloc_unpack, dim3(ceil_div(n, 256)), dim3(256), 0, st, X, n, *agents);
+	const hipError_t e = hipGetLastError();
+	if (e != hipSuccess) return na_fail(err, err_len, UMANS_EHIP, hipGetErrorString(e));
+	return UMANS_OK;
+}

@@ -929,11 +929,12 @@ __global__ __launch_bounds__(BS) void k_step(StepArgs A) {
 	const int tid = threadIdx.x;
 	const int t = xcd_tile(blockIdx.x, gridDim.x) * BS + tid;
 	const int n_here = A.agent_count ? (int)*A.agent_count : A.n;
+	const int base = A.dev_base ? (int)*A.dev_base : A.base;
 	const bool active = t < n_here;
 	// a wave without agents (most of a launch over a list counted on the device) leaves
 	// at once: it takes part in no wave-level step and adds nothing to the stats
 	if (__ballot(active) == 0ull) return;
-	const int k = active ? (A.agent_list ? A.agent_list[A.base + t] : A.class_list[A.base + t]) : 0;
+	const int k = active ? (A.agent_list ? A.agent_list[base + t] : A.class_list[base + t]) : 0;
 	int total = 0, cnt = 0;
 	const State& S = A.in;
 	int4 meta = make_int4(0, 0, 0, 0);
@@ -954,7 +955,7 @@ __global__ __launch_bounds__(BS) void k_step(StepArgs A) {
 		a.dt = wi.dt;
 		a.relaxation = A.pol[pi].relaxation;
 		if constexpr (kLong) {
-			total = collect_neighbors(A, wi, g, k, a.pos, A.pol[pi].query_range, A.base + t);
+			total = collect_neighbors(A, wi, g, k, a.pos, A.pol[pi].query_range, base + t);
 		} else {
 			cnt = find_neighbors<KMAX, BS>(A, wi, g, k, a.pos, A.pol[pi].query_range, s_d2 + tid, s_nb + tid, total);
 		}
@@ -964,7 +965,7 @@ __global__ __launch_bounds__(BS) void k_step(StepArgs A) {
 	const int kcap_eff = A.kcap < KL ? A.kcap : KL;
 	if constexpr (kLong) {
 		const bool keep = active && total <= kLongCap;
-		rank_long_lists(A, A.base + t, keep ? total : 0, kcap_eff, a.pos, wi);
+		rank_long_lists(A, base + t, keep ? total : 0, kcap_eff, a.pos, wi);
 		cnt = keep ? (total < kcap_eff ? total : kcap_eff) : 0;
 	}
 	// obstacle edges: only the generic class runs when a world has obstacles
@@ -999,7 +1000,7 @@ __global__ __launch_bounds__(BS) void k_step(StepArgs A) {
 	}
 	const DevPolicy& P = A.pol[pi];
 	// the lane's sorted list: its LDS column, or its row of the long-list buffer
-	const unsigned* lst = kLong ? A.long_u + (A.base + t) : s_nb + tid;
+	const unsigned* lst = kLong ? A.long_u + (base + t) : s_nb + tid;
 	const int lstride = kLong ? A.long_stride : BS;
 	if (A.rec_count) {
 		A.rec_count[k] = cnt;
@@ -1198,8 +1199,12 @@ __global__ __launch_bounds__(64) void k_step_coop(StepArgs A) {
 	__shared__ unsigned long long s_key[BS];
 	const int tid = threadIdx.x;
 	const int t = xcd_tile(blockIdx.x, gridDim.x) * BS + tid;
-	const bool active = t < A.n;
-	const int k = active ? A.class_list[A.base + t] : 0;
+	const int n_here = A.agent_count ? (int)*A.agent_count : A.n;
+	const bool active = t < n_here;
+	// one wave per block: a block without agents (a launch sized for a count known only
+	// on the device) leaves at once, before any barrier
+	if (__ballot(active) == 0ull) return;
+	const int k = active ? A.class_list[(A.dev_base ? (int)*A.dev_base : A.base) + t] : 0;
 	int total = 0, cnt = 0;
 	const State& S = A.in;
 	int4 meta = make_int4(0, 0, 0, 0);

@@ -119,6 +119,9 @@ struct StepArgs {
 	// list outgrew its capacity (ovf_list[ovf_base + i], i counted in *ovf_count)
 	const int* agent_list;
 	const unsigned* agent_count;
+	// optional: the launch's first list entry read from the device instead of `base` (the
+	// class ranges of a context-free step are known only on the device)
+	const unsigned* dev_base;
 	int* ovf_list;
 	unsigned* ovf_count;
 	int ovf_base;

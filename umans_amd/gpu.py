@@ -31,7 +31,7 @@ EXPORTS = [
     "umans_scenario_warning", "umans_scenario_get", "umans_scenario_obstacles", "umans_csv_start",
     "umans_csv_flush", "umans_csv_stop", "umans_add_agents", "umans_remove_agent", "umans_set_agent_goal",
     "umans_set_agent_states", "umans_set_remove_at_goal", "umans_num_scheduled", "umans_download_records",
-    "umans_set_obstacles",
+    "umans_set_obstacles", "umans_step_noalloc",
 ]
 
 ENOTFOUND = -2
@@ -150,6 +150,8 @@ def lib():
         L.umans_csv_stop.argtypes = [vp]
         L.umans_set_profiling.argtypes = [vp, C.c_int]
         L.umans_get_profile.argtypes = [vp, C.POINTER(Profile)]
+        L.umans_step_noalloc.argtypes = [C.POINTER(WorldDesc), C.POINTER(PolicyC), C.c_int, C.POINTER(AgentsC),
+                                         C.c_int, C.c_int, vp, C.c_char_p, C.c_int]
         _lib = L
     return _lib
 
@@ -172,6 +174,28 @@ def policy_struct(packed):
         for j in range(8):
             q.cost_params[c][j] = float(packed["cost_params"][c][j])
     return q
+
+
+AGENT_FIELDS = ("id", "pos", "vel", "goal", "viewdir", "radius", "pref_speed", "max_speed", "max_accel", "policy",
+                "rng", "pref_vel", "world")
+
+
+def step_noalloc(world_type, width, height, dt, policies, max_neighbors, ptrs, n, nsteps=1, stream=0,
+                 cell_size=0.0, block_size=0):
+    """umans_step_noalloc: nsteps steps of one world on caller-owned device arrays.
+    `ptrs` maps umans_agents field names to device addresses (ints; a missing field is
+    null); `stream` is a hipStream_t as an int (0: the null stream). Asynchronous."""
+    L = lib()
+    desc = WorldDesc(int(world_type), float(width), float(height), float(dt), int(max_neighbors), float(cell_size),
+                     int(block_size), 0)
+    types = {"id": _I, "policy": _I, "world": _I, "rng": _U}
+    a = AgentsC(*[C.cast(C.c_void_p(ptrs[f]), types.get(f, _F)) if ptrs.get(f) else None for f in AGENT_FIELDS])
+    pols = (PolicyC * len(policies))(*[policy_struct(p) for p in policies])
+    err = C.create_string_buffer(256)
+    rc = L.umans_step_noalloc(C.byref(desc), pols, len(policies), C.byref(a), int(n), int(nsteps),
+                              C.c_void_p(stream or None), err, len(err))
+    if rc != 0:
+        raise UmansError(f"umans_step_noalloc failed ({rc}): {err.value.decode()}")
 
 
 class World:
