@@ -246,13 +246,15 @@ int umans_csv_stop(umans_ctx* ctx);
  * (the minimum-image difference in toric worlds); targets are nsteps x n x 2 floats in
  * upload order, target t being the position after step t+1. umans_adjoint_backward
  * then recomputes each segment from its checkpoint and applies the transposed Jacobian
- * of every step in reverse (the neighbour lists, filters and clamp branches of the
- * forward step held fixed), and returns dL/d(parameters) as 2 floats per policy —
- * (V0, sigma) of SocialForcesAvoidance or (k, tau0) of PowerLaw, chosen by param_kind
- * — and dL/d(initial position) and dL/d(initial velocity), n x 2 floats each in upload
- * order (either may be null). The state is left at the end of the rollout. Every
- * policy must use the gradient method with force-model costs (PowerLaw,
- * SocialForcesAvoidance, GoalReachingForce). */
+ * of every step in reverse — the neighbour lists, filters and clamp branches held fixed
+ * as the FP32 forward step decided them, the Jacobians and adjoint vectors evaluated in
+ * double — and returns dL/d(parameters) as 2 floats per policy — (V0, sigma) of
+ * SocialForcesAvoidance or (k, tau0) of PowerLaw, chosen by param_kind — and
+ * dL/d(initial position) and dL/d(initial velocity), n x 2 floats each in upload order
+ * (either may be null). targets, d_pos0 and d_vel0 may be host or device memory
+ * (unified addressing; device memory must be ready when the call is made). The state
+ * is left at the end of the rollout. Every policy must use the gradient method with
+ * force-model costs (PowerLaw, SocialForcesAvoidance, GoalReachingForce). */
 typedef struct {
 	int32_t nsteps;
 	int32_t checkpoint_every;
@@ -260,6 +262,11 @@ typedef struct {
 } umans_adjoint_desc;
 int umans_adjoint_forward(umans_ctx* ctx, const umans_adjoint_desc* desc, const float* targets, double* loss);
 int umans_adjoint_backward(umans_ctx* ctx, float* d_params, float* d_pos0, float* d_vel0);
+/* The clamp branches the last umans_adjoint_backward held fixed: nsteps x n bytes (step
+ * order, then upload order), bit 0 = the acceleration was clamped to the maximum
+ * acceleration, bit 1 = the velocity to the maximum speed (policy.cpp:72-75), as the
+ * FP32 step decided them. Host or device memory. */
+int umans_adjoint_clamps(umans_ctx* ctx, uint8_t* flags);
 
 /* nsteps × WorldBase::DoStep of one world on caller-owned DEVICE arrays, allocating
  * nothing. No counterpart in the reference: it is for callers that cannot allocate

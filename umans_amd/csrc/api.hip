@@ -231,9 +231,12 @@ struct umans_ctx {
 	float2* targets = nullptr;           // [steps][n], upload-row order
 	unsigned* seg_nb = nullptr;          // [ck][cap][KMAX]
 	int* seg_cnt = nullptr;              // [ck][cap]
-	float2* seg_lg = nullptr;            // [ck][n] loss gradient of each step's output
-	float2* adj_buf = nullptr;           // 5 x n: p, v, p', v', pref
-	float* adj_params = nullptr;         // [policies][2]
+	double2* seg_lg = nullptr;           // [ck][n] loss gradient of each step's output
+	double2* adj_buf = nullptr;          // 5 x n: p, v, p', v', pref
+	float2* adj_out = nullptr;           // n: an adjoint vector handed out in float
+	unsigned char* adj_clamps = nullptr; // [steps][n] clamp branches of the reverse pass (row order)
+	bool adj_clamps_ready = false;
+	double* adj_params = nullptr;        // [copies][policies][2], see AdjArgs::adj_params
 	double* dloss = nullptr;
 	double last_loss = 0.0;
 
@@ -313,6 +316,11 @@ struct umans_ctx {
 		ckpts.clear();
 		seg_states.clear();
 		(void)hipFree(targets); (void)hipFree(seg_nb); (void)hipFree(seg_cnt); (void)hipFree(seg_lg); (void)hipFree(adj_buf);
+		(void)hipFree(adj_out);
+		(void)hipFree(adj_clamps);
+		adj_out = nullptr;
+		adj_clamps = nullptr;
+		adj_clamps_ready = false;
 		(void)hipFree(adj_params); (void)hipFree(dloss);
 		targets = nullptr; seg_nb = nullptr; seg_cnt = nullptr; seg_lg = nullptr; adj_buf = nullptr;
 		adj_params = nullptr; dloss = nullptr;
@@ -1216,21 +1224,16 @@ bool adjoint_policy_ok(const DevPolicy& p) {
 	return true;
 }
 
-void launch_loss(umans_ctx* c, const float2* target, double* loss, float2* grad, float scale) {
+void launch_loss(umans_ctx* c, const float2* target, double* loss, double2* grad, double scale) {
 	hipLaunchKernelGGL(k_loss, dim3(ceil_div(c->n, 256)), dim3(256), 0, c->stream, c->X.s, c->n, c->dworlds, target,
 	                   loss, grad, scale);
 }
 
-// Floats between copies of the adjoint's parameter accumulators: whole 128-byte lines.
-int adj_param_stride(int npol) { return (2 * npol + 31) / 32 * 32; }
+// Doubles between copies of the adjoint's parameter accumulators: whole 128-byte lines.
+int adj_param_stride(int npol) { return (2 * npol + 15) / 16 * 16; }
 
 void fill_f(umans_ctx* c, void* p, long long nfloats) {
 	hipLaunchKernelGGL(k_fill_f, dim3(ceil_div(nfloats, 256)), dim3(256), 0, c->stream, (float*)p, nfloats, 0.0f);
-}
-
-__global__ void k_axpy2(float2* y, const float2* x, int n) {
-	const int i = blockIdx.x * blockDim.x + threadIdx.x;
-	if (i < n) { y[i].x += x[i].x; y[i].y += x[i].y; }
 }
 
 int create_impl(const umans_world_desc* descs, int nworlds, umans_ctx** out) {
@@ -1796,10 +1799,13 @@ int umans_adjoint_forward(umans_ctx* c, const umans_adjoint_desc* d, const float
 	if ((rc = c->hip(dalloc(&c->seg_cnt, (size_t)segn * n), "alloc lists"))) return rc;
 	if ((rc = c->hip(dalloc(&c->seg_lg, (size_t)segn * n), "alloc loss grad"))) return rc;
 	if ((rc = c->hip(dalloc(&c->adj_buf, (size_t)5 * n), "alloc adjoint"))) return rc;
+	if ((rc = c->hip(dalloc(&c->adj_out, (size_t)n), "alloc adjoint"))) return rc;
+	if ((rc = c->hip(dalloc(&c->adj_clamps, (size_t)T * n), "alloc adjoint"))) return rc;
 	if ((rc = c->hip(dalloc(&c->adj_params, (size_t)kStatCopies * adj_param_stride((int)c->hpol.size())), "alloc adjoint")))
 		return rc;
 	if ((rc = c->hip(dalloc(&c->dloss, 1), "alloc loss"))) return rc;
-	if ((rc = c->hip(hipMemcpyAsync(c->targets, targets, (size_t)T * n * sizeof(float2), hipMemcpyHostToDevice,
+	// host or device memory (unified addressing)
+	if ((rc = c->hip(hipMemcpyAsync(c->targets, targets, (size_t)T * n * sizeof(float2), hipMemcpyDefault,
 	                                c->stream), "targets"))) return rc;
 	fill_f(c, c->dloss, 2);
 	for (int t = 0; t < T; ++t) {
@@ -1830,18 +1836,19 @@ int umans_adjoint_backward(umans_ctx* c, float* d_params, float* d_pos0, float* 
 	const int nseg = ceil_div(T, ck);
 	const int kmax = kmax_bucket(c->kcap);
 	const int npol = (int)c->hpol.size();
-	const float scale = (float)(1.0 / ((double)T * (double)n));
+	const double scale = 1.0 / ((double)T * (double)n);
 	int rc;
 	if (!c->B.s.pos && (rc = c->hip(c->B.alloc(c->cap), "alloc backup"))) return rc;
 	if ((rc = c->hip(copy_state(c->B.s, c->X.s, n, c->stream), "backup"))) return rc;
-	float2* ap = c->adj_buf;
-	float2* av = c->adj_buf + n;
-	float2* ap2 = c->adj_buf + 2 * n;
-	float2* av2 = c->adj_buf + 3 * n;
-	float2* apref = c->adj_buf + 4 * n;
-	fill_f(c, c->adj_buf, 2LL * 5 * n);
+	double2* ap = c->adj_buf;
+	double2* av = c->adj_buf + n;
+	double2* ap2 = c->adj_buf + 2 * n;
+	double2* av2 = c->adj_buf + 3 * n;
+	double2* apref = c->adj_buf + 4 * n;
+	// doubles are cleared as zero bits: 4 floats per double2
+	fill_f(c, c->adj_buf, 4LL * 5 * n);
 	const int pstride = adj_param_stride(npol);
-	fill_f(c, c->adj_params, (long long)kStatCopies * pstride);
+	fill_f(c, c->adj_params, 2LL * kStatCopies * pstride);     // doubles: zero bits
 	for (int seg = nseg - 1; seg >= 0; --seg) {
 		const int t0 = seg * ck, t1 = std::min(t0 + ck, T);
 		if ((rc = c->hip(copy_state(c->X.s, c->ckpts[seg].s, n, c->stream), "restore checkpoint"))) return rc;
@@ -1849,15 +1856,15 @@ int umans_adjoint_backward(umans_ctx* c, float* d_params, float* d_pos0, float* 
 			const int s = t - t0;
 			if ((rc = enqueue_step(c, false, &c->seg_states[s].s, c->seg_nb + (size_t)s * n * kmax,
 			                       c->seg_cnt + (size_t)s * n))) return rc;
-			fill_f(c, c->seg_lg + (size_t)s * n, 2LL * n);
+			fill_f(c, c->seg_lg + (size_t)s * n, 4LL * n);
 			launch_loss(c, c->targets + (size_t)t * n, nullptr, c->seg_lg + (size_t)s * n, scale);
 		}
 		for (int t = t1 - 1; t >= t0; --t) {
 			const int s = t - t0;
 			hipLaunchKernelGGL(k_axpy2, dim3(ceil_div(n, 256)), dim3(256), 0, c->stream, ap, c->seg_lg + (size_t)s * n, n);
-			fill_f(c, ap2, 2LL * n);
-			fill_f(c, av2, 2LL * n);
-			fill_f(c, apref, 2LL * n);
+			fill_f(c, ap2, 4LL * n);
+			fill_f(c, av2, 4LL * n);
+			fill_f(c, apref, 4LL * n);
 			AdjArgs A{};
 			A.in = c->seg_states[s].s;
 			A.nb_count = c->seg_cnt + (size_t)s * n;
@@ -1874,28 +1881,49 @@ int umans_adjoint_backward(umans_ctx* c, float* d_params, float* d_pos0, float* 
 			A.adj_params = c->adj_params;
 			A.param_stride = pstride;
 			A.param_kind = c->adj_param_kind;
+			A.clamps = c->adj_clamps + (size_t)t * n;
 			hipLaunchKernelGGL(k_step_adjoint, dim3(ceil_div(n, 128)), dim3(128), 0, c->stream, A);
 			hipLaunchKernelGGL(k_pref_adjoint, dim3(ceil_div(n, 256)), dim3(256), 0, c->stream, c->seg_states[s].s, n,
-			                   (const float2*)apref, ap2);
+			                   (const double2*)apref, ap2);
 			std::swap(ap, ap2);
 			std::swap(av, av2);
 		}
 	}
 	if ((rc = c->hip(hipGetLastError(), "adjoint launch"))) return rc;
-	std::vector<float> hc((size_t)kStatCopies * pstride);
-	if ((rc = c->hip(hipMemcpyAsync(hc.data(), c->adj_params, hc.size() * sizeof(float), hipMemcpyDeviceToHost, c->stream), "adjoint"))) return rc;
-	if (d_pos0 && (rc = c->hip(hipMemcpyAsync(d_pos0, ap, (size_t)n * sizeof(float2), hipMemcpyDeviceToHost, c->stream), "adjoint"))) return rc;
-	if (d_vel0 && (rc = c->hip(hipMemcpyAsync(d_vel0, av, (size_t)n * sizeof(float2), hipMemcpyDeviceToHost, c->stream), "adjoint"))) return rc;
+	std::vector<double> hc((size_t)kStatCopies * pstride);
+	if ((rc = c->hip(hipMemcpyAsync(hc.data(), c->adj_params, hc.size() * sizeof(double), hipMemcpyDeviceToHost, c->stream), "adjoint"))) return rc;
+	// handed out in float, to host or device memory (unified addressing); the stream
+	// orders the staging buffer's second use after the first copy
+	const double2* outs[2] = {ap, av};
+	float* dsts[2] = {d_pos0, d_vel0};
+	for (int q = 0; q < 2; ++q) {
+		if (!dsts[q]) continue;
+		hipLaunchKernelGGL(k_to_float2, dim3(ceil_div(n, 256)), dim3(256), 0, c->stream, outs[q], c->adj_out, n);
+		if ((rc = c->hip(hipMemcpyAsync(dsts[q], c->adj_out, (size_t)n * sizeof(float2), hipMemcpyDefault, c->stream),
+		                 "adjoint"))) return rc;
+	}
 	if ((rc = c->hip(copy_state(c->X.s, c->B.s, n, c->stream), "restore"))) return rc;
 	if ((rc = c->hip(hipStreamSynchronize(c->stream), "adjoint"))) return rc;
 	if (d_params) {
 		for (int i = 0; i < 2 * npol; ++i) {
-			float sum = 0.0f;
+			double sum = 0.0;
 			for (int cp = 0; cp < kStatCopies; ++cp) sum += hc[(size_t)cp * pstride + i];
-			d_params[i] = sum;
+			d_params[i] = (float)sum;
 		}
 	}
+	c->adj_clamps_ready = true;
 	return UMANS_OK;
+}
+
+int umans_adjoint_clamps(umans_ctx* c, uint8_t* flags) {
+	if (!c || !flags) return UMANS_EINVAL;
+	if (!c->adj_ready || !c->adj_clamps_ready || c->adj_n != c->n)
+		return c->fail(UMANS_ESTATE, "run umans_adjoint_forward and umans_adjoint_backward first");
+	if (const int rc_dev = c->hip(hipSetDevice(c->device), "set device")) return rc_dev;
+	const int rc = c->hip(hipMemcpyAsync(flags, c->adj_clamps, (size_t)c->adj_steps * c->n, hipMemcpyDefault, c->stream),
+	                      "adjoint clamps");
+	if (rc) return rc;
+	return c->hip(hipStreamSynchronize(c->stream), "adjoint clamps");
 }
 
 } // extern "C"

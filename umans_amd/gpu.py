@@ -31,7 +31,7 @@ EXPORTS = [
     "umans_scenario_warning", "umans_scenario_get", "umans_scenario_obstacles", "umans_csv_start",
     "umans_csv_flush", "umans_csv_stop", "umans_add_agents", "umans_remove_agent", "umans_set_agent_goal",
     "umans_set_agent_states", "umans_set_remove_at_goal", "umans_num_scheduled", "umans_download_records",
-    "umans_set_obstacles", "umans_step_noalloc",
+    "umans_set_obstacles", "umans_step_noalloc", "umans_adjoint_clamps",
 ]
 
 ENOTFOUND = -2
@@ -127,6 +127,7 @@ def lib():
         L.umans_set_tuning.argtypes = [vp, C.POINTER(Tuning)]
         L.umans_adjoint_forward.argtypes = [vp, C.POINTER(AdjointDesc), _F, C.POINTER(C.c_double)]
         L.umans_adjoint_backward.argtypes = [vp, _F, _F, _F]
+        L.umans_adjoint_clamps.argtypes = [vp, C.POINTER(C.c_uint8)]
         L.umans_cost_param_read_name.argtypes = [C.c_int, C.c_int]
         L.umans_cost_param_read_name.restype = C.c_char_p
         L.umans_scenario_load.argtypes = [C.c_char_p, C.POINTER(vp), C.c_char_p, C.c_int]
@@ -419,14 +420,21 @@ class World:
 
     def adjoint_forward(self, targets, nsteps, checkpoint_every=10, param_kind=2):
         """Run nsteps from the current state with checkpoints; return the position loss
-        against targets (nsteps x n x 2, upload order)."""
+        against targets (nsteps x n x 2, upload order). param_kind: the cost whose two
+        parameters are differentiated (2 SocialForcesAvoidance, 1 PowerLaw)."""
         t = np.ascontiguousarray(targets, dtype=np.float32)
         if t.shape != (nsteps, self.n, 2):
             raise ValueError(f"targets must have shape {(nsteps, self.n, 2)}")
         self._adj_keep = t
+        return self.adjoint_forward_ptr(t.ctypes.data, nsteps, checkpoint_every, param_kind)
+
+    def adjoint_forward_ptr(self, targets_ptr, nsteps, checkpoint_every=10, param_kind=2):
+        """adjoint_forward with the targets given by address (host or device memory of
+        nsteps x n x 2 floats, ready when called)."""
         loss = C.c_double()
         d = AdjointDesc(int(nsteps), int(checkpoint_every), int(param_kind))
-        self._check(lib().umans_adjoint_forward(self._h, C.byref(d), _p(t, _F), C.byref(loss)))
+        self._check(lib().umans_adjoint_forward(self._h, C.byref(d), C.cast(C.c_void_p(targets_ptr), _F),
+                                                C.byref(loss)))
         return loss.value
 
     def adjoint_backward(self, n_policies):
@@ -435,6 +443,21 @@ class World:
         dvel = np.zeros((self.n, 2), np.float32)
         self._check(lib().umans_adjoint_backward(self._h, _p(dp, _F), _p(dpos, _F), _p(dvel, _F)))
         return dp, dpos, dvel
+
+    def adjoint_clamps(self, nsteps):
+        """The clamp branches of the last backward pass: (nsteps, n) uint8, bit 0 the
+        acceleration clamp, bit 1 the speed clamp."""
+        out = np.zeros((nsteps, self.n), np.uint8)
+        self._check(lib().umans_adjoint_clamps(self._h, out.ctypes.data_as(C.POINTER(C.c_uint8))))
+        return out
+
+    def adjoint_backward_ptr(self, n_policies, dpos_ptr, dvel_ptr):
+        """adjoint_backward writing dL/dpos0 and dL/dvel0 to the given addresses (host or
+        device memory of n x 2 floats; 0 = not wanted); returns dL/d(parameters)."""
+        dp = np.zeros((n_policies, 2), np.float32)
+        cast = lambda q: C.cast(C.c_void_p(q), _F) if q else None
+        self._check(lib().umans_adjoint_backward(self._h, _p(dp, _F), cast(dpos_ptr), cast(dvel_ptr)))
+        return dp
 
     def set_profiling(self, enable=True):
         self._check(lib().umans_set_profiling(self._h, 1 if enable else 0))

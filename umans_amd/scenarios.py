@@ -227,8 +227,14 @@ def sweep(n_worlds=512, n_min=20, n_max=5000, steps=500, seed=4, max_neighbors=0
     return Batch("sweep%d" % n_worlds, worlds, pols, agents, max_neighbors, steps, sizes)
 
 
+def powerlaw_gradient_policy(k=1.5, tau0=3.0, rng=5.0, relax=0.5):
+    """{PowerLaw + GoalReachingForce} with the gradient method (examples/policies/PowerLaw.xml)."""
+    return pack_policy("gradient", [("PowerLaw", 1.0, {"k": k, "tau0": tau0, "range": rng}),
+                                    ("GoalReachingForce", 1.0, {})], relaxation_time=relax)
+
+
 def calibration(n=10_000, size=100.0, V0=1500.0, sigma=0.12, seed=5, dt=0.1, steps=100, max_accel=5.0,
-                max_neighbors=0, spread_deg=20.0):
+                max_neighbors=0, spread_deg=20.0, model="socialforces", k=1.5, tau0=3.0):
     """Adjoint calibration crowd: n agents in a size x size toric world at one agent per
     square metre, on a jittered lattice (spacing size / sqrt(n), jitter +-20% of it, so
     nobody starts overlapping), all walking in one general direction (+x within
@@ -238,7 +244,8 @@ def calibration(n=10_000, size=100.0, V0=1500.0, sigma=0.12, seed=5, dt=0.1, ste
     v0 = v_pref (1 + 0.1 N(0,1)). Random placement at this density starts half the
     agents in contact and the 5000 N/m contact forces make the rollout chaotic within
     a few steps (FP32 and FP64 trajectories part by metres within 40 steps); this
-    lattice flow stays within about 1e-4 m for 20 steps."""
+    lattice flow stays within about 1e-4 m for 20 steps. model="powerlaw": the same crowd
+    with {PowerLaw (k, tau0, range 5 m) + GoalReachingForce} instead."""
     r = np.random.default_rng(seed)
     a = _agents(n)
     m = int(np.ceil(np.sqrt(n)))
@@ -253,16 +260,33 @@ def calibration(n=10_000, size=100.0, V0=1500.0, sigma=0.12, seed=5, dt=0.1, ste
     pref = d / np.linalg.norm(d, axis=1, keepdims=True) * a["pref_speed"][:, None]
     a["vel"][:] = (pref * (1.0 + 0.1 * r.standard_normal(n))[:, None]).astype(np.float32)
     a["viewdir"] = initial_viewdir(a["pos"], a["goal"])
-    pol = socialforces_gradient_policy(V0=V0, sigma=sigma, rng=5.0, mass=80.0, relax=0.5)
-    return Scenario("calibration%d" % n, 1, size, size, dt, max_neighbors, [pol], a, steps)
+    if model == "powerlaw":
+        pol = powerlaw_gradient_policy(k=k, tau0=tau0, rng=5.0, relax=0.5)
+        name = "calibration_powerlaw%d" % n
+    else:
+        pol = socialforces_gradient_policy(V0=V0, sigma=sigma, rng=5.0, mass=80.0, relax=0.5)
+        name = "calibration%d" % n
+    return Scenario(name, 1, size, size, dt, max_neighbors, [pol], a, steps)
+
+
+# the two parameters the adjoint differentiates, per cost kind: parameter slots of
+# (V0, sigma) of SocialForcesAvoidance and (k, tau0) of PowerLaw
+PARAM_SLOTS = {2: (2, 3), 1: (1, 2)}
+
+
+def with_params(packed, kind, p0, p1):
+    """Copy of a packed policy with the two differentiated parameters of every cost of
+    `kind` replaced: (V0, sigma) for SocialForcesAvoidance (2), (k, tau0) for PowerLaw (1)."""
+    q = {k: (list(v) if isinstance(v, list) else v) for k, v in packed.items()}
+    q["cost_params"] = [list(x) for x in packed["cost_params"]]
+    s0, s1 = PARAM_SLOTS[kind]
+    for c in range(q["n_cost"]):
+        if q["cost_kind"][c] == kind:
+            q["cost_params"][c][s0] = float(p0)
+            q["cost_params"][c][s1] = float(p1)
+    return q
 
 
 def with_sf_params(packed, V0, sigma):
     """Copy of a packed policy with the SocialForcesAvoidance V0 and sigma replaced."""
-    q = {k: (list(v) if isinstance(v, list) else v) for k, v in packed.items()}
-    q["cost_params"] = [list(x) for x in packed["cost_params"]]
-    for c in range(q["n_cost"]):
-        if q["cost_kind"][c] == 2:
-            q["cost_params"][c][2] = float(V0)
-            q["cost_params"][c][3] = float(sigma)
-    return q
+    return with_params(packed, 2, V0, sigma)
