@@ -1,11 +1,13 @@
 """The host scenario loader against the reference's own loader.
 
-For every obstacle-free scenario of the reference's examples/ (read from the reference
-checkout where it exists; skipped elsewhere): the agents our loader produces (IDs,
-positions, goals, viewing directions) equal the reference's after
-CrowdSimulator::FromConfigFile, and two DoSteps of the reference equal two oracle steps
-run with our loaded policies, bit for bit — which pins names, parameters, defaults and
-read order of the registry end to end. Error paths follow crowdSimulator.cpp."""
+For every scenario of the reference's examples/ (read from the reference checkout where
+it exists; skipped elsewhere): the agents our loader produces (IDs, positions, goals,
+viewing directions) equal the reference's after CrowdSimulator::FromConfigFile, and two
+DoSteps of the reference equal two oracle steps run with our loaded policies and
+obstacles, bit for bit — which pins names, parameters, defaults and read order of the
+registry end to end. Error paths follow crowdSimulator.cpp. Packed policies written back
+as the reference's XML (registry.to_reference_xml_from_packed, which the CPU baselines
+use) load to the same packed policies."""
 import glob
 import os
 
@@ -45,8 +47,8 @@ def test_loader_matches_reference(path):
         if "not supported by this backend" in str(e):
             pytest.skip(str(e))
         raise
-    if len(meta["obstacle_counts"]):
-        pytest.skip("scenario has obstacles (not on the GPU path)")
+    edges = (O.polygon_edges(meta["obstacle_counts"], meta["obstacle_points"]) if len(meta["obstacle_counts"])
+             else None)
     ref = _quiet(lambda: R.RefSim.from_file(path))
     st = ref.state()
     np.testing.assert_array_equal(sc.agents["id"], st["id"])
@@ -54,7 +56,7 @@ def test_loader_matches_reference(path):
     np.testing.assert_array_equal(sc.agents["goal"], st["goal"])
     np.testing.assert_array_equal(sc.agents["viewdir"], st["viewdir"])
     a = dict(sc.agents)
-    wd = O.make_world(sc.world_type, sc.width, sc.height, sc.dt, 0)
+    wd = O.make_world(sc.world_type, sc.width, sc.height, sc.dt, 0, edges=edges)
     for _ in range(2):
         s1 = ref.state()               # each step starts from the reference's state
         a["pos"], a["vel"], a["viewdir"] = s1["pos"], s1["vel"], s1["viewdir"]
@@ -131,3 +133,27 @@ def test_loader_ids_files_and_unknown_cost(tmp_path):
     np.testing.assert_array_equal(sc.agents["goal"][1], [3, 4])          # no goal element: stays put
     np.testing.assert_allclose(sc.agents["viewdir"][2], [0.6, 0.8], rtol=1e-6)
     assert any("Nope" in w for w in meta["warnings"]) and any("sampling type" in w for w in meta["warnings"])
+
+
+def test_packed_policies_round_trip_through_reference_xml(tmp_path):
+    from umans_amd import registry, scenarios as S
+    pols = [S.powerlaw_sampling_policy(2.25, 3.5, 4.0, 12, 6), S.socialforces_gradient_policy(2000.0, 0.08, 4.0),
+            S.orca_policy(3.0, 4.0), S.karamouzas_sampling_policy(5.0, 16, 8), S.powerlaw_gradient_policy(1.5, 3.0),
+            registry.pack_policy("sampling", [("RVO", 0.5, {"w": 2.0, "range": 3.0}), ("VanToll", 1.5, {})],
+                                 sampling={"SamplingType": "random", "RandomSamples": 250,
+                                           "SamplingBase": "current velocity", "SamplingAngle": 90},
+                                 relaxation_time=0.25, contact_force_scale=0.0)]
+    xml = R.scenario_xml(1, 50.0, 40.0, 0.1, [registry.to_reference_xml_from_packed(p) for p in pols])
+    sc, meta = gpu.load_scenario(_write(tmp_path, xml))
+    assert len(sc.policies) == len(pols)
+    for got, want in zip(sc.policies, pols):
+        for k in ("method", "s_type", "s_base", "s_base_dir", "s_radius", "s_angle", "s_speed_samples",
+                  "s_angle_samples", "s_random_samples", "s_include_base", "relaxation_time", "contact_force_scale",
+                  "n_cost"):
+            assert got[k] == pytest.approx(want[k], rel=1e-7), k
+        n = want["n_cost"]
+        for k in ("cost_kind", "cost_coeff", "cost_range"):
+            np.testing.assert_allclose(np.asarray(got[k][:n], np.float64), np.asarray(want[k][:n], np.float64),
+                                       rtol=1e-7, err_msg=k)
+        np.testing.assert_allclose(np.asarray(got["cost_params"][:n], np.float32),
+                                   np.asarray(want["cost_params"][:n], np.float32), rtol=1e-7)

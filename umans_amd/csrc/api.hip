@@ -9,6 +9,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <fstream>
 #include <mutex>
@@ -179,6 +180,9 @@ struct umans_ctx {
 	                                          // 128-entry launch; 3: long lists collected and ranked per
 	                                          // wave (autotuned; lists longer than 32 only)
 	int cell_alloc = 0;          // allocated cell counters (>= total_cells + 1)
+	// sleeps a scan tile waits for its predecessors before it sums their counts itself
+	// (UMANS_SCAN_SPIN_LIMIT in the environment at creation; negative: never wait)
+	int scan_spin_limit = kScanSpinLimit;
 	int counts_zero_n = -1;      // the counters are zero over this key space (-1: unknown)
 	// the previous step of this run_steps call counted X for this step (cell_count,
 	// cell_of): the step skips k_count
@@ -758,7 +762,8 @@ int enqueue_step(umans_ctx* c, bool events, const State* ysave = nullptr, unsign
 		hipLaunchKernelGGL(k_fill, dim3(ceil_div(nb + 1, 256)), dim3(256), 0, st, c->scan_sums, nb + 1, 0u);
 	constexpr int stat_words = 2 * kStatCopies * kStatLine;
 	hipLaunchKernelGGL(k_scan_lookback, dim3(nb), dim3(kScanThreads), 0, st, c->cell_count, c->cell_start, c->scan_sums,
-	                   c->scan_sums + 1, scan_n, nb, (unsigned*)c->dstats, stat_words, c->ovf_count, 2 * NUM_CLS);
+	                   c->scan_sums + 1, scan_n, nb, (unsigned*)c->dstats, stat_words, c->ovf_count, 2 * NUM_CLS,
+	                   c->scan_spin_limit);
 	// one class slot: the per-cell starts are the class-major ones; with several the
 	// reorder also writes them
 	const unsigned* cell_first = c->nslots > 1 ? c->cell_first : c->cell_start;
@@ -1245,6 +1250,7 @@ int create_impl(const umans_world_desc* descs, int nworlds, umans_ctx** out) {
 	if (kcap > kUncappedMax) return UMANS_EINVAL;
 	if (descs[0].block_size > 0 && !block_supported(kmax_bucket(kcap), descs[0].block_size)) return UMANS_EINVAL;
 	umans_ctx* c = new umans_ctx();
+	if (const char* sl = std::getenv("UMANS_SCAN_SPIN_LIMIT")) c->scan_spin_limit = std::atoi(sl);
 	c->worlds.assign(descs, descs + nworlds);
 	c->device = descs[0].device;
 	c->kcap = kcap;
@@ -2389,7 +2395,7 @@ extern "C" int umans_step_noalloc(const umans_world_desc* world, const umans_pol
 		hipLaunchKernelGGL(k_count, dim3(ceil_div(n, 256)), dim3(256), 0, st, X.pos, X.meta, n, dworld, dgrid, dpol,
 		                   dslot, total_cells, cell_count, cell_of);
 		hipLaunchKernelGGL(k_scan_lookback, dim3(nb), dim3(kScanThreads), 0, st, cell_count, cell_start, scan, scan + 1,
-		                   scan_n, nb, (unsigned*)dstats, stat_words, nullptr, 0);
+		                   scan_n, nb, (unsigned*)dstats, stat_words, nullptr, 0, kScanSpinLimit);
 		const unsigned* cf = nslots > 1 ? cell_first : cell_start;
 		const int nr = std::max(nslots > 1 ? std::max(n, total_cells + 1) : n, nb + 1);
 		hipLaunchKernelGGL(k_reorder, dim3(ceil_div(nr, 256)), dim3(256), 0, st, X, Y, n, cell_of, cell_count, cell_start,

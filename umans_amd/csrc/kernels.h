@@ -227,11 +227,15 @@ UD unsigned block_exclusive_scan(unsigned v, unsigned* sh, unsigned& total) {
 // the totals of the tiles before it, walking back until one has published its inclusive
 // prefix. A tile waits only on tiles with lower tickets, whose blocks have started and
 // publish their totals without waiting on anyone, so the walk always ends (tile 0
-// publishes its inclusive prefix at once). status[t] = flag | value (flag kScanAgg: the
-// tile's total, kScanInc: its inclusive prefix; values below 2^30); `ticket` and
+// publishes its inclusive prefix at once). The wait is bounded all the same: after
+// spin_limit sleeps (negative: at once) the tile stops waiting and sums the counts of
+// every earlier tile itself — they are all in memory before the scan starts — so its
+// prefix is exact whatever the other tiles do. status[t] = flag | value (flag kScanAgg:
+// the tile's total, kScanInc: its inclusive prefix; values below 2^30); `ticket` and
 // status[0..nb) must be zero at launch. Also clears the step's counters
 // (zero0[0..n0), zero1[0..n1)).
 constexpr unsigned kScanAgg = 1u << 30, kScanInc = 2u << 30, kScanValMask = kScanAgg - 1u;
+constexpr int kScanSpinLimit = 1 << 20;   // sleeps before a tile sums its predecessors' counts itself
 UD unsigned ld_status(unsigned* p) {
 	return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
@@ -240,7 +244,7 @@ UD void st_status(unsigned* p, unsigned v) {
 }
 __global__ __launch_bounds__(kScanThreads) void k_scan_lookback(const unsigned* in, unsigned* out, unsigned* ticket,
                                                                 unsigned* status, int n, int nb, unsigned* zero0,
-                                                                int n0, unsigned* zero1, int n1) {
+                                                                int n0, unsigned* zero1, int n1, int spin_limit) {
 	__shared__ unsigned sh[kScanThreads / 64];
 	__shared__ unsigned s_tile, s_prefix;
 	for (int i = blockIdx.x * kScanThreads + threadIdx.x; i < n0; i += gridDim.x * kScanThreads) zero0[i] = 0u;
@@ -269,8 +273,7 @@ __global__ __launch_bounds__(kScanThreads) void k_scan_lookback(const unsigned* 
 	for (int i = 0; i < kScanItems; ++i) s += v[i];
 	unsigned total;
 	unsigned run = block_exclusive_scan(s, sh, total);
-	// the first wave walks back over windows of 64 predecessors at once; the wait is
-	// bounded (~2 s) so that the kernel can never hang, whatever the state
+	// the first wave walks back over windows of 64 predecessors at once
 	if (threadIdx.x < 64) {
 		const int lane = threadIdx.x;
 		unsigned prefix = 0;
@@ -278,7 +281,8 @@ __global__ __launch_bounds__(kScanThreads) void k_scan_lookback(const unsigned* 
 			if (lane == 0) st_status(&status[0], kScanInc | total);
 		} else {
 			if (lane == 0) st_status(&status[tile], kScanAgg | total);
-			for (int top = tile - 1, spins = 0;;) {
+			bool own_sum = spin_limit < 0;
+			for (int top = tile - 1, spins = 0; !own_sum;) {
 				const int j = top - lane;
 				// before tile 0: an inclusive prefix of zero
 				const unsigned w = j >= 0 ? ld_status(&status[j]) : kScanInc;
@@ -287,7 +291,11 @@ __global__ __launch_bounds__(kScanThreads) void k_scan_lookback(const unsigned* 
 				const unsigned long long unpublished = __ballot(flag == 0u);
 				const int first_inc = inc ? __builtin_ctzll(inc) : 64;
 				const unsigned long long needed = first_inc >= 63 ? ~0ull : ((2ull << first_inc) - 1ull);
-				if ((unpublished & needed) && ++spins < (1 << 26)) {
+				if (unpublished & needed) {
+					if (++spins > spin_limit) {
+						own_sum = true;
+						break;
+					}
 					__builtin_amdgcn_s_sleep(1);
 					continue;
 				}
@@ -297,6 +305,18 @@ __global__ __launch_bounds__(kScanThreads) void k_scan_lookback(const unsigned* 
 				prefix += val;
 				if (first_inc < 64) break;
 				top -= 64;
+			}
+			if (own_sum) {
+				// the counts of tiles 0 .. tile-1: tile * kScanTile of them (a multiple of 4,
+				// within n), read 16 bytes at a time by the wave
+				const int end = tile * kScanTile;
+				unsigned val = 0;
+				for (int i = 4 * lane; i < end; i += 256) {
+					const uint4 x = *reinterpret_cast<const uint4*>(in + i);
+					val += x.x + x.y + x.z + x.w;
+				}
+				for (int o = 32; o > 0; o >>= 1) val += __shfl_xor(val, o);
+				prefix = val;
 			}
 			if (lane == 0) st_status(&status[tile], kScanInc | (prefix + total));
 		}

@@ -120,3 +120,32 @@ def to_reference_xml_policy(method, costs, sampling=None, relaxation_time=None, 
         attrs.update(params or {})
         d["costs"].append((name, attrs))
     return d
+
+
+def to_reference_xml_from_packed(packed):
+    """A packed policy as the reference XML loader's policy (crowdSimulator.cpp:327-422):
+    the optimisation method and SamplingParameters attributes, RelaxationTime,
+    ContactForceScale and each cost function with its coefficient, range and parameters
+    (registry attribute names; "U0" is not written: it only aliases V0)."""
+    inv = lambda table, v: [k for k, x in table.items() if x == v][0]
+    method = inv(METHODS, packed["method"])
+    sampling = None
+    if method != "gradient":
+        sampling = {"SamplingType": inv(SAMPLING_TYPE, packed["s_type"]),
+                    "SamplingBase": inv(SAMPLING_BASE, packed["s_base"]),
+                    "SamplingBaseDirection": inv(SAMPLING_BASE_DIR, packed["s_base_dir"]),
+                    "SamplingRadius": inv(SAMPLING_RADIUS, packed["s_radius"]),
+                    "SamplingAngle": float(packed["s_angle"]), "SpeedSamples": int(packed["s_speed_samples"]),
+                    "AngleSamples": int(packed["s_angle_samples"]), "RandomSamples": int(packed["s_random_samples"]),
+                    "IncludeBaseAsSample": "true" if packed["s_include_base"] else "false"}
+    costs = []
+    for c in range(packed["n_cost"]):
+        name = KIND_NAMES[packed["cost_kind"][c]]
+        params = {"range": float(packed["cost_range"][c])} if name != "GoalReachingForce" else {}
+        for attr, slot, _ in COSTS[name][2]:
+            if attr != "U0":
+                params[attr] = float(packed["cost_params"][c][slot])
+        costs.append((name, float(packed["cost_coeff"][c]), params))
+    return to_reference_xml_policy(method, costs, sampling=sampling,
+                                   relaxation_time=float(packed["relaxation_time"]),
+                                   contact_force_scale=float(packed["contact_force_scale"]))
