@@ -35,7 +35,12 @@ enum {
 	UMANS_EINVAL = -22,    /* bad argument */
 	UMANS_ENOMEM = -12,    /* device allocation failed */
 	UMANS_EHIP = -1000,    /* HIP runtime error (message in umans_last_error) */
-	UMANS_ESTATE = -1001   /* call out of order (e.g. step before upload) */
+	UMANS_ESTATE = -1001,  /* call out of order (e.g. step before upload) */
+	UMANS_ETRUNC = -1002   /* a step kept other neighbours than the reference would: an agent
+	                          with no cap, or with more than 32 obstacle edges in range, had
+	                          more than 4096 neighbours in range (reported, and cleared, by the
+	                          next umans_synchronize, umans_get_stats or umans_download; their
+	                          outputs are still written) */
 };
 
 /* Cost-function kinds. Registry names (costFunctionFactory.cpp:47-60):
@@ -76,9 +81,11 @@ typedef struct {
 /* World description. world_type 0 = WorldInfinite (worldInfinite.cpp), 1 = WorldToric
  * (worldToric.cpp) centred on the origin. max_neighbors caps each agent's neighbour
  * list at the first k entries of the reference's list in canonical order (ascending
- * double d2, ties by agent ID; toric segments concatenated); <= 0 means no cap
- * (lists longer than the backend maximum, 128, are truncated and counted in
- * umans_stats.overflow_agents). cell_size 0 = automatic (max interaction range);
+ * double d2, ties by agent ID; toric segments concatenated); 1..128, or <= 0 for no
+ * cap: every neighbour in range, as AgentKDTree::FindNeighbors returns them
+ * (AgentKDTree.cpp:35-59). Uncapped lists of up to 4096 entries are exact (agents with
+ * more than 128 are ranked by a wave each in LDS); a longer one is truncated and the
+ * step reports UMANS_ETRUNC. cell_size 0 = automatic (max interaction range);
  * block_size 0 = automatic. */
 typedef struct {
 	int32_t world_type;
@@ -111,7 +118,8 @@ typedef struct {
 
 typedef struct {
 	int64_t steps;             /* steps run on this context */
-	int64_t overflow_agents;   /* agents whose neighbour list hit the cap, last step */
+	int64_t overflow_agents;   /* agents with more neighbours in range than their list keeps
+	                              (the cap; with no cap 4096), last step */
 	int64_t max_list;          /* longest uncapped neighbour list, last step */
 	double sim_time;           /* WorldBase::time_ */
 	int64_t obstacle_overflow_agents;   /* agents with more obstacle edges in range than kept, last step */
@@ -224,8 +232,9 @@ int umans_download_records(umans_ctx* ctx, umans_agent_record* out, int capacity
  * (i + 1) mod n (Polygon2D.cpp:27-38). Replaces any previous obstacles; n_polygons 0
  * removes them. Every agent within range of an edge sees it: time-to-collision models,
  * SocialForcesAvoidance's obstacle force and the contact forces (costFunction.cpp:229-325,
- * SocialForcesAvoidance.cpp:112-140, policy.cpp:126-137). Up to 32 edges per agent are
- * kept (umans_stats.obstacle_overflow_agents counts agents with more). */
+ * SocialForcesAvoidance.cpp:112-140, policy.cpp:126-137). Every edge in range is kept:
+ * agents with more than 32 are redone by a wave each with the edges in device scratch,
+ * so umans_stats.obstacle_overflow_agents (edges in range not kept) stays 0. */
 int umans_set_obstacles(umans_ctx* ctx, int n_polygons, const int32_t* counts, const float* points);
 
 /* CSV trajectory output: CrowdSimulator::StartCSVOutput (crowdSimulator.cpp:37-51) and

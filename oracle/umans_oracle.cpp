@@ -974,7 +974,8 @@ int step_impl(const oracle_world& w, const oracle_policy* pols, int npol, int n,
 		a.max_speed = in.max_speed[i];
 		a.max_accel = in.max_accel[i];
 		a.policy = in.policy[i];
-		a.rng = in.rng ? in.rng[i] : 1u;
+		// no state given: the engine as Agent::Agent seeds it, with the ID (agent.cpp:38)
+		a.rng = in.rng ? in.rng[i] : oracle_rng_seed((uint32_t)in.id[i]);
 		if (a.policy < 0 || a.policy >= npol) return -2;
 	}
 	// Agent::ComputePreferredVelocity (agent.cpp:52-59, 99-102)

@@ -51,14 +51,15 @@ def test_list_capacity_tiers_do_not_change_results(case, world_type):
 
 @pytest.mark.parametrize("world_type", [0, 1])
 def test_ranked_lists_beyond_their_capacity(world_type):
-    """Agents with more than 128 neighbours in range leave the ranked lists for the
-    128-entry sorted insertion, which keeps the nearest 128: the same lists and state as
-    a single sorted-insertion launch, and the overflow counted once."""
+    """With a cap of 128, agents with more than 128 neighbours in range leave the ranked
+    lists for the 128-entry sorted insertion, which keeps the nearest 128: the same lists
+    and state as a single sorted-insertion launch, and the overflow counted once. (With no
+    cap such lists are kept whole: tests/test_gpu_dense.py.)"""
     def run(list_capacity):
         method, costs, samp, relax = CASES["powerlaw_sampling"]
         sc = S.random_cloud(2200, 25.0, [None], world_type=world_type, seed=32)
         w = gpu.World(world_type, 25.0, 25.0, sc.dt, [pack_policy(method, costs, sampling=samp,
-                                                                   relaxation_time=relax)])
+                                                                   relaxation_time=relax)], max_neighbors=128)
         w.upload(sc.agents)
         w.set_tuning(1.0, 0, list_capacity)
         w.record_neighbors(True)

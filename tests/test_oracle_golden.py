@@ -88,3 +88,20 @@ def test_obstacle_fixtures_depend_on_obstacles(name):
     a["rng"] = np.array([O.rng_seed(i) for i in agents["id"]], np.uint32)
     out = O.step(wd, packed, a, precision=0)
     assert (out["vel"] != d["state_vel"][1]).any(axis=1).sum() >= 3
+
+
+@pytest.mark.parametrize("name", [n for n in NAMES if "random" in n])
+def test_oracle_default_engine_state_is_the_reference_seed(name):
+    """Given no engine state, the oracle seeds each agent's engine with its ID, as
+    Agent::Agent does (agent.cpp:38): the reference's first step of a random-sampling
+    fixture is reproduced bit for bit with no rng array, and not with a planted
+    state of 1 for every agent."""
+    d, agents, packed, world, w, h, dt = load(name)
+    wd = O.make_world(world, w, h, dt, 0, edges=d.get("obstacle_edges"))
+    a = state_at(agents, d, 0)
+    a.pop("rng", None)
+    out = O.step(wd, packed, a, precision=0)
+    np.testing.assert_array_equal(out["vel"], d["state_vel"][1])
+    a["rng"] = np.ones(len(a["id"]), np.uint32)
+    planted = O.step(wd, packed, a, precision=0)
+    assert not np.array_equal(planted["vel"], d["state_vel"][1])

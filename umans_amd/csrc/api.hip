@@ -25,6 +25,8 @@ namespace {
 constexpr int kMaxPolicies = 1 << kPolicyBits;
 constexpr int kMaxWorlds = (1 << (31 - kPolicyBits)) - 1;
 constexpr int kUncappedMax = 128;
+constexpr int kRecordUncapped = 512;   // neighbour-record entries per agent with no cap
+constexpr int kHugeBlocks = 256;       // blocks (one wave each) of the very-long-list launch
 
 template <typename T> hipError_t dalloc(T** p, size_t count) {
 	return hipMalloc((void**)p, std::max<size_t>(count, 1) * sizeof(T));
@@ -175,7 +177,16 @@ struct umans_ctx {
 	// list-capacity tiers (uncapped and large caps): agents whose list outgrew a tier,
 	// per tier, in their class's range; and the counts per class and tier
 	int* ovf[2] = {nullptr, nullptr};
-	unsigned* ovf_count = nullptr;            // [NUM_CLS][2]
+	unsigned* ovf_count = nullptr;            // [NUM_CLS][2], then the count of the very long list
+	// very long lists (k_step_huge): the agents handed on, and per block of the launch its
+	// obstacle edges and ORCA half-planes; used with no cap and with obstacles
+	bool uncapped = false;                    // max_neighbors <= 0: every neighbour in range
+	bool huge_enabled = false;
+	int* huge_list = nullptr;
+	float4* huge_edges = nullptr;
+	Line* huge_lines = nullptr;
+	int huge_edge_cap = 0, huge_alloc = 0;
+	unsigned* dflags = nullptr;               // sticky error bits (kFlag*)
 	int tier_first = 3;                       // 0: tiers of 32, 64, 128 entries; 1: of 64, 128; 2: a single
 	                                          // 128-entry launch; 3: long lists collected and ranked per
 	                                          // wave (autotuned; lists longer than 32 only)
@@ -352,7 +363,10 @@ struct umans_ctx {
 		free_adjoint();
 		X.release(); Y.release(); B.release();
 		(void)hipFree(cell_of); (void)hipFree(ovf[0]); (void)hipFree(ovf[1]); (void)hipFree(class_list);
-		(void)hipFree(long_key); (void)hipFree(long_u);
+		(void)hipFree(long_key); (void)hipFree(long_u); (void)hipFree(huge_list);
+		huge_list = nullptr;
+		huge_alloc = 0;
+		huge_enabled = false;
 		ovf[0] = ovf[1] = nullptr;
 		class_list = nullptr;
 		long_key = nullptr;
@@ -585,6 +599,61 @@ int setup_worlds(umans_ctx* c, const std::vector<int>& per_world) {
 	              "world upload");
 }
 
+// Entries per agent of the neighbour record: the list capacity, or with no cap 512 (the
+// count is always the whole list's; longer lists keep their nearest 512 in the record).
+int rec_stride_of(const umans_ctx* c) { return c->uncapped ? kRecordUncapped : kmax_bucket(c->kcap); }
+
+// Buffers of the very-long-list launch (k_step_huge): with no cap, or with obstacles.
+// Allocated outside any graph capture (agent upload, obstacle change); a step without
+// them in place fails.
+int ensure_huge(umans_ctx* c) {
+	c->huge_enabled = false;
+	if (!c->uncapped && c->n_edges == 0) return UMANS_OK;
+	int rc;
+	if (c->huge_alloc < c->cap) {
+		(void)hipFree(c->huge_list);
+		c->huge_list = nullptr;
+		c->huge_alloc = 0;
+		if ((rc = c->hip(dalloc(&c->huge_list, std::max(c->cap, 1)), "alloc long lists"))) return rc;
+		c->huge_alloc = c->cap;
+	}
+	if (!c->huge_lines &&
+	    (rc = c->hip(dalloc(&c->huge_lines, (size_t)kHugeBlocks * 2 * kHugeMax), "alloc long lists"))) return rc;
+	// an edge is in range of an agent at most once per periodic image (toric: 5 images)
+	bool toric = false;
+	for (const umans_world_desc& d : c->worlds) toric = toric || d.world_type == 1;
+	const int edge_cap = std::max(1, (toric ? 5 : 1) * c->n_edges);
+	if (c->huge_edge_cap < edge_cap) {
+		(void)hipFree(c->huge_edges);
+		c->huge_edges = nullptr;
+		c->huge_edge_cap = 0;
+		if ((rc = c->hip(dalloc(&c->huge_edges, (size_t)kHugeBlocks * edge_cap), "alloc long lists"))) return rc;
+		c->huge_edge_cap = edge_cap;
+	}
+	c->huge_enabled = c->cap > 0;
+	return UMANS_OK;
+}
+
+// The sticky error bits the kernels set since the last call (the stream is synchronised);
+// a set bit is cleared once reported. kFlagListTooLong: an agent of the very-long-list
+// launch (no cap, or more than 32 obstacle edges in range) had more than kHugeMax
+// neighbours in range and its list kept kHugeMax of them, not the nearest.
+// kFlagRawListTooLong: an adjoint segment met a list longer than the adjoint keeps.
+int take_flags(umans_ctx* c, unsigned mask) {
+	unsigned h = 0;
+	int rc;
+	if ((rc = c->hip(hipMemcpyAsync(&h, c->dflags, sizeof(unsigned), hipMemcpyDeviceToHost, c->stream), "flags"))) return rc;
+	if ((rc = c->hip(hipStreamSynchronize(c->stream), "flags"))) return rc;
+	if (!(h & mask)) return UMANS_OK;
+	const unsigned keep = h & ~mask;
+	if ((rc = c->hip(hipMemcpy(c->dflags, &keep, sizeof(unsigned), hipMemcpyHostToDevice), "flags"))) return rc;
+	if (h & mask & kFlagListTooLong)
+		return c->fail(UMANS_ETRUNC, "an agent had more than " + std::to_string(kHugeMax) +
+		                                 " neighbours in range: its list was truncated (results differ from the reference's)");
+	return c->fail(UMANS_ETRUNC, "an agent had more than " + std::to_string(kUncappedMax) +
+	                                 " neighbours in range: the adjoint keeps " + std::to_string(kUncappedMax));
+}
+
 // Agents per policy class from the uploaded policies (host copy); the classes present
 // get consecutive slots of the sort key, so class q occupies sorted indices
 // [cls_off[q], cls_off[q] + cls_count[q]).
@@ -762,7 +831,7 @@ int enqueue_step(umans_ctx* c, bool events, const State* ysave = nullptr, unsign
 		hipLaunchKernelGGL(k_fill, dim3(ceil_div(nb + 1, 256)), dim3(256), 0, st, c->scan_sums, nb + 1, 0u);
 	constexpr int stat_words = 2 * kStatCopies * kStatLine;
 	hipLaunchKernelGGL(k_scan_lookback, dim3(nb), dim3(kScanThreads), 0, st, c->cell_count, c->cell_start, c->scan_sums,
-	                   c->scan_sums + 1, scan_n, nb, (unsigned*)c->dstats, stat_words, c->ovf_count, 2 * NUM_CLS,
+	                   c->scan_sums + 1, scan_n, nb, (unsigned*)c->dstats, stat_words, c->ovf_count, 2 * NUM_CLS + 1,
 	                   c->scan_spin_limit);
 	// one class slot: the per-cell starts are the class-major ones; with several the
 	// reorder also writes them
@@ -802,6 +871,14 @@ int enqueue_step(umans_ctx* c, bool events, const State* ysave = nullptr, unsign
 	a.edges = c->d_edges;
 	a.n_edges = c->n_edges;
 	a.tier_final = 1;
+	a.flags = c->dflags;
+	a.uncapped = c->uncapped ? 1 : 0;
+	const bool huge = c->uncapped || c->n_edges > 0;
+	if (huge) {
+		if (!c->huge_enabled || c->huge_alloc < n) return c->fail(UMANS_ESTATE, "long-list buffers missing");
+		a.huge_list = c->huge_list;
+		a.huge_count = c->ovf_count + 2 * NUM_CLS;
+	}
 	if (fuse) {
 		a.next_count = c->cell_count;
 		a.next_cell_of = c->cell_of;
@@ -845,11 +922,13 @@ int enqueue_step(umans_ctx* c, bool events, const State* ysave = nullptr, unsign
 			al.ovf_count = c->ovf_count + 2 * q;
 			al.ovf_base = c->cls_off[q];
 			launch_step(q, 0, 64, al, sq);
-			StepArgs af = aq;
-			af.agent_list = c->ovf[0];
-			af.agent_count = c->ovf_count + 2 * q;
-			af.tier_final = 1;
-			launch_step(q, 128, 64, af, sq);
+			if (!c->uncapped) {           // with no cap the long lists went to k_step_huge
+				StepArgs af = aq;
+				af.agent_list = c->ovf[0];
+				af.agent_count = c->ovf_count + 2 * q;
+				af.tier_final = 1;
+				launch_step(q, 128, 64, af, sq);
+			}
 		} else {
 			// Long lists (no cap, or a cap above 32): a first launch with 32-entry lists
 			// (4x the occupancy of 128) hands the agents whose list is longer to a 64-entry
@@ -890,6 +969,11 @@ int enqueue_step(umans_ctx* c, bool events, const State* ysave = nullptr, unsign
 			}
 			first = false;
 		}
+	}
+	// the agents every class kernel handed on: their count is final once all have joined
+	if (huge) {
+		HugeArgs H{a, c->d_cls_slot, c->total_cells, c->huge_edges, c->huge_edge_cap, c->huge_lines};
+		hipLaunchKernelGGL(k_step_huge, dim3(kHugeBlocks), dim3(64), 0, st, H);
 	}
 	if (e2) {
 		if (int rc = c->hip(hipEventRecord(e2, st), "profile event")) return rc;
@@ -978,14 +1062,14 @@ int grow_agents(umans_ctx* c, int n, int keep) {
 	if ((rc = c->hip(dalloc(&c->ovf[1], cap), "alloc grid"))) return rc;
 	c->cap = cap;
 	if (rec) {
-		const int stride = kmax_bucket(c->kcap);
+		const int stride = rec_stride_of(c);
 		if ((rc = c->hip(dalloc(&c->rec_count, cap), "alloc record"))) return rc;
 		if ((rc = c->hip(dalloc(&c->rec_id, (size_t)cap * stride), "alloc record"))) return rc;
 		if ((rc = c->hip(dalloc(&c->rec_d2, (size_t)cap * stride), "alloc record"))) return rc;
 		if ((rc = c->hip(dalloc(&c->rec_seg, (size_t)cap * stride), "alloc record"))) return rc;
 		c->rec_stride = stride;
 	}
-	return UMANS_OK;
+	return ensure_huge(c);
 }
 
 int upload_remove_flags(umans_ctx* c) {
@@ -1247,6 +1331,7 @@ int create_impl(const umans_world_desc* descs, int nworlds, umans_ctx** out) {
 	for (int w = 0; w < nworlds; ++w)
 		if (check_world(descs[w])) return UMANS_EINVAL;
 	const int kcap = descs[0].max_neighbors > 0 ? descs[0].max_neighbors : kUncappedMax;
+	const bool uncapped = descs[0].max_neighbors <= 0;
 	if (kcap > kUncappedMax) return UMANS_EINVAL;
 	if (descs[0].block_size > 0 && !block_supported(kmax_bucket(kcap), descs[0].block_size)) return UMANS_EINVAL;
 	umans_ctx* c = new umans_ctx();
@@ -1254,6 +1339,7 @@ int create_impl(const umans_world_desc* descs, int nworlds, umans_ctx** out) {
 	c->worlds.assign(descs, descs + nworlds);
 	c->device = descs[0].device;
 	c->kcap = kcap;
+	c->uncapped = uncapped;
 	c->block = std::max(descs[0].block_size, 0);
 	bool streams_ok = hipSetDevice(c->device) == hipSuccess &&
 	                  hipEventCreateWithFlags(&c->fork_ev, hipEventDisableTiming) == hipSuccess;
@@ -1264,7 +1350,9 @@ int create_impl(const umans_world_desc* descs, int nworlds, umans_ctx** out) {
 	    dalloc(&c->dgrids, nworlds) != hipSuccess || dalloc(&c->dgrids_next, nworlds) != hipSuccess || dalloc(&c->dworlds, nworlds) != hipSuccess ||
 	    dalloc(&c->dbbox, (size_t)kBoxCopies * kBoxLine * nworlds) != hipSuccess ||
 	    dalloc(&c->dstats, (size_t)kStatCopies * kStatLine) != hipSuccess ||
-	    dalloc(&c->ovf_count, 2 * NUM_CLS) != hipSuccess || dalloc(&c->dpol, kMaxPolicies) != hipSuccess ||
+	    dalloc(&c->ovf_count, 2 * NUM_CLS + 1) != hipSuccess || dalloc(&c->dpol, kMaxPolicies) != hipSuccess ||
+	    dalloc(&c->dflags, 32) != hipSuccess ||
+	    hipMemsetAsync(c->dflags, 0, 32 * sizeof(unsigned), c->stream) != hipSuccess ||
 	    dalloc(&c->d_cls_slot, NUM_CLS) != hipSuccess ||
 	    // the counters read before the first step: zero
 	    hipMemsetAsync(c->dstats, 0, (size_t)kStatCopies * kStatLine * sizeof(unsigned long long), c->stream) != hipSuccess) {
@@ -1296,7 +1384,8 @@ void umans_destroy(umans_ctx* c) {
 	c->free_pop_buffers();
 	c->free_agents();
 	(void)hipFree(c->dgrids); (void)hipFree(c->dgrids_next); (void)hipFree(c->dworlds); (void)hipFree(c->dbbox); (void)hipFree(c->dstats); (void)hipFree(c->dpol);
-	(void)hipFree(c->d_edges); (void)hipFree(c->ovf_count);
+	(void)hipFree(c->d_edges); (void)hipFree(c->ovf_count); (void)hipFree(c->dflags);
+	(void)hipFree(c->huge_edges); (void)hipFree(c->huge_lines);
 	(void)hipFree(c->d_cls_slot);
 	(void)hipFree(c->cell_count); (void)hipFree(c->cell_start); (void)hipFree(c->cell_first); (void)hipFree(c->scan_sums);
 	c->free_events();
@@ -1399,6 +1488,7 @@ int umans_upload(umans_ctx* c, const umans_agents* a, int n) {
 		if ((rc = c->hip(dalloc(&c->ovf[0], n), "alloc grid"))) return rc;
 		if ((rc = c->hip(dalloc(&c->ovf[1], n), "alloc grid"))) return rc;
 		c->cap = n;
+		if ((rc = ensure_huge(c))) return rc;
 	}
 	c->n = n;
 	c->h_policy.assign(a->policy, a->policy + n);
@@ -1438,7 +1528,7 @@ int umans_set_record_neighbors(umans_ctx* c, int enable) {
 	if (enable && !c->rec_count) {
 		if (c->cap <= 0) return c->fail(UMANS_ESTATE, "upload agents before enabling the neighbour record");
 		if (const int rc_dev = c->hip(hipSetDevice(c->device), "set device")) return rc_dev;
-		const int stride = kmax_bucket(c->kcap);
+		const int stride = rec_stride_of(c);
 		int rc;
 		if ((rc = c->hip(dalloc(&c->rec_count, c->cap), "alloc record"))) return rc;
 		if ((rc = c->hip(dalloc(&c->rec_id, (size_t)c->cap * stride), "alloc record"))) return rc;
@@ -1679,7 +1769,8 @@ int umans_get_profile(umans_ctx* c, umans_profile* out) {
 int umans_synchronize(umans_ctx* c) {
 	if (!c) return UMANS_EINVAL;
 	if (const int rc_dev = c->hip(hipSetDevice(c->device), "set device")) return rc_dev;
-	return c->hip(hipStreamSynchronize(c->stream), "synchronize");
+	if (const int rc = c->hip(hipStreamSynchronize(c->stream), "synchronize")) return rc;
+	return take_flags(c, kFlagListTooLong);
 }
 
 int umans_get_stats(umans_ctx* c, umans_stats* out) {
@@ -1702,7 +1793,7 @@ int umans_get_stats(umans_ctx* c, umans_stats* out) {
 	out->max_list = (int64_t)h[1];
 	out->sim_time = c->sim_time;
 	out->obstacle_overflow_agents = (int64_t)h[2];
-	return UMANS_OK;
+	return take_flags(c, kFlagListTooLong);
 }
 
 int umans_download(umans_ctx* c, umans_agents* a) {
@@ -1740,7 +1831,7 @@ int umans_download(umans_ctx* c, umans_agents* a) {
 		if (a->max_speed) a->max_speed[i] = prm[k].z;
 		if (a->max_accel) a->max_accel[i] = prm[k].w;
 	}
-	return UMANS_OK;
+	return take_flags(c, kFlagListTooLong);
 }
 
 int umans_dump_neighbors(umans_ctx* c, int stride, int32_t* count, int32_t* ids, double* d2, int32_t* seg) {
@@ -1918,7 +2009,7 @@ int umans_adjoint_backward(umans_ctx* c, float* d_params, float* d_pos0, float* 
 		}
 	}
 	c->adj_clamps_ready = true;
-	return UMANS_OK;
+	return take_flags(c, kFlagListTooLong | kFlagRawListTooLong);
 }
 
 int umans_adjoint_clamps(umans_ctx* c, uint8_t* flags) {
@@ -2143,6 +2234,7 @@ int umans_set_obstacles(umans_ctx* c, int n_polygons, const int32_t* counts, con
 	}
 	c->drop_graph();
 	c->adj_ready = false;
+	if ((rc = ensure_huge(c))) return rc;
 	return update_classes(c);
 }
 

@@ -188,9 +188,9 @@ __global__ __launch_bounds__(256) void k_count(const float2* pos, const int4* me
 
 // k_count's work for agent k of the next step, done by the step kernel that computed its
 // new position (same key: the class slot of the launch, the world's fixed grid).
-UD void count_next(const StepArgs& A, const WorldInfo& wi, int wix, int k, float2 p) {
+UD void count_next(const StepArgs& A, const WorldInfo& wi, int wix, int k, float2 p, int key_base) {
 	const GridInfo g = A.next_grids[wix];
-	const int c = A.next_key_base + wi.cell_offset + bin_axis(p.y, g.oy, g.invy, g.ncy) * g.ncx +
+	const int c = key_base + wi.cell_offset + bin_axis(p.y, g.oy, g.invy, g.ncy) * g.ncx +
 	              bin_axis(p.x, g.ox, g.invx, g.ncx);
 	A.next_cell_of[k] = (unsigned)c;
 	atomicAdd(&A.next_count[c], 1u);
@@ -748,9 +748,9 @@ UD void rank_long_lists(const StepArgs& A, int row, int n_row, int kcap, V2 p, c
 // WorldBase::computeNeighboringObstacles_Flat (worldBase.cpp:62-77) for each query of
 // WorldInfinite / WorldToric::ComputeNeighbors (worldToric.cpp:32-79): the edges whose
 // distance to float(position + displacement) is within the range, in edge order, moved
-// back by the displacement. Returns the number kept (at most kMaxObstacleEdges); `total`
-// counts them all.
-UD int find_obstacles(const StepArgs& A, const WorldInfo& wi, V2 p, float range, float4* out, int& total) {
+// back by the displacement. Returns the number kept (at most cap); `total` counts them all.
+UD int find_obstacles(const StepArgs& A, const WorldInfo& wi, V2 p, float range, float4* out, int& total,
+                      int cap = kMaxObstacleEdges) {
 	const float r2 = range * range;
 	int n = 0;
 	total = 0;
@@ -768,7 +768,7 @@ UD int find_obstacles(const StepArgs& A, const WorldInfo& wi, V2 p, float range,
 			const float4 E = A.edges[e];
 			if (!(dist_line_sq(q, seg_a(E), seg_b(E), true) <= r2)) continue;
 			++total;
-			if (n < kMaxObstacleEdges) out[n++] = make_float4(E.x - dx, E.y - dy, E.z - dx, E.w - dy);
+			if (n < cap) out[n++] = make_float4(E.x - dx, E.y - dy, E.z - dx, E.w - dy);
 		}
 	}
 	return n;
@@ -935,6 +935,172 @@ UD int xcd_tile(int b, int nb) {
 #endif
 }
 
+// The rest of one DoStep for agent k once its neighbour list is known (entry e at
+// lst[e * lstride], cnt of them; obstacle edges obs[0 .. n_obs)): the neighbour record,
+// Policy::ComputeNewVelocity, contact forces, integration, the outputs and the next
+// step's count. BIG: the half-planes of ORCA costs live in caller storage (big_lines,
+// big_proj, one per list entry) instead of private arrays of KMAX entries.
+template <int KMAX, int CLS, bool BIG>
+UD void finish_agent(const StepArgs& A, int k, int4 meta, int pi, int wix, const WorldInfo& wi, const Self& a,
+                     const unsigned* lst, int lstride, int cnt, const float4* obs, int n_obs, Line* big_lines,
+                     Line* big_proj, int next_key_base) {
+	constexpr bool kLong = KMAX == 0;
+	constexpr int KL = kLong ? kLongCap : KMAX;
+	const State& S = A.in;
+	const DevPolicy& P = A.pol[pi];
+	if (A.rec_count) {
+		A.rec_count[k] = cnt;
+		const int m = cnt < A.rec_stride ? cnt : A.rec_stride;
+		for (int e = 0; e < m; ++e) {
+			const unsigned u = lst[e * lstride];
+			const long long o = (long long)k * A.rec_stride + e;
+			A.rec_id[o] = S.meta[u & kIndexMask].x;
+			A.rec_d2[o] = entry_d2(S, a.pos, wi, u);
+			A.rec_seg[o] = (int)(u >> kSegShift);
+		}
+	}
+	if (A.raw_nb) {
+		// the adjoint's copy of the list holds raw_stride entries
+		const int m = cnt < A.raw_stride ? cnt : A.raw_stride;
+		if (cnt > m) atomicOr(A.flags, kFlagRawListTooLong);
+		A.raw_count[k] = m;
+		for (int e = 0; e < m; ++e) A.raw_nb[(size_t)k * A.raw_stride + e] = lst[e * lstride];
+	}
+	const NbList L{lst, lstride, cnt, S.pos, S.vel, S.prm, S.pref, wi.W, wi.H, a.pos, obs, n_obs};
+	uint32_t rng = (uint32_t)meta.w;
+
+	// Force classes: one pass over the neighbours gathers each neighbour once for the
+	// pair forces of every cost (ObjectInteractionForces::ComputeForce,
+	// ObjectInteractionForces.cpp:26-48) and for the contact forces (policy.cpp:107-141),
+	// each sum still in list order.
+	constexpr bool kFused = (CLS == CLS_FORCE_GRAD || CLS == CLS_FORCE_SAMPLE);
+	V2 fsum[kFused ? 4 : 1];
+	V2 contact{0.0f, 0.0f};
+	if constexpr (kFused) {
+		float rsq[4];
+#pragma unroll
+		for (int c = 0; c < 4; ++c) {
+			fsum[c] = V2{0.0f, 0.0f};
+			rsq[c] = P.range[c] * P.range[c];
+		}
+		const bool do_contact = P.contact_scale > 0.0f;
+		for (int e = 0; e < cnt; ++e) {
+			const Nb n = L.get(e);
+#pragma unroll
+			for (int c = 0; c < 4; ++c) {
+				if (c < P.n_cost && P.kind[c] != KIND_GOAL_REACHING && n.dsq <= rsq[c]) {
+					const V2 f = (P.kind[c] == KIND_POWER_LAW)
+					                 ? power_law_pair(P.params[c], a, n)
+					                 : social_force_pair(P.params[c], a, n, L.pref_of(n.j), P.view_cos);
+					fsum[c] = fsum[c] + f;
+				}
+			}
+			if (do_contact) {
+				const V2 diff = a.pos - n.pos;
+				const float inter = a.radius + n.radius - mag(diff);
+				if (inter > 0.0f) contact = contact + normalized(diff) * (P.contact_scale * inter);
+			}
+		}
+	}
+
+	// Policy::ComputeNewVelocity (policy.cpp:43-76)
+	V2 next{0.0f, 0.0f};
+	if (!is_zero(a.pref)) {
+		Prep<KMAX, CLS> pr;
+		pr.n_lines = 0;
+		if constexpr (CLS == CLS_ORCA) {
+			Line proj_priv[BIG ? 1 : KL];
+			const Lines proj{BIG ? big_proj : proj_priv, 1};
+			V2 best;
+			if constexpr (!kLong && KMAX <= 16) {
+				// half-planes in registers; only the rare projected lines of
+				// linearProgram3 go to private memory
+				RegLines<KMAX> R;
+				orca_lines_reg<KMAX>(P.params[0], P.range[0], a, L, R);
+				best = orca_solve_reg<KMAX>(R, a, proj);
+			} else {
+				Line lines_priv[BIG ? 1 : KL];
+				pr.lines = Lines{BIG ? big_lines : lines_priv, 1};
+				pr.n_lines = orca_lines(P.params[0], P.range[0], a, L, pr.lines);
+				best = orca_solve(pr.lines, pr.n_lines, a, proj);
+			}
+			const V2 acc = clamp_vec((best - a.vel) / smax(a.relaxation, a.dt), a.max_accel);
+			next = clamp_vec(a.vel + acc * a.dt, a.max_speed);
+		} else {
+			Line lines_priv[CLS == CLS_GENERIC && !BIG ? KL : 1];
+			Line proj_priv[CLS == CLS_GENERIC && !BIG ? KL : 1];
+			pr.lines = Lines{BIG ? big_lines : lines_priv, 1};
+			for_costs<CLS>(P, [&](int c) {
+				const int kind = P.kind[c];
+				if constexpr (kFused) {
+					// ForceBasedFunction::ComputeTargetVelocity (ForceBasedFunction.cpp:49-52)
+					const V2 f = (kind == KIND_GOAL_REACHING) ? (a.pref - a.vel) / smax(a.relaxation, a.dt)
+					                                          : fsum[c] + V2{0.0f, 0.0f};
+					pr.target[c] = a.vel + f / P.params[c][0] * a.dt;
+				} else if (CLS != CLS_GENERIC || is_force_based(kind)) pr.target[c] = target_velocity(P, c, a, L);
+				else if (kind == KIND_ORCA) pr.n_lines = orca_lines(P.params[c], P.range[c], a, L, pr.lines);
+				else if (kind == KIND_KARAMOUZAS)
+					pr.ttcp[c] = time_to_first_collision(L, a.pos, a.pref, a.radius, P.range[c], true);
+			});
+			V2 acc;
+			if (CLS == CLS_FORCE_GRAD || (CLS == CLS_GENERIC && P.mode == MODE_GRADIENT)) {
+				if (CLS == CLS_FORCE_GRAD && P.mode == MODE_GLOBAL_FORCE) {
+					acc = (pr.target[0] - a.vel) / smax(a.relaxation, a.dt);
+				} else {
+					V2 tg{0.0f, 0.0f};
+					for_costs<CLS>(P, [&](int c) { tg = tg + P.coeff[c] * gradient_of<KMAX, CLS>(P, c, a, L, pr, a.vel); });
+					acc = -1.0f * tg;
+				}
+			} else {
+				V2 best;
+				if (CLS == CLS_GENERIC && P.mode == MODE_GLOBAL_ORCA)
+					best = orca_solve(pr.lines, pr.n_lines, a, Lines{BIG ? big_proj : proj_priv, 1});
+				else if (CLS == CLS_GENERIC && P.mode == MODE_GLOBAL_FORCE) best = pr.target[0];
+				else best = sample_best<KMAX, CLS>(P, a, L, pr, rng);
+				acc = (best - a.vel) / smax(a.relaxation, a.dt);
+			}
+			acc = clamp_vec(acc, a.max_accel);
+			next = clamp_vec(a.vel + acc * a.dt, a.max_speed);
+		}
+	}
+
+	// Policy::ComputeContactForces (policy.cpp:107-141)
+	if (!kFused && P.contact_scale > 0.0f) {
+		for (int e = 0; e < cnt; ++e) {
+			const Nb n = L.get(e);
+			const V2 diff = a.pos - n.pos;
+			const float inter = a.radius + n.radius - mag(diff);
+			if (inter > 0.0f) contact = contact + normalized(diff) * (P.contact_scale * inter);
+		}
+		for (int e = 0; e < n_obs; ++e) {
+			const V2 diff = a.pos - nearest_point(a.pos, seg_a(obs[e]), seg_b(obs[e]), true);
+			const float inter = a.radius - mag(diff);
+			if (inter > 0.0f) contact = contact + normalized(diff) * (P.contact_scale * inter);
+		}
+	}
+
+	// Agent::UpdateVelocityAndPosition (agent.cpp:71-93) + toric wrap (worldToric.cpp:81-108)
+	const V2 vel = next + ((contact / 80.0f) * a.dt);
+	V2 view = v2(S.viewdir[k]);
+	const V2 vd = (2.0f * vel + a.pref) / 3.0f;
+	if ((double)sqr_mag(vd) > 0.01) view = normalized(vd);
+	V2 pos = a.pos + (vel * a.dt);
+	if (wi.world_type == 1) {
+		const float hw = 0.5f * wi.W, hh = 0.5f * wi.H;
+		if (pos.x > hw) pos.x -= wi.W; else if (pos.x < -hw) pos.x += wi.W;
+		if (pos.y > hh) pos.y -= wi.H; else if (pos.y < -hh) pos.y += wi.H;
+	}
+	A.out.pos[k] = f2(pos);
+	A.out.vel[k] = f2(vel);
+	A.out.viewdir[k] = f2(view);
+	A.out.goal[k] = S.goal[k];
+	A.out.prm[k] = S.prm[k];
+	A.out.pref[k] = f2(a.pref);
+	meta.w = (int)rng;
+	A.out.meta[k] = meta;
+	if (A.next_count) count_next(A, wi, wix, k, f2(pos), next_key_base);
+}
+
 // One lane per agent of the class's list entries [A.base, A.base + A.n).
 template <int KMAX, int BS, int CLS>
 __global__ __launch_bounds__(BS) void k_step(StepArgs A) {
@@ -991,19 +1157,24 @@ __global__ __launch_bounds__(BS) void k_step(StepArgs A) {
 	// obstacle edges: only the generic class runs when a world has obstacles
 	float4 obs_priv[CLS == CLS_GENERIC ? kMaxObstacleEdges : 1];
 	int n_obs = 0;
+	bool many_edges = false;
 	if constexpr (CLS == CLS_GENERIC) {
 		if (active && A.n_edges > 0) {
 			int obs_total = 0;
 			n_obs = find_obstacles(A, wi, a.pos, A.pol[pi].query_range, obs_priv, obs_total);
-			if (obs_total > n_obs) atomicAdd(&A.stats[stat_copy() * kStatLine + 2], 1ull);
+			many_edges = obs_total > n_obs;
+			if (many_edges && !A.huge_list) atomicAdd(&A.stats[stat_copy() * kStatLine + 2], 1ull);
 		}
 	}
-	// a list that outgrew this launch's capacity is redone by the next launch, which
-	// counts it in the stats
-	const bool handover = kLong ? total > kLongCap : (!A.tier_final && total > KMAX);
+	// a list that outgrew this launch's capacity is redone by the next launch (with a cap
+	// or below 128 entries) or, with no cap, by the long-list launch k_step_huge, as is
+	// an agent with more obstacle edges in range than a lane keeps
+	const bool beyond = kLong ? total > kLongCap : (A.tier_final ? total > KMAX : false);
+	const bool to_huge = active && A.huge_list && ((A.uncapped && beyond) || many_edges);
+	const bool handover = !to_huge && (kLong ? total > kLongCap : (!A.tier_final && total > KMAX));
 	// stats: one atomic per wave
 	{
-		const bool final_here = kLong ? !handover : A.tier_final != 0;
+		const bool final_here = !to_huge && (kLong ? !handover : A.tier_final != 0);
 		const unsigned long long ov = __ballot(active && final_here && total > A.kcap);
 		const int mx = wave_max_i(total);
 		if ((tid & 63) == 0) {
@@ -1014,162 +1185,168 @@ __global__ __launch_bounds__(BS) void k_step(StepArgs A) {
 		}
 	}
 	if (!active) return;
+	if (to_huge) {
+		A.huge_list[atomicAdd(A.huge_count, 1u)] = k;
+		return;
+	}
 	if (handover) {
 		A.ovf_list[A.ovf_base + atomicAdd(A.ovf_count, 1u)] = k;
 		return;
 	}
-	const DevPolicy& P = A.pol[pi];
 	// the lane's sorted list: its LDS column, or its row of the long-list buffer
 	const unsigned* lst = kLong ? A.long_u + (base + t) : s_nb + tid;
 	const int lstride = kLong ? A.long_stride : BS;
-	if (A.rec_count) {
-		A.rec_count[k] = cnt;
-		const int m = cnt < A.rec_stride ? cnt : A.rec_stride;
-		for (int e = 0; e < m; ++e) {
-			const unsigned u = lst[e * lstride];
-			const long long o = (long long)k * A.rec_stride + e;
-			A.rec_id[o] = S.meta[u & kIndexMask].x;
-			A.rec_d2[o] = entry_d2(S, a.pos, wi, u);
-			A.rec_seg[o] = (int)(u >> kSegShift);
-		}
-	}
-	if (A.raw_nb) {
-		A.raw_count[k] = cnt;
-		for (int e = 0; e < cnt; ++e) A.raw_nb[(size_t)k * A.raw_stride + e] = lst[e * lstride];
-	}
-	const NbList L{lst, lstride, cnt, S.pos, S.vel, S.prm, S.pref, wi.W, wi.H, a.pos, obs_priv, n_obs};
-	uint32_t rng = (uint32_t)meta.w;
+	finish_agent<KMAX, CLS, false>(A, k, meta, pi, wix, wi, a, lst, lstride, cnt, obs_priv, n_obs, nullptr, nullptr,
+	                               A.next_key_base);
+}
 
-	// Force classes: one pass over the neighbours gathers each neighbour once for the
-	// pair forces of every cost (ObjectInteractionForces::ComputeForce,
-	// ObjectInteractionForces.cpp:26-48) and for the contact forces (policy.cpp:107-141),
-	// each sum still in list order.
-	constexpr bool kFused = (CLS == CLS_FORCE_GRAD || CLS == CLS_FORCE_SAMPLE);
-	V2 fsum[kFused ? 4 : 1];
-	V2 contact{0.0f, 0.0f};
-	if constexpr (kFused) {
-		float rsq[4];
-#pragma unroll
-		for (int c = 0; c < 4; ++c) {
-			fsum[c] = V2{0.0f, 0.0f};
-			rsq[c] = P.range[c] * P.range[c];
-		}
-		const bool do_contact = P.contact_scale > 0.0f;
-		for (int e = 0; e < cnt; ++e) {
-			const Nb n = L.get(e);
-#pragma unroll
-			for (int c = 0; c < 4; ++c) {
-				if (c < P.n_cost && P.kind[c] != KIND_GOAL_REACHING && n.dsq <= rsq[c]) {
-					const V2 f = (P.kind[c] == KIND_POWER_LAW)
-					                 ? power_law_pair(P.params[c], a, n)
-					                 : social_force_pair(P.params[c], a, n, L.pref_of(n.j), P.view_cos);
-					fsum[c] = fsum[c] + f;
+
+// ----------------------------------------------------------- very long lists
+// The agents k_step hands on: with no cap, those with more than 128 neighbours in range
+// (the reference keeps every neighbour, AgentKDTree.cpp:35-59), and in any world those
+// with more than kMaxObstacleEdges obstacle edges in range (worldBase.cpp:62-77). One
+// wave per agent, in a grid-stride loop over the handed-on agents (launched after every
+// class kernel of the step, so the count is final): the lanes collect the candidates in
+// range into LDS as packed keys (segment | float(d2) | index), sort them with a bitonic
+// network over the next power of two, order the runs of equal floats by their exact
+// doubles and agent IDs, and one lane runs the rest of the step with the generic code
+// path, the half-planes of ORCA costs and the obstacle edges in the block's slice of
+// scratch. Lists longer than kHugeMax are not kept whole: kFlagListTooLong tells the host.
+struct HugeArgs {
+	StepArgs s;
+	const int* cls_slot;     // sort-key slot of each policy class (the next step's keys)
+	int slot_cells;
+	float4* edges;           // edge_cap per block
+	int edge_cap;
+	Line* lines;             // 2 * kHugeMax per block: half-planes, projected lines
+};
+
+// entry (index | segment) u before entry w in the canonical order: segment, exact d2, ID
+UD bool entry_before(const State& S, V2 p, const WorldInfo& wi, unsigned u, unsigned w) {
+	if ((u >> kSegShift) != (w >> kSegShift)) return (u >> kSegShift) < (w >> kSegShift);
+	const double du = entry_d2(S, p, wi, u), dw = entry_d2(S, p, wi, w);
+	if (du != dw) return du < dw;
+	return S.meta[u & kIndexMask].x < S.meta[w & kIndexMask].x;
+}
+
+__global__ __launch_bounds__(64) void k_step_huge(HugeArgs H) {
+	__shared__ unsigned long long s_key[kHugeMax];
+	__shared__ unsigned s_u[kHugeMax];
+	__shared__ unsigned s_cnt;
+	const StepArgs& A = H.s;
+	const State& S = A.in;
+	const int lane = threadIdx.x;
+	const unsigned n_huge = *A.huge_count;
+	float4* edges = H.edges + (size_t)blockIdx.x * H.edge_cap;
+	Line* lines = H.lines + (size_t)blockIdx.x * 2 * kHugeMax;
+	for (unsigned h = blockIdx.x; h < n_huge; h += gridDim.x) {
+		const int k = A.huge_list[h];
+		const int4 meta = S.meta[k];
+		const int pi = meta.y & kPolicyMask, wix = meta.y >> kPolicyBits;
+		const WorldInfo wi = A.worlds[wix];
+		const GridInfo g = A.grids[wix];
+		const DevPolicy& P = A.pol[pi];
+		const float4 prm = S.prm[k];
+		Self a;
+		a.pos = v2(S.pos[k]);
+		a.vel = v2(S.vel[k]);
+		a.pref = v2(S.pref[k]);
+		a.radius = prm.x; a.pref_speed = prm.y; a.max_speed = prm.z; a.max_accel = prm.w;
+		a.dt = wi.dt;
+		a.relaxation = P.relaxation;
+		const float range = P.query_range;
+		// the queries of find_neighbors, the candidates of each row range spread over the lanes
+		if (lane == 0) s_cnt = 0u;
+		__syncthreads();
+		const double r2 = (double)(range * range);
+		const float rs = range * 1.0001f + 1e-3f;
+		for (int s = 0; s < 5; ++s) {
+			float dx = 0.0f, dy = 0.0f;
+			if (s > 0) {
+				if (wi.world_type != 1) break;
+				if (s == 1) { if (!((double)(a.pos.x - range) < -0.5 * (double)wi.W)) continue; dx = wi.W; }
+				else if (s == 2) { if (!((double)(a.pos.x + range) > 0.5 * (double)wi.W)) continue; dx = -wi.W; }
+				else if (s == 3) { if (!((double)(a.pos.y - range) < -0.5 * (double)wi.H)) continue; dy = wi.H; }
+				else { if (!((double)(a.pos.y + range) > 0.5 * (double)wi.H)) continue; dy = -wi.H; }
+			}
+			const V2 q{a.pos.x + dx, a.pos.y + dy};
+			const float m = rs + (fabsf(q.x) + fabsf(q.y)) * 1e-6f;
+			const int cx0 = bin_axis(q.x - m, g.ox, g.invx, g.ncx), cx1 = bin_axis(q.x + m, g.ox, g.invx, g.ncx);
+			const int cy0 = bin_axis(q.y - m, g.oy, g.invy, g.ncy), cy1 = bin_axis(q.y + m, g.oy, g.invy, g.ncy);
+			const double qx = (double)q.x, qy = (double)q.y;
+			for (int cy = cy0; cy <= cy1; ++cy) {
+				const int crow = wi.cell_offset + cy * g.ncx;
+				const unsigned b = A.cell_start[crow + cx0], e = A.cell_start[crow + cx1 + 1];
+				for (unsigned j = b + lane; j < e; j += 64) {
+					const float2 pj = S.pos[j];
+					const double ddx = qx - (double)pj.x;
+					const double ddy = qy - (double)pj.y;
+					const double d2 = ddx * ddx + ddy * ddy;
+					if (!(d2 < r2) || (int)j == k) continue;
+					const unsigned slot = atomicAdd(&s_cnt, 1u);
+					if (slot < (unsigned)kHugeMax) s_key[slot] = long_key(s, d2, j);
 				}
 			}
-			if (do_contact) {
-				const V2 diff = a.pos - n.pos;
-				const float inter = a.radius + n.radius - mag(diff);
-				if (inter > 0.0f) contact = contact + normalized(diff) * (P.contact_scale * inter);
-			}
 		}
-	}
-
-	// Policy::ComputeNewVelocity (policy.cpp:43-76)
-	V2 next{0.0f, 0.0f};
-	if (!is_zero(a.pref)) {
-		Prep<KMAX, CLS> pr;
-		pr.n_lines = 0;
-		if constexpr (CLS == CLS_ORCA) {
-			Line proj_priv[KL];
-			const Lines proj{proj_priv, 1};
-			V2 best;
-			if constexpr (!kLong && KMAX <= 16) {
-				// half-planes in registers; only the rare projected lines of
-				// linearProgram3 go to private memory
-				RegLines<KMAX> R;
-				orca_lines_reg<KMAX>(P.params[0], P.range[0], a, L, R);
-				best = orca_solve_reg<KMAX>(R, a, proj);
-			} else {
-				Line lines_priv[KL];
-				pr.lines = Lines{lines_priv, 1};
-				pr.n_lines = orca_lines(P.params[0], P.range[0], a, L, pr.lines);
-				best = orca_solve(pr.lines, pr.n_lines, a, proj);
-			}
-			const V2 acc = clamp_vec((best - a.vel) / smax(a.relaxation, a.dt), a.max_accel);
-			next = clamp_vec(a.vel + acc * a.dt, a.max_speed);
-		} else {
-			Line lines_priv[CLS == CLS_GENERIC ? KL : 1];
-			Line proj_priv[CLS == CLS_GENERIC ? KL : 1];
-			pr.lines = Lines{lines_priv, 1};
-			for_costs<CLS>(P, [&](int c) {
-				const int kind = P.kind[c];
-				if constexpr (kFused) {
-					// ForceBasedFunction::ComputeTargetVelocity (ForceBasedFunction.cpp:49-52)
-					const V2 f = (kind == KIND_GOAL_REACHING) ? (a.pref - a.vel) / smax(a.relaxation, a.dt)
-					                                          : fsum[c] + V2{0.0f, 0.0f};
-					pr.target[c] = a.vel + f / P.params[c][0] * a.dt;
-				} else if (CLS != CLS_GENERIC || is_force_based(kind)) pr.target[c] = target_velocity(P, c, a, L);
-				else if (kind == KIND_ORCA) pr.n_lines = orca_lines(P.params[c], P.range[c], a, L, pr.lines);
-				else if (kind == KIND_KARAMOUZAS)
-					pr.ttcp[c] = time_to_first_collision(L, a.pos, a.pref, a.radius, P.range[c], true);
-			});
-			V2 acc;
-			if (CLS == CLS_FORCE_GRAD || (CLS == CLS_GENERIC && P.mode == MODE_GRADIENT)) {
-				if (CLS == CLS_FORCE_GRAD && P.mode == MODE_GLOBAL_FORCE) {
-					acc = (pr.target[0] - a.vel) / smax(a.relaxation, a.dt);
-				} else {
-					V2 tg{0.0f, 0.0f};
-					for_costs<CLS>(P, [&](int c) { tg = tg + P.coeff[c] * gradient_of<KMAX, CLS>(P, c, a, L, pr, a.vel); });
-					acc = -1.0f * tg;
+		__syncthreads();
+		const int total = (int)s_cnt;
+		const int n = total < kHugeMax ? total : kHugeMax;
+		int P2 = 64;
+		while (P2 < n) P2 <<= 1;
+		for (int i = n + lane; i < P2; i += 64) s_key[i] = kKeyPad;
+		__syncthreads();
+		// bitonic sort of the packed keys, ascending
+		for (int w = 2; w <= P2; w <<= 1) {
+			for (int j = w >> 1; j > 0; j >>= 1) {
+				for (int i = lane; i < P2; i += 64) {
+					const int ixj = i ^ j;
+					if (ixj > i) {
+						const unsigned long long x = s_key[i], y = s_key[ixj];
+						if ((x > y) == ((i & w) == 0)) { s_key[i] = y; s_key[ixj] = x; }
+					}
 				}
-			} else {
-				V2 best;
-				if (CLS == CLS_GENERIC && P.mode == MODE_GLOBAL_ORCA)
-					best = orca_solve(pr.lines, pr.n_lines, a, Lines{proj_priv, 1});
-				else if (CLS == CLS_GENERIC && P.mode == MODE_GLOBAL_FORCE) best = pr.target[0];
-				else best = sample_best<KMAX, CLS>(P, a, L, pr, rng);
-				acc = (best - a.vel) / smax(a.relaxation, a.dt);
+				__syncthreads();
 			}
-			acc = clamp_vec(acc, a.max_accel);
-			next = clamp_vec(a.vel + acc * a.dt, a.max_speed);
 		}
-	}
-
-	// Policy::ComputeContactForces (policy.cpp:107-141)
-	if (!kFused && P.contact_scale > 0.0f) {
-		for (int e = 0; e < cnt; ++e) {
-			const Nb n = L.get(e);
-			const V2 diff = a.pos - n.pos;
-			const float inter = a.radius + n.radius - mag(diff);
-			if (inter > 0.0f) contact = contact + normalized(diff) * (P.contact_scale * inter);
+		for (int i = lane; i < n; i += 64) s_u[i] = key_entry(s_key[i]);
+		__syncthreads();
+		if (lane == 0) {
+			// runs of equal segment and float(d2) in the canonical order (insertion by exact
+			// d2 and ID; the runs are a few entries long)
+			for (int i = 0; i + 1 < n;) {
+				int r = i + 1;
+				while (r < n && same_float_key(s_key[i], s_key[r])) ++r;
+				for (int x = i + 1; x < r; ++x) {
+					const unsigned u = s_u[x];
+					int y = x;
+					while (y > i && entry_before(S, a.pos, wi, u, s_u[y - 1])) {
+						s_u[y] = s_u[y - 1];
+						--y;
+					}
+					s_u[y] = u;
+				}
+				i = r;
+			}
+			int n_obs = 0;
+			if (A.n_edges > 0) {
+				int obs_total = 0;
+				n_obs = find_obstacles(A, wi, a.pos, range, edges, obs_total, H.edge_cap);
+			}
+			const bool too_long = A.uncapped ? total > kHugeMax : total > A.kcap;
+			unsigned long long* sc = A.stats + stat_copy() * kStatLine;
+			if (too_long) atomicAdd(&sc[0], 1ull);
+			atomicMax(&sc[1], (unsigned long long)total);
+			// the LDS list holds kHugeMax of the candidates, in the order they were found
+			if (total > kHugeMax) atomicOr(A.flags, kFlagListTooLong);
+			const int cnt = A.uncapped ? n : (n < A.kcap ? n : A.kcap);
+			// the next step's key: the slot of the agent's class (with obstacles every
+			// agent is in the generic class)
+			const int cls = A.n_edges > 0 ? (int)CLS_GENERIC : P.cls;
+			finish_agent<0, CLS_GENERIC, true>(A, k, meta, pi, wix, wi, a, s_u, 1, cnt, edges, n_obs, lines,
+			                                   lines + kHugeMax, H.cls_slot[cls] * H.slot_cells);
 		}
-		for (int e = 0; e < n_obs; ++e) {
-			const V2 diff = a.pos - nearest_point(a.pos, seg_a(obs_priv[e]), seg_b(obs_priv[e]), true);
-			const float inter = a.radius - mag(diff);
-			if (inter > 0.0f) contact = contact + normalized(diff) * (P.contact_scale * inter);
-		}
+		__syncthreads();          // the next agent reuses the LDS lists
 	}
-
-	// Agent::UpdateVelocityAndPosition (agent.cpp:71-93) + toric wrap (worldToric.cpp:81-108)
-	const V2 vel = next + ((contact / 80.0f) * a.dt);
-	V2 view = v2(S.viewdir[k]);
-	const V2 vd = (2.0f * vel + a.pref) / 3.0f;
-	if ((double)sqr_mag(vd) > 0.01) view = normalized(vd);
-	V2 pos = a.pos + (vel * a.dt);
-	if (wi.world_type == 1) {
-		const float hw = 0.5f * wi.W, hh = 0.5f * wi.H;
-		if (pos.x > hw) pos.x -= wi.W; else if (pos.x < -hw) pos.x += wi.W;
-		if (pos.y > hh) pos.y -= wi.H; else if (pos.y < -hh) pos.y += wi.H;
-	}
-	A.out.pos[k] = f2(pos);
-	A.out.vel[k] = f2(vel);
-	A.out.viewdir[k] = f2(view);
-	A.out.goal[k] = S.goal[k];
-	A.out.prm[k] = S.prm[k];
-	A.out.pref[k] = f2(a.pref);
-	meta.w = (int)rng;
-	A.out.meta[k] = meta;
-	if (A.next_count) count_next(A, wi, wix, k, f2(pos));
 }
 
 
@@ -1246,8 +1423,11 @@ __global__ __launch_bounds__(64) void k_step_coop(StepArgs A) {
 		a.relaxation = A.pol[pi].relaxation;
 		cnt = find_neighbors<KMAX, BS>(A, wi, g, k, a.pos, A.pol[pi].query_range, s_d2 + tid, s_nb + tid, total);
 	}
+	// with no cap, a list longer than KMAX is redone whole by k_step_huge; the lane still
+	// takes part in the wave's barriers but has no candidates
+	const bool to_huge = active && A.huge_list && A.uncapped && total > KMAX;
 	{
-		const unsigned long long ov = __ballot(active && total > A.kcap);
+		const unsigned long long ov = __ballot(active && !to_huge && total > A.kcap);
 		const int mx = wave_max_i(total);
 		if (tid == 0) {
 			const int c = __popcll(ov);
@@ -1257,7 +1437,7 @@ __global__ __launch_bounds__(64) void k_step_coop(StepArgs A) {
 		}
 	}
 	const DevPolicy& P = A.pol[pi];
-	if (active && A.rec_count) {
+	if (active && !to_huge && A.rec_count) {
 		A.rec_count[k] = cnt;
 		const int m = cnt < A.rec_stride ? cnt : A.rec_stride;
 		for (int e = 0; e < m; ++e) {
@@ -1271,7 +1451,7 @@ __global__ __launch_bounds__(64) void k_step_coop(StepArgs A) {
 	const NbList L{s_nb + tid, BS, cnt, S.pos, S.vel, S.prm, S.pref, wi.W, wi.H, a.pos, nullptr, 0};
 	// Policy::ComputeNewVelocity (policy.cpp:43-76): agents with a zero preferred
 	// velocity keep a zero new velocity and have no candidates
-	const bool sample = active && !is_zero(a.pref);
+	const bool sample = active && !to_huge && !is_zero(a.pref);
 	const int ns = sample ? regular_count(P) : 0;
 	float ttcp[4] = {0.0f, 0.0f, 0.0f, 0.0f};
 	if (sample) {
@@ -1350,6 +1530,10 @@ __global__ __launch_bounds__(64) void k_step_coop(StepArgs A) {
 	if (ag >= 0 && local != ~0ull) atomicMin(&s_key[ag], local);
 	__syncthreads();
 	if (!active) return;
+	if (to_huge) {
+		A.huge_list[atomicAdd(A.huge_count, 1u)] = k;
+		return;
+	}
 
 	V2 next{0.0f, 0.0f};
 	if (sample) {
@@ -1397,7 +1581,7 @@ __global__ __launch_bounds__(64) void k_step_coop(StepArgs A) {
 	A.out.prm[k] = S.prm[k];
 	A.out.pref[k] = f2(a.pref);
 	A.out.meta[k] = meta;
-	if (A.next_count) count_next(A, wi, meta.y >> kPolicyBits, k, f2(pos));
+	if (A.next_count) count_next(A, wi, meta.y >> kPolicyBits, k, f2(pos), A.next_key_base);
 }
 
 } // namespace umans

@@ -126,6 +126,14 @@ struct StepArgs {
 	unsigned* ovf_count;
 	int ovf_base;
 	int tier_final;               // 1: this launch's lists are final (counts go to stats)
+	// with no cap (uncapped = 1) an agent with more than 128 neighbours in range, and in
+	// any world an agent with more than kMaxObstacleEdges obstacle edges in range, is
+	// handed to k_step_huge: huge_list[i], i counted in *huge_count (null: no such path)
+	int* huge_list;
+	unsigned* huge_count;
+	int uncapped;
+	// sticky error bits (kFlag*), kept until the host reads and clears them
+	unsigned* flags;
 	// optional: the next step's grid count done here (fixed grids only): the key of every
 	// agent, from its new position, written into next_cell_of (indexed like out) and
 	// counted in next_count; next_key_base = this class's slot x slot cells
@@ -146,7 +154,13 @@ struct StepArgs {
 // to a launch with the 128-entry sorted insertion, which keeps the nearest.
 constexpr int kLongCap = 128;
 
-// Obstacle edges kept per agent (the generic class only); more are counted in stats[2].
+// Obstacle edges kept per agent by a lane of the generic class; an agent with more is
+// handed to k_step_huge, which keeps them all.
 constexpr int kMaxObstacleEdges = 32;
+
+// Entries of a neighbour list k_step_huge ranks in LDS: an uncapped list longer than this
+// sets kFlagListTooLong (the host reports it as an error).
+constexpr int kHugeMax = 4096;
+enum : unsigned { kFlagListTooLong = 1u, kFlagRawListTooLong = 2u };
 
 } // namespace umans

@@ -87,6 +87,15 @@ class UmansError(RuntimeError):
     pass
 
 
+class UmansTruncated(UmansError):
+    """UMANS_ETRUNC: a step kept fewer neighbours than the reference would (an uncapped
+    list longer than 4096 entries, or a list longer than the adjoint keeps). The call's
+    outputs are still written."""
+
+
+ETRUNC = -1002
+
+
 _lib = None
 
 
@@ -224,7 +233,8 @@ class World:
     def _check(self, rc):
         if rc != 0:
             msg = lib().umans_last_error(self._h)
-            raise UmansError(f"umans call failed ({rc}): {msg.decode() if msg else ''}")
+            cls = UmansTruncated if rc == ETRUNC else UmansError
+            raise cls(f"umans call failed ({rc}): {msg.decode() if msg else ''}")
 
     def close(self):
         if getattr(self, "_h", None):
