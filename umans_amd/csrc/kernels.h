@@ -435,6 +435,14 @@ UD bool orders_after(const State& S, V2 p, const WorldInfo& wi, double d2, float
 	return S.meta[j].x > S.meta[u & kIndexMask].x;
 }
 
+// entry (index | segment) u before entry w in the canonical order: segment, exact d2, ID
+UD bool entry_before(const State& S, V2 p, const WorldInfo& wi, unsigned u, unsigned w) {
+	if ((u >> kSegShift) != (w >> kSegShift)) return (u >> kSegShift) < (w >> kSegShift);
+	const double du = entry_d2(S, p, wi, u), dw = entry_d2(S, p, wi, w);
+	if (du != dw) return du < dw;
+	return S.meta[u & kIndexMask].x < S.meta[w & kIndexMask].x;
+}
+
 template <int KMAX, int BS>
 UD int find_neighbors(const StepArgs& A, const WorldInfo& wi, const GridInfo& g, int self, V2 p, float range,
                       float* d2s, unsigned* nbs, int& total) {
@@ -586,6 +594,155 @@ UD bool key_less(double da, unsigned ua, int ia, double db, unsigned ub, int ib)
 }
 UD bool same_float_key(unsigned long long a, unsigned long long b) {
 	return a != kKeyPad && ((a ^ b) >> 28) == 0;
+}
+
+// ------------------------------------------------------------------ short lists
+// Lists of up to kShortList entries (caps of 12 and 16): every candidate in range is
+// appended, unsorted, to the lane's column of a small LDS buffer (one store), and the
+// buffer is sorted once at the end by a fixed compare-exchange network in registers;
+// a buffer that fills up is sorted and cut to the cap on the spot. Sorting the packed
+// keys (segment, float(d2), index) orders the entries canonically wherever their floats
+// differ; runs of equal segment and float are put in exact-d2 and ID order before
+// anything is cut. This replaces an insertion per candidate, whose shift loop ran as
+// long as the wave's longest shift and cost more instructions than the rest of the step.
+constexpr int kShortList = 16;
+template <int KMAX> constexpr int short_buffer() { return KMAX + 4; }
+
+// Batcher's odd-even merge sort for n keys: the network for the next power of two with
+// the comparators that touch a slot >= n left out (those slots would hold +inf).
+template <int N>
+struct SortNet {
+	int a[160], b[160], n;
+	constexpr SortNet() : a(), b(), n(0) {
+		int P = 1;
+		while (P < N) P <<= 1;
+		for (int q = 1; q < P; q <<= 1)
+			for (int k = q; k >= 1; k >>= 1)
+				for (int j = k % q; j + k < P; j += 2 * k)
+					for (int i = 0; i < k; ++i)
+						if (i + j + k < N && (i + j) / (2 * q) == (i + j + k) / (2 * q)) {
+							a[n] = i + j;
+							b[n] = i + j + k;
+							++n;
+						}
+	}
+};
+template <int N> inline constexpr SortNet<N> kSortNet{};
+
+template <int N, int C = 0>
+UD void sort_net(unsigned long long* k) {
+	if constexpr (C < kSortNet<N>.n) {
+		constexpr int x = kSortNet<N>.a[C], y = kSortNet<N>.b[C];
+		const unsigned long long lo = k[x] < k[y] ? k[x] : k[y];
+		const unsigned long long hi = k[x] < k[y] ? k[y] : k[x];
+		k[x] = lo;
+		k[y] = hi;
+		sort_net<N, C + 1>(k);
+	}
+}
+
+// Sorts the lane's first cnt buffered keys (buf[i * 64]) into k[0..KC) (pads after them),
+// orders runs of equal floats exactly, and returns min(cnt, kcap); the first that many
+// keys are also left in the buffer, in order.
+template <int KC>
+UD int sort_short(const State& S, V2 p, const WorldInfo& wi, unsigned long long* buf, int cnt, int kcap,
+                  unsigned long long* k) {
+#pragma unroll
+	for (int i = 0; i < KC; ++i) {
+		const unsigned long long v = buf[i * 64];
+		k[i] = i < cnt ? v : kKeyPad;
+	}
+	sort_net<KC>(k);
+	// a run of equal segment and float that reaches into the kept entries
+	bool tie = false;
+#pragma unroll
+	for (int i = 0; i + 1 < KC; ++i) tie = tie | ((i < kcap) & same_float_key(k[i], k[i + 1]));
+	if (__ballot(tie) != 0ull) {
+		if (tie) {
+#pragma unroll
+			for (int i = 0; i < KC; ++i)
+				if (i < cnt) buf[i * 64] = k[i];
+			for (int i = 0; i + 1 < cnt;) {
+				int r = i + 1;
+				while (r < cnt && same_float_key(buf[i * 64], buf[r * 64])) ++r;
+				for (int x = i + 1; x < r; ++x) {
+					const unsigned long long v = buf[x * 64];
+					int y = x;
+					while (y > i && entry_before(S, p, wi, key_entry(v), key_entry(buf[(y - 1) * 64]))) {
+						buf[y * 64] = buf[(y - 1) * 64];
+						--y;
+					}
+					buf[y * 64] = v;
+				}
+				i = r;
+			}
+#pragma unroll
+			for (int i = 0; i < KC; ++i) {
+				const unsigned long long v = buf[i * 64];
+				k[i] = i < cnt ? v : kKeyPad;
+			}
+		}
+	}
+	const int m = cnt < kcap ? cnt : kcap;
+#pragma unroll
+	for (int i = 0; i < KC; ++i)
+		if (i < m) buf[i * 64] = k[i];
+	return m;
+}
+
+// find_neighbors for KMAX <= kShortList: the same queries, exact test and canonical list.
+// buf: the lane's column of KMAX + 4 keys (stride 64, a wave's own region); the finished
+// list (index | segment, stride 64) is written over the start of the wave's region, lane
+// by lane as the keys were.
+template <int KMAX>
+UD int find_short(const StepArgs& A, const WorldInfo& wi, const GridInfo& g, int self, V2 p, float range,
+                  unsigned long long* buf, unsigned* list, int& total) {
+	constexpr int KC = short_buffer<KMAX>();
+	const State& S = A.in;
+	const double r2 = (double)(range * range);
+	const float rs = range * 1.0001f + 1e-3f;
+	const int kcap = A.kcap < KMAX ? A.kcap : KMAX;
+	unsigned long long k[KC];
+	int cnt = 0;
+	total = 0;
+	for (int s = 0; s < 5; ++s) {
+		float dx = 0.0f, dy = 0.0f;
+		if (s > 0) {
+			if (wi.world_type != 1) break;
+			if (s == 1) { if (!((double)(p.x - range) < -0.5 * (double)wi.W)) continue; dx = wi.W; }
+			else if (s == 2) { if (!((double)(p.x + range) > 0.5 * (double)wi.W)) continue; dx = -wi.W; }
+			else if (s == 3) { if (!((double)(p.y - range) < -0.5 * (double)wi.H)) continue; dy = wi.H; }
+			else { if (!((double)(p.y + range) > 0.5 * (double)wi.H)) continue; dy = -wi.H; }
+		}
+		const V2 q{p.x + dx, p.y + dy};
+		const float m = rs + (fabsf(q.x) + fabsf(q.y)) * 1e-6f;
+		const int cx0 = bin_axis(q.x - m, g.ox, g.invx, g.ncx), cx1 = bin_axis(q.x + m, g.ox, g.invx, g.ncx);
+		const int cy0 = bin_axis(q.y - m, g.oy, g.invy, g.ncy), cy1 = bin_axis(q.y + m, g.oy, g.invy, g.ncy);
+		const double qx = (double)q.x, qy = (double)q.y;
+		for (int cy = cy0; cy <= cy1; ++cy) {
+			const int row = wi.cell_offset + cy * g.ncx;
+			const unsigned b = A.cell_start[row + cx0], e = A.cell_start[row + cx1 + 1];
+			for (unsigned j = b; j < e; ++j) {
+				const float2 pj = S.pos[j];
+				const double ddx = qx - (double)pj.x;
+				const double ddy = qy - (double)pj.y;
+				const double d2 = ddx * ddx + ddy * ddy;
+				const bool hit = d2 < r2 && (int)j != self;
+				total += hit ? 1 : 0;
+				if (hit) {
+					buf[cnt * 64] = long_key(s, d2, j);
+					++cnt;
+				}
+				if (__ballot(cnt == KC) != 0ull) cnt = sort_short<KC>(S, p, wi, buf, cnt, kcap, k);
+			}
+		}
+	}
+	cnt = sort_short<KC>(S, p, wi, buf, cnt, kcap, k);
+	// the list overlays the keys of the wave's lanes: every lane has read its keys
+	__builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+#pragma unroll
+	for (int i = 0; i < KMAX; ++i) list[i * 64] = key_entry(k[i]);
+	return cnt;
 }
 // One bitonic sort of packed keys over blocks of S lanes (P = 1, S <= 64) or over 128
 // entries, two per lane (P = 2: entry lane and lane + 64).
@@ -1108,8 +1265,12 @@ __global__ __launch_bounds__(BS) void k_step(StepArgs A) {
 	constexpr bool kLong = KMAX == 0;
 	constexpr int KL = kLong ? kLongCap : KMAX;    // the longest list a lane can hold
 	static_assert(!kLong || BS == 64, "long lists are ranked per wave: one wave per block");
-	__shared__ float s_d2[kLong ? 1 : KMAX * BS];
-	__shared__ unsigned s_nb[kLong ? 1 : KMAX * BS];
+	constexpr bool kShort = !kLong && KMAX <= kShortList;
+	constexpr int KC = short_buffer<KMAX>();
+	__shared__ float s_d2[kLong || kShort ? 1 : KMAX * BS];
+	__shared__ unsigned s_nb[kLong || kShort ? 1 : KMAX * BS];
+	// short lists: each wave's region of KC keys per lane, its list written over the start
+	__shared__ unsigned long long s_buf[kShort ? KC * BS : 1];
 	// The ORCA half-planes live in private memory: kept in LDS they bound the class to
 	// ~2 waves per SIMD, and the private copy (L1-resident) measured faster at 6.
 	const int tid = threadIdx.x;
@@ -1142,6 +1303,10 @@ __global__ __launch_bounds__(BS) void k_step(StepArgs A) {
 		a.relaxation = A.pol[pi].relaxation;
 		if constexpr (kLong) {
 			total = collect_neighbors(A, wi, g, k, a.pos, A.pol[pi].query_range, base + t);
+		} else if constexpr (kShort) {
+			unsigned long long* region = s_buf + (tid >> 6) * (KC * 64);
+			cnt = find_short<KMAX>(A, wi, g, k, a.pos, A.pol[pi].query_range, region + (tid & 63),
+			                       reinterpret_cast<unsigned*>(region) + (tid & 63), total);
 		} else {
 			cnt = find_neighbors<KMAX, BS>(A, wi, g, k, a.pos, A.pol[pi].query_range, s_d2 + tid, s_nb + tid, total);
 		}
@@ -1194,8 +1359,10 @@ __global__ __launch_bounds__(BS) void k_step(StepArgs A) {
 		return;
 	}
 	// the lane's sorted list: its LDS column, or its row of the long-list buffer
-	const unsigned* lst = kLong ? A.long_u + (base + t) : s_nb + tid;
-	const int lstride = kLong ? A.long_stride : BS;
+	const unsigned* lst = kLong ? A.long_u + (base + t)
+	                            : (kShort ? reinterpret_cast<const unsigned*>(s_buf + (tid >> 6) * (KC * 64)) + (tid & 63)
+	                                      : s_nb + tid);
+	const int lstride = kLong ? A.long_stride : (kShort ? 64 : BS);
 	finish_agent<KMAX, CLS, false>(A, k, meta, pi, wix, wi, a, lst, lstride, cnt, obs_priv, n_obs, nullptr, nullptr,
 	                               A.next_key_base);
 }
@@ -1220,14 +1387,6 @@ struct HugeArgs {
 	int edge_cap;
 	Line* lines;             // 2 * kHugeMax per block: half-planes, projected lines
 };
-
-// entry (index | segment) u before entry w in the canonical order: segment, exact d2, ID
-UD bool entry_before(const State& S, V2 p, const WorldInfo& wi, unsigned u, unsigned w) {
-	if ((u >> kSegShift) != (w >> kSegShift)) return (u >> kSegShift) < (w >> kSegShift);
-	const double du = entry_d2(S, p, wi, u), dw = entry_d2(S, p, wi, w);
-	if (du != dw) return du < dw;
-	return S.meta[u & kIndexMask].x < S.meta[w & kIndexMask].x;
-}
 
 __global__ __launch_bounds__(64) void k_step_huge(HugeArgs H) {
 	__shared__ unsigned long long s_key[kHugeMax];
@@ -1387,8 +1546,13 @@ UD int regular_count(const DevPolicy& P) {
 template <int KMAX>
 __global__ __launch_bounds__(64) void k_step_coop(StepArgs A) {
 	constexpr int BS = 64;
-	__shared__ float s_d2[KMAX * BS];
-	__shared__ unsigned s_nb[KMAX * BS];
+	constexpr bool kShort = KMAX <= kShortList;
+	constexpr int KC = short_buffer<KMAX>();
+	__shared__ float s_d2[kShort ? 1 : KMAX * BS];
+	__shared__ unsigned s_nbl[kShort ? 1 : KMAX * BS];
+	__shared__ unsigned long long s_buf[kShort ? KC * BS : 1];
+	// the lists, lane-interleaved (stride BS = 64): short lists are written over the keys
+	unsigned* const s_nb = kShort ? reinterpret_cast<unsigned*>(s_buf) : s_nbl;
 	__shared__ float s_self[12][BS];    // pos, vel, pref, radius, pref speed, max speed, max accel, dt, relaxation
 	__shared__ float s_ttcp[4][BS];
 	__shared__ float s_wh[2][BS];
@@ -1421,7 +1585,10 @@ __global__ __launch_bounds__(64) void k_step_coop(StepArgs A) {
 		a.radius = prm.x; a.pref_speed = prm.y; a.max_speed = prm.z; a.max_accel = prm.w;
 		a.dt = wi.dt;
 		a.relaxation = A.pol[pi].relaxation;
-		cnt = find_neighbors<KMAX, BS>(A, wi, g, k, a.pos, A.pol[pi].query_range, s_d2 + tid, s_nb + tid, total);
+		if constexpr (kShort)
+			cnt = find_short<KMAX>(A, wi, g, k, a.pos, A.pol[pi].query_range, s_buf + tid, s_nb + tid, total);
+		else
+			cnt = find_neighbors<KMAX, BS>(A, wi, g, k, a.pos, A.pol[pi].query_range, s_d2 + tid, s_nb + tid, total);
 	}
 	// with no cap, a list longer than KMAX is redone whole by k_step_huge; the lane still
 	// takes part in the wave's barriers but has no candidates
