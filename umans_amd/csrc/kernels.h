@@ -600,7 +600,7 @@ UD bool same_float_key(unsigned long long a, unsigned long long b) {
 // Lists of up to kShortList entries (caps of 12 and 16): every candidate in range is
 // appended, unsorted, to the lane's column of a small LDS buffer (one store), and the
 // buffer is sorted once at the end by a fixed compare-exchange network in registers;
-// a buffer that fills up is sorted and cut to the cap on the spot. Sorting the packed
+// a buffer that fills up drops its canonical last entry. Sorting the packed
 // keys (segment, float(d2), index) orders the entries canonically wherever their floats
 // differ; runs of equal segment and float are put in exact-d2 and ID order before
 // anything is cut. This replaces an insertion per candidate, whose shift loop ran as
@@ -641,9 +641,29 @@ UD void sort_net(unsigned long long* k) {
 	}
 }
 
-// Sorts the lane's first cnt buffered keys (buf[i * 64]) into k[0..KC) (pads after them),
-// orders runs of equal floats exactly, and returns min(cnt, kcap); the first that many
-// keys are also left in the buffer, in order.
+// A full buffer drops its canonical last entry: KC - 1 >= kcap entries order before it,
+// so it is in no list, nor is any later candidate whose (segment, float) is above its
+// own. Returns the dropped key; the caller's count goes down by one.
+template <int KC>
+UD unsigned long long drop_last(const State& S, V2 p, const WorldInfo& wi, unsigned long long* buf) {
+	int mi = 0;
+	unsigned long long mk = buf[0];
+#pragma unroll
+	for (int i = 1; i < KC; ++i) {
+		const unsigned long long v = buf[i * 64];
+		// (segment, float) decides; equal ones by exact d2 and ID (rare)
+		bool later = (v >> 28) > (mk >> 28);
+		if (same_float_key(v, mk)) later = entry_before(S, p, wi, key_entry(mk), key_entry(v));
+		mi = later ? i : mi;
+		mk = later ? v : mk;
+	}
+	buf[mi * 64] = buf[(KC - 1) * 64];
+	return mk;
+}
+
+// Sorts the lane's cnt (<= KC) buffered keys (buf[i * 64]) into k[0..KC), pads after
+// them, puts runs of equal segment and float that reach into the kept entries in exact
+// d2 and ID order, and returns min(cnt, kcap).
 template <int KC>
 UD int sort_short(const State& S, V2 p, const WorldInfo& wi, unsigned long long* buf, int cnt, int kcap,
                   unsigned long long* k) {
@@ -653,7 +673,6 @@ UD int sort_short(const State& S, V2 p, const WorldInfo& wi, unsigned long long*
 		k[i] = i < cnt ? v : kKeyPad;
 	}
 	sort_net<KC>(k);
-	// a run of equal segment and float that reaches into the kept entries
 	bool tie = false;
 #pragma unroll
 	for (int i = 0; i + 1 < KC; ++i) tie = tie | ((i < kcap) & same_float_key(k[i], k[i + 1]));
@@ -683,11 +702,7 @@ UD int sort_short(const State& S, V2 p, const WorldInfo& wi, unsigned long long*
 			}
 		}
 	}
-	const int m = cnt < kcap ? cnt : kcap;
-#pragma unroll
-	for (int i = 0; i < KC; ++i)
-		if (i < m) buf[i * 64] = k[i];
-	return m;
+	return cnt < kcap ? cnt : kcap;
 }
 
 // find_neighbors for KMAX <= kShortList: the same queries, exact test and canonical list.
@@ -703,6 +718,7 @@ UD int find_short(const StepArgs& A, const WorldInfo& wi, const GridInfo& g, int
 	const float rs = range * 1.0001f + 1e-3f;
 	const int kcap = A.kcap < KMAX ? A.kcap : KMAX;
 	unsigned long long k[KC];
+	unsigned long long lim = kKeyPad;   // (segment, float) above that of a dropped key: in no list
 	int cnt = 0;
 	total = 0;
 	for (int s = 0; s < 5; ++s) {
@@ -729,11 +745,17 @@ UD int find_short(const StepArgs& A, const WorldInfo& wi, const GridInfo& g, int
 				const double d2 = ddx * ddx + ddy * ddy;
 				const bool hit = d2 < r2 && (int)j != self;
 				total += hit ? 1 : 0;
-				if (hit) {
-					buf[cnt * 64] = long_key(s, d2, j);
+				const unsigned long long key = long_key(s, d2, j);
+				if (hit && (key >> 28) <= (lim >> 28)) {
+					buf[cnt * 64] = key;
 					++cnt;
 				}
-				if (__ballot(cnt == KC) != 0ull) cnt = sort_short<KC>(S, p, wi, buf, cnt, kcap, k);
+				if (__ballot(cnt == KC) != 0ull) {
+					if (cnt == KC) {
+						lim = drop_last<KC>(S, p, wi, buf);
+						--cnt;
+					}
+				}
 			}
 		}
 	}
@@ -1289,18 +1311,13 @@ __global__ __launch_bounds__(BS) void k_step(StepArgs A) {
 	Self a;
 	WorldInfo wi{};
 	if (active) {
-		meta = S.meta[k];
-		pi = meta.y & kPolicyMask;
-		wix = meta.y >> kPolicyBits;
+		// only what the search needs is loaded before it (fewer registers live across it)
+		const int my = S.meta[k].y;
+		pi = my & kPolicyMask;
+		wix = my >> kPolicyBits;
 		wi = A.worlds[wix];
 		const GridInfo g = A.grids[wix];
-		const float4 prm = S.prm[k];
 		a.pos = v2(S.pos[k]);
-		a.vel = v2(S.vel[k]);
-		a.pref = v2(S.pref[k]);
-		a.radius = prm.x; a.pref_speed = prm.y; a.max_speed = prm.z; a.max_accel = prm.w;
-		a.dt = wi.dt;
-		a.relaxation = A.pol[pi].relaxation;
 		if constexpr (kLong) {
 			total = collect_neighbors(A, wi, g, k, a.pos, A.pol[pi].query_range, base + t);
 		} else if constexpr (kShort) {
@@ -1310,6 +1327,13 @@ __global__ __launch_bounds__(BS) void k_step(StepArgs A) {
 		} else {
 			cnt = find_neighbors<KMAX, BS>(A, wi, g, k, a.pos, A.pol[pi].query_range, s_d2 + tid, s_nb + tid, total);
 		}
+		meta = S.meta[k];
+		const float4 prm = S.prm[k];
+		a.vel = v2(S.vel[k]);
+		a.pref = v2(S.pref[k]);
+		a.radius = prm.x; a.pref_speed = prm.y; a.max_speed = prm.z; a.max_accel = prm.w;
+		a.dt = wi.dt;
+		a.relaxation = A.pol[pi].relaxation;
 	}
 	// long lists: a lane with more than kLongCap in range is handed on; the others are
 	// ranked by the wave (every lane takes part, active or not)
