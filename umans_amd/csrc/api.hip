@@ -230,10 +230,18 @@ struct umans_ctx {
 	int rec_stride = 0;
 	bool have_record = false;
 
-	// hipGraph of one step (kernels only: counters are cleared by a fill kernel)
+	// hipGraphs of one step (kernels only: counters are cleared by a fill kernel). A step
+	// leaves the static fields in the buffers it read them from and swaps those buffers
+	// between X and Y, so consecutive steps see two buffer layouts: one graph for each,
+	// told apart by the buffers they were captured with.
 	int use_graph = 0;
-	hipGraphExec_t graph_exec = nullptr;
-	hipGraph_t graph = nullptr;
+	struct StepGraph {
+		hipGraphExec_t exec = nullptr;
+		hipGraph_t graph = nullptr;
+		const void* key[3] = {nullptr, nullptr, nullptr};
+	};
+	StepGraph graphs[2];
+	int graph_next = 0;
 
 	long long steps = 0;
 	double sim_time = 0.0;
@@ -320,10 +328,21 @@ struct umans_ctx {
 		return UMANS_EHIP;
 	}
 	void drop_graph() {
-		if (graph_exec) (void)hipGraphExecDestroy(graph_exec);
-		if (graph) (void)hipGraphDestroy(graph);
-		graph_exec = nullptr;
-		graph = nullptr;
+		for (StepGraph& g : graphs) {
+			if (g.exec) (void)hipGraphExecDestroy(g.exec);
+			if (g.graph) (void)hipGraphDestroy(g.graph);
+			g = StepGraph{};
+		}
+	}
+	// The static fields (goal, radius and speeds, preferred velocity, ID / policy / row /
+	// engine state) of the state a step read (Y, cell-sorted) are those of its output (X,
+	// same order): the step writes only positions, velocities and viewing directions, and
+	// the buffers trade places.
+	void swap_static() {
+		std::swap(X.s.goal, Y.s.goal);
+		std::swap(X.s.prm, Y.s.prm);
+		std::swap(X.s.pref, Y.s.pref);
+		std::swap(X.s.meta, Y.s.meta);
 	}
 	void free_adjoint() {
 		for (DeviceState& d : ckpts) d.release();
@@ -846,9 +865,18 @@ int enqueue_step(umans_ctx* c, bool events, const State* ysave = nullptr, unsign
 		hipLaunchKernelGGL(k_grid_setup, dim3(ceil_div(nw, 64)), dim3(64), 0, st, c->dworlds, c->dbbox, c->dgrids_next,
 		                   nw, nullptr, 0, nullptr, 0, 1);
 
+	// without a saved copy of the sorted state (the adjoint's), the step writes only what
+	// it changes and the static fields move with the buffers (swap_static)
+	const bool lean = ysave == nullptr;
 	StepArgs a{};
 	a.in = Ys;
 	a.out = c->X.s;
+	if (lean) {
+		a.out.goal = nullptr;
+		a.out.prm = nullptr;
+		a.out.pref = nullptr;
+		a.out.meta = nullptr;
+	}
 	a.pol = c->dpol;
 	a.worlds = c->dworlds;
 	a.grids = c->dgrids;
@@ -982,15 +1010,39 @@ int enqueue_step(umans_ctx* c, bool events, const State* ysave = nullptr, unsign
 	if (!rc) {
 		c->counts_zero_n = fuse ? -1 : scan_n;
 		c->keys_ready = fuse_next;
+		if (lean) c->swap_static();
 	}
 	return rc;
 }
 
-int capture_graph(umans_ctx* c, bool fuse_next) {
-	c->drop_graph();
+// The buffers a captured step reads and writes.
+void graph_key(const umans_ctx* c, const void** key) {
+	key[0] = c->X.s.pos;
+	key[1] = c->X.s.goal;
+	key[2] = c->Y.s.pos;
+}
+
+// The graph of one step for the current buffers, captured if there is none yet; the
+// buffers are left as before (the capture's swap_static is undone: nothing ran).
+int step_graph(umans_ctx* c, bool fuse_next, hipGraphExec_t* out) {
+	const void* key[3];
+	graph_key(c, key);
+	for (umans_ctx::StepGraph& sg : c->graphs)
+		if (sg.exec && sg.key[0] == key[0] && sg.key[1] == key[1] && sg.key[2] == key[2]) {
+			*out = sg.exec;
+			return UMANS_OK;
+		}
+	umans_ctx::StepGraph& sg = c->graphs[c->graph_next];
+	c->graph_next ^= 1;
+	if (sg.exec) (void)hipGraphExecDestroy(sg.exec);
+	if (sg.graph) (void)hipGraphDestroy(sg.graph);
+	sg = umans_ctx::StepGraph{};
 	int rc = c->hip(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal), "graph capture");
 	if (rc) return rc;
+	const State x0 = c->X.s, y0 = c->Y.s;
 	const int erc = enqueue_step(c, false, nullptr, nullptr, nullptr, fuse_next);
+	c->X.s = x0;
+	c->Y.s = y0;
 	hipGraph_t g = nullptr;
 	const hipError_t e = hipStreamEndCapture(c->stream, &g);
 	if (erc) {
@@ -998,8 +1050,11 @@ int capture_graph(umans_ctx* c, bool fuse_next) {
 		return erc;
 	}
 	if ((rc = c->hip(e, "graph capture"))) return rc;
-	c->graph = g;
-	return c->hip(hipGraphInstantiate(&c->graph_exec, g, nullptr, nullptr, 0), "graph instantiate");
+	sg.graph = g;
+	if ((rc = c->hip(hipGraphInstantiate(&sg.exec, g, nullptr, nullptr, 0), "graph instantiate"))) return rc;
+	for (int i = 0; i < 3; ++i) sg.key[i] = key[i];
+	*out = sg.exec;
+	return UMANS_OK;
 }
 
 // ------------------------------------------------------------------ population
@@ -1273,8 +1328,10 @@ int run_steps_inner(umans_ctx* c, int nsteps) {
 		if (c->n > 0) {
 			int rc;
 			if (c->use_graph && !c->profiling && c->keys_ready) {
-				if (!c->graph_exec && (rc = capture_graph(c, true))) return rc;
-				if ((rc = c->hip(hipGraphLaunch(c->graph_exec, c->stream), "graph launch"))) return rc;
+				hipGraphExec_t ge = nullptr;
+				if ((rc = step_graph(c, true, &ge))) return rc;
+				if ((rc = c->hip(hipGraphLaunch(ge, c->stream), "graph launch"))) return rc;
+				c->swap_static();
 				c->keys_ready = true;
 				c->counts_zero_n = -1;
 			} else if ((rc = enqueue_step(c, c->profiling != 0, nullptr, nullptr, nullptr, true))) {

@@ -1272,11 +1272,18 @@ UD void finish_agent(const StepArgs& A, int k, int4 meta, int pi, int wix, const
 	A.out.pos[k] = f2(pos);
 	A.out.vel[k] = f2(vel);
 	A.out.viewdir[k] = f2(view);
-	A.out.goal[k] = S.goal[k];
-	A.out.prm[k] = S.prm[k];
-	A.out.pref[k] = f2(a.pref);
-	meta.w = (int)rng;
-	A.out.meta[k] = meta;
+	if (A.out.meta) {
+		// a full copy of the record
+		A.out.goal[k] = S.goal[k];
+		A.out.prm[k] = S.prm[k];
+		A.out.pref[k] = f2(a.pref);
+		meta.w = (int)rng;
+		A.out.meta[k] = meta;
+	} else if (rng != (uint32_t)meta.w) {
+		// the static fields stay where they were read (the host swaps the buffers): only
+		// the sampling engine's state changes, in place (no other agent reads it)
+		reinterpret_cast<int*>(S.meta + k)[3] = (int)rng;
+	}
 	if (A.next_count) count_next(A, wi, wix, k, f2(pos), next_key_base);
 }
 
@@ -1768,10 +1775,12 @@ __global__ __launch_bounds__(64) void k_step_coop(StepArgs A) {
 	A.out.pos[k] = f2(pos);
 	A.out.vel[k] = f2(vel);
 	A.out.viewdir[k] = f2(view);
-	A.out.goal[k] = S.goal[k];
-	A.out.prm[k] = S.prm[k];
-	A.out.pref[k] = f2(a.pref);
-	A.out.meta[k] = meta;
+	if (A.out.meta) {             // a full copy (else the static fields stay where they are)
+		A.out.goal[k] = S.goal[k];
+		A.out.prm[k] = S.prm[k];
+		A.out.pref[k] = f2(a.pref);
+		A.out.meta[k] = meta;
+	}
 	if (A.next_count) count_next(A, wi, meta.y >> kPolicyBits, k, f2(pos), A.next_key_base);
 }
 

@@ -86,7 +86,8 @@ struct State {
 
 struct StepArgs {
 	State in;                // cell-sorted state of this step (read only)
-	State out;               // next state, same order
+	State out;               // next state, same order; goal, prm, pref and meta null: those stay in
+	                         // `in` (only the engine state in meta.w is updated there)
 	const DevPolicy* pol;
 	const WorldInfo* worlds;
 	const GridInfo* grids;
