@@ -766,6 +766,37 @@ UD int find_short(const StepArgs& A, const WorldInfo& wi, const GridInfo& g, int
 	for (int i = 0; i < KMAX; ++i) list[i * 64] = key_entry(k[i]);
 	return cnt;
 }
+// The value of lane ^ J (every lane active): DPP for J = 1, 2 (quad permutes) and 8 (a
+// rotation by 8 within rows of 16), ds_swizzle for 4 and 16 (within 32 lanes), a
+// permute for 32 — the first three never leave the VALU.
+template <int J>
+UD unsigned xor_lane(unsigned v) {
+	if constexpr (J == 1) return (unsigned)__builtin_amdgcn_update_dpp(0, (int)v, 0xB1, 0xF, 0xF, true);
+	else if constexpr (J == 2) return (unsigned)__builtin_amdgcn_update_dpp(0, (int)v, 0x4E, 0xF, 0xF, true);
+	else if constexpr (J == 8) return (unsigned)__builtin_amdgcn_update_dpp(0, (int)v, 0x128, 0xF, 0xF, true);
+	else if constexpr (J == 4 || J == 16) return (unsigned)__builtin_amdgcn_ds_swizzle((int)v, 0x1F | (J << 10));
+	else return (unsigned)__shfl_xor((int)v, J);
+}
+template <int J>
+UD unsigned long long xor_lane_u64(unsigned long long v) {
+	return ((unsigned long long)xor_lane<J>((unsigned)(v >> 32)) << 32) | xor_lane<J>((unsigned)v);
+}
+// Bitonic sort, ascending, of one key per lane over blocks of S lanes, with every stride
+// known at compile time: stage (W, J) of the merge of blocks of W.
+template <int S, int W, int J>
+UD void bitonic_stage(unsigned long long& k, int lane) {
+	const unsigned long long kp = xor_lane_u64<J>(k);
+	const bool take_min = (((lane & (S - 1)) & W) == 0) == ((lane & J) == 0);
+	const bool mine_lower = k < kp;
+	k = (take_min == mine_lower) ? k : kp;
+	if constexpr (J > 1) bitonic_stage<S, W, J / 2>(k, lane);
+	else if constexpr (W < S) bitonic_stage<S, 2 * W, W>(k, lane);
+}
+template <int S>
+UD void bitonic_keys_fixed(unsigned long long& k) {
+	bitonic_stage<S, 2, 1>(k, threadIdx.x & 63);
+}
+
 // One bitonic sort of packed keys over blocks of S lanes (P = 1, S <= 64) or over 128
 // entries, two per lane (P = 2: entry lane and lane + 64).
 template <int P>
@@ -866,7 +897,9 @@ UD void rank_long_lists(const StepArgs& A, int row, int n_row, int kcap, V2 p, c
 			const int i = lane & (SZ - 1);
 			unsigned long long k[2] = {kKeyPad, kKeyPad};
 			if (i < n) k[0] = A.long_key[(size_t)i * st + rw];
-			bitonic_keys<1>(k, SZ);
+			if (lg == 4) bitonic_keys_fixed<16>(k[0]);
+			else if (lg == 5) bitonic_keys_fixed<32>(k[0]);
+			else bitonic_keys_fixed<64>(k[0]);
 			// ties sit next to each other once sorted
 			const unsigned long long nx = shfl_u64(k[0], (lane + 1) & 63);
 			const int keep = n < kcap ? n : kcap;
