@@ -796,64 +796,69 @@ template <int S>
 UD void bitonic_keys_fixed(unsigned long long& k) {
 	bitonic_stage<S, 2, 1>(k, threadIdx.x & 63);
 }
+// The same network on full entries (exact d2, index | segment, ID; canonical order).
+template <int S, int W, int J>
+UD void bitonic_exact_stage(double& d, unsigned& u, int& id, int lane) {
+	const double dp = __longlong_as_double((long long)xor_lane_u64<J>((unsigned long long)__double_as_longlong(d)));
+	const unsigned up = xor_lane<J>(u);
+	const int ip = (int)xor_lane<J>((unsigned)id);
+	const bool take_min = (((lane & (S - 1)) & W) == 0) == ((lane & J) == 0);
+	const bool take = take_min ? key_less(dp, up, ip, d, u, id) : key_less(d, u, id, dp, up, ip);
+	d = take ? dp : d;
+	u = take ? up : u;
+	id = take ? ip : id;
+	if constexpr (J > 1) bitonic_exact_stage<S, W, J / 2>(d, u, id, lane);
+	else if constexpr (W < S) bitonic_exact_stage<S, 2 * W, W>(d, u, id, lane);
+}
+template <int S>
+UD void bitonic_exact_fixed(double& d, unsigned& u, int& id) {
+	bitonic_exact_stage<S, 2, 1>(d, u, id, threadIdx.x & 63);
+}
+// 128 entries, two per lane (entry lane and lane + 64): the stride-64 exchanges stay in
+// the lane.
+template <int W, int J>
+UD void bitonic128_stage(unsigned long long (&k)[2], int lane) {
+	if constexpr (J == 64) {
+		const unsigned long long lo = k[0] < k[1] ? k[0] : k[1];
+		const unsigned long long hi = k[0] < k[1] ? k[1] : k[0];
+		k[0] = lo;
+		k[1] = hi;
+	} else {
+#pragma unroll
+		for (int q = 0; q < 2; ++q) {
+			const unsigned long long kp = xor_lane_u64<J>(k[q]);
+			const bool take_min = (((lane + 64 * q) & W) == 0) == ((lane & J) == 0);
+			k[q] = (take_min == (k[q] < kp)) ? k[q] : kp;
+		}
+	}
+	if constexpr (J > 1) bitonic128_stage<W, J / 2>(k, lane);
+	else if constexpr (W < 128) bitonic128_stage<2 * W, W>(k, lane);
+}
+template <int W, int J>
+UD void bitonic128_exact_stage(double (&d)[2], unsigned (&u)[2], int (&id)[2], int lane) {
+	if constexpr (J == 64) {
+		if (key_less(d[1], u[1], id[1], d[0], u[0], id[0])) {
+			const double td = d[0]; d[0] = d[1]; d[1] = td;
+			const unsigned tu = u[0]; u[0] = u[1]; u[1] = tu;
+			const int ti = id[0]; id[0] = id[1]; id[1] = ti;
+		}
+	} else {
+#pragma unroll
+		for (int q = 0; q < 2; ++q) {
+			const double dp = __longlong_as_double((long long)xor_lane_u64<J>((unsigned long long)__double_as_longlong(d[q])));
+			const unsigned up = xor_lane<J>(u[q]);
+			const int ip = (int)xor_lane<J>((unsigned)id[q]);
+			const bool take_min = (((lane + 64 * q) & W) == 0) == ((lane & J) == 0);
+			const bool take = take_min ? key_less(dp, up, ip, d[q], u[q], id[q]) : key_less(d[q], u[q], id[q], dp, up, ip);
+			d[q] = take ? dp : d[q];
+			u[q] = take ? up : u[q];
+			id[q] = take ? ip : id[q];
+		}
+	}
+	if constexpr (J > 1) bitonic128_exact_stage<W, J / 2>(d, u, id, lane);
+	else if constexpr (W < 128) bitonic128_exact_stage<2 * W, W>(d, u, id, lane);
+}
 
-// One bitonic sort of packed keys over blocks of S lanes (P = 1, S <= 64) or over 128
-// entries, two per lane (P = 2: entry lane and lane + 64).
-template <int P>
-UD void bitonic_keys(unsigned long long (&k)[2], int S) {
-	const int lane = threadIdx.x & 63;
-	const int N = P == 1 ? S : 128;
-	for (int w = 2; w <= N; w <<= 1) {
-		for (int j = w >> 1; j > 0; j >>= 1) {
-			if (P == 2 && j == 64) {
-				// entries lane and lane + 64 of one lane; w == 128: ascending
-				const unsigned long long lo = k[0] < k[1] ? k[0] : k[1];
-				const unsigned long long hi = k[0] < k[1] ? k[1] : k[0];
-				k[0] = lo; k[1] = hi;
-				continue;
-			}
-#pragma unroll
-			for (int q = 0; q < P; ++q) {
-				const unsigned long long kp = shfl_xor_u64(k[q], j);
-				const int idx = P == 1 ? (lane & (S - 1)) : lane + 64 * q;
-				const bool take_min = ((idx & w) == 0) == ((lane & j) == 0);
-				const bool p_less = kp < k[q];
-				if (take_min ? p_less : !p_less) k[q] = kp;
-			}
-		}
-	}
-}
-// The same network over full entries (segment, exact d2, agent ID): the rare second
-// sort of a pass whose keys tie. Padding entries (u = ~0) compare after all.
-template <int P>
-UD void bitonic_exact(double (&d)[2], unsigned (&u)[2], int (&id)[2], int S) {
-	const int lane = threadIdx.x & 63;
-	const int N = P == 1 ? S : 128;
-	for (int w = 2; w <= N; w <<= 1) {
-		for (int j = w >> 1; j > 0; j >>= 1) {
-			if (P == 2 && j == 64) {
-				if (key_less(d[1], u[1], id[1], d[0], u[0], id[0])) {
-					const double td = d[0]; d[0] = d[1]; d[1] = td;
-					const unsigned tu = u[0]; u[0] = u[1]; u[1] = tu;
-					const int ti = id[0]; id[0] = id[1]; id[1] = ti;
-				}
-				continue;
-			}
-#pragma unroll
-			for (int q = 0; q < P; ++q) {
-				const double dp = shfl_xor_d(d[q], j);
-				const unsigned up = (unsigned)__shfl_xor((int)u[q], j);
-				const int ip = __shfl_xor(id[q], j);
-				const int idx = P == 1 ? (lane & (S - 1)) : lane + 64 * q;
-				const bool asc = (idx & w) == 0;
-				const bool lower = (lane & j) == 0;
-				const bool p_less = key_less(dp, up, ip, d[q], u[q], id[q]);
-				const bool m_less = key_less(d[q], u[q], id[q], dp, up, ip);
-				if (lower == asc ? p_less : m_less) { d[q] = dp; u[q] = up; id[q] = ip; }
-			}
-		}
-	}
-}
 // Full entry of key k of the row whose agent stands at (px, py) in a world of width W
 // and height H: exact d2 (as entry_d2 computes it), index | segment, agent ID.
 UD void exact_entry(const State& S, float px, float py, float W, float H, unsigned long long k,
@@ -910,7 +915,9 @@ UD void rank_long_lists(const StepArgs& A, int row, int n_row, int kcap, V2 p, c
 				unsigned u[2];
 				int id[2];
 				exact_entry(S, px, py, W, H, k[0], d[0], u[0], id[0]);
-				bitonic_exact<1>(d, u, id, SZ);
+				if (lg == 4) bitonic_exact_fixed<16>(d[0], u[0], id[0]);
+				else if (lg == 5) bitonic_exact_fixed<32>(d[0], u[0], id[0]);
+				else bitonic_exact_fixed<64>(d[0], u[0], id[0]);
 				if (i < keep) A.long_u[(size_t)i * st + rw] = u[0];
 			} else if (i < keep) {
 				A.long_u[(size_t)i * st + rw] = key_entry(k[0]);
@@ -929,7 +936,7 @@ UD void rank_long_lists(const StepArgs& A, int row, int n_row, int kcap, V2 p, c
 			const int e = lane + 64 * q;
 			k[q] = e < n ? A.long_key[(size_t)e * st + rw] : kKeyPad;
 		}
-		bitonic_keys<2>(k, 128);
+		bitonic128_stage<2, 1>(k, lane);
 		// sorted position of k[q] is lane + 64 q: the next entry of k[0] on lane 63 is k[1]
 		// of lane 0
 		const unsigned long long n0 = shfl_u64(k[0], (lane + 1) & 63);
@@ -945,7 +952,7 @@ UD void rank_long_lists(const StepArgs& A, int row, int n_row, int kcap, V2 p, c
 			int id[2];
 			exact_entry(S, px, py, W, H, k[0], d[0], u[0], id[0]);
 			exact_entry(S, px, py, W, H, k[1], d[1], u[1], id[1]);
-			bitonic_exact<2>(d, u, id, 128);
+			bitonic128_exact_stage<2, 1>(d, u, id, lane);
 			if (lane < keep) A.long_u[(size_t)lane * st + rw] = u[0];
 			if (lane + 64 < keep) A.long_u[(size_t)(lane + 64) * st + rw] = u[1];
 		} else {
