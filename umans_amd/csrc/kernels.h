@@ -877,6 +877,61 @@ UD void exact_entry(const State& S, float px, float py, float W, float H, unsign
 	d = ddx * ddx + ddy * ddy;
 	id = S.meta[u & kIndexMask].x;
 }
+// The rows of more than 2^(LG-1) (LG = 4: any) and at most 2^LG entries, 64 >> LG rows
+// per pass, one entry per lane. The next pass's keys are loaded before this pass sorts,
+// so their latency is spent sorting.
+template <int LG>
+UD void rank_rows(const StepArgs& A, int row, int n_row, int kcap, V2 p, const WorldInfo& wi, int lane) {
+	constexpr int SZ = 1 << LG, G = 64 >> LG;
+	const size_t st = (size_t)A.long_stride;
+	const State& S = A.in;
+	unsigned long long m = __ballot(n_row > (LG == 4 ? 0 : SZ / 2) && n_row <= SZ);
+	if (!m) return;
+	const int i = lane & (SZ - 1);
+	// a pass's rows: the next G set bits of m (uniform), lane group g = lane >> LG
+	auto select = [&](int& src, int& n, int& rw) {
+		int rsel = -1;
+		for (int g = 0; g < G; ++g) {
+			const int r = m ? __builtin_ctzll(m) : -1;
+			if (m) m &= m - 1;
+			if ((lane >> LG) == g) rsel = r;
+		}
+		src = rsel < 0 ? 0 : rsel;          // every lane shuffles
+		const int n_src = __shfl(n_row, src);
+		rw = __shfl(row, src);
+		n = rsel >= 0 ? n_src : 0;
+	};
+	int src, n, rw;
+	select(src, n, rw);
+	unsigned long long kn = i < n ? A.long_key[(size_t)i * st + rw] : kKeyPad;
+	for (;;) {
+		const int csrc = src, cn = n, crw = rw;
+		unsigned long long k = kn;
+		const bool more = m != 0ull;
+		if (more) {
+			select(src, n, rw);
+			kn = i < n ? A.long_key[(size_t)i * st + rw] : kKeyPad;
+		}
+		bitonic_keys_fixed<SZ>(k);
+		// ties sit next to each other once sorted
+		const unsigned long long nx = shfl_u64(k, (lane + 1) & 63);
+		const int keep = cn < kcap ? cn : kcap;
+		if (__ballot(i + 1 < SZ && same_float_key(k, nx))) {
+			const float px = __shfl(p.x, csrc), py = __shfl(p.y, csrc);
+			const float W = __shfl(wi.W, csrc), H = __shfl(wi.H, csrc);
+			double d;
+			unsigned u;
+			int id;
+			exact_entry(S, px, py, W, H, k, d, u, id);
+			bitonic_exact_fixed<SZ>(d, u, id);
+			if (i < keep) A.long_u[(size_t)i * st + crw] = u;
+		} else if (i < keep) {
+			A.long_u[(size_t)i * st + crw] = key_entry(k);
+		}
+		if (!more) break;
+	}
+}
+
 UD void rank_long_lists(const StepArgs& A, int row, int n_row, int kcap, V2 p, const WorldInfo& wi) {
 	const int lane = threadIdx.x & 63;
 	const size_t st = (size_t)A.long_stride;
@@ -885,45 +940,9 @@ UD void rank_long_lists(const StepArgs& A, int row, int n_row, int kcap, V2 p, c
 	// (one wave per workgroup, so workgroup scope is the wave)
 	__builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup");
 	// rows of up to 16, 32 and 64 entries: 4, 2 and 1 per pass, one entry per lane
-	for (int lg = 4; lg <= 6; ++lg) {
-		const int SZ = 1 << lg, G = 64 >> lg;
-		unsigned long long m = __ballot(n_row > (lg == 4 ? 0 : SZ / 2) && n_row <= SZ);
-		while (m) {
-			// the pass's rows: the next G set bits of m (uniform), lane group g = lane >> lg
-			int rsel = -1;
-			for (int g = 0; g < G; ++g) {
-				const int r = m ? __builtin_ctzll(m) : -1;
-				if (m) m &= m - 1;
-				if ((lane >> lg) == g) rsel = r;
-			}
-			const int src = rsel < 0 ? 0 : rsel;          // every lane shuffles
-			const int n_src = __shfl(n_row, src), rw = __shfl(row, src);
-			const int n = rsel >= 0 ? n_src : 0;
-			const int i = lane & (SZ - 1);
-			unsigned long long k[2] = {kKeyPad, kKeyPad};
-			if (i < n) k[0] = A.long_key[(size_t)i * st + rw];
-			if (lg == 4) bitonic_keys_fixed<16>(k[0]);
-			else if (lg == 5) bitonic_keys_fixed<32>(k[0]);
-			else bitonic_keys_fixed<64>(k[0]);
-			// ties sit next to each other once sorted
-			const unsigned long long nx = shfl_u64(k[0], (lane + 1) & 63);
-			const int keep = n < kcap ? n : kcap;
-			if (__ballot(i + 1 < SZ && same_float_key(k[0], nx))) {
-				const float px = __shfl(p.x, src), py = __shfl(p.y, src);
-				const float W = __shfl(wi.W, src), H = __shfl(wi.H, src);
-				double d[2];
-				unsigned u[2];
-				int id[2];
-				exact_entry(S, px, py, W, H, k[0], d[0], u[0], id[0]);
-				if (lg == 4) bitonic_exact_fixed<16>(d[0], u[0], id[0]);
-				else if (lg == 5) bitonic_exact_fixed<32>(d[0], u[0], id[0]);
-				else bitonic_exact_fixed<64>(d[0], u[0], id[0]);
-				if (i < keep) A.long_u[(size_t)i * st + rw] = u[0];
-			} else if (i < keep) {
-				A.long_u[(size_t)i * st + rw] = key_entry(k[0]);
-			}
-		}
-	}
+	rank_rows<4>(A, row, n_row, kcap, p, wi, lane);
+	rank_rows<5>(A, row, n_row, kcap, p, wi, lane);
+	rank_rows<6>(A, row, n_row, kcap, p, wi, lane);
 	// rows of up to 128 entries: one per pass, two entries per lane
 	unsigned long long m = __ballot(n_row > 64);
 	while (m) {
