@@ -110,20 +110,45 @@ void launch_step_c(int kmax, int bs, const StepArgs& a, hipStream_t st) {
 	}
 }
 
-template <int KMAX>
-void launch_coop_t(const StepArgs& a, hipStream_t st) {
-	const int blocks = (a.n + 63) / 64;
-	if (blocks > 0) hipLaunchKernelGGL((k_step_coop<KMAX>), dim3(blocks), dim3(64), 0, st, a);
+// Agents per wave of the cooperative kernel: 64 while that makes at least ~4 waves per
+// SIMD (1024 SIMDs), else 32 (twice the waves, half the candidates per lane).
+int coop_agents_per_wave(int n) { return n >= 262144 ? 64 : 32; }
+
+template <int KMAX, int KIND>
+void launch_coop_k(const StepArgs& a, hipStream_t st) {
+	const int apw = coop_agents_per_wave(a.n);
+	const int blocks = (a.n + apw - 1) / apw;
+	if (blocks <= 0) return;
+	if (apw == 64) hipLaunchKernelGGL((k_step_coop<KMAX, 64, KIND>), dim3(blocks), dim3(64), 0, st, a);
+	else hipLaunchKernelGGL((k_step_coop<KMAX, 32, KIND>), dim3(blocks), dim3(64), 0, st, a);
 }
 
-void launch_step(int cls, int kmax, int bs, const StepArgs& a, hipStream_t st) {
+// kind: the one cost of every cooperative policy, or -1; instances of single kinds for
+// lists of up to 16 entries only
+template <int KMAX>
+void launch_coop_t(const StepArgs& a, hipStream_t st, int kind) {
+	if constexpr (KMAX <= 16) {
+		switch (kind) {
+		case KIND_KARAMOUZAS: launch_coop_k<KMAX, KIND_KARAMOUZAS>(a, st); return;
+		case KIND_RVO: launch_coop_k<KMAX, KIND_RVO>(a, st); return;
+		case KIND_MOUSSAID: launch_coop_k<KMAX, KIND_MOUSSAID>(a, st); return;
+		case KIND_TTCA_DCA: launch_coop_k<KMAX, KIND_TTCA_DCA>(a, st); return;
+		case KIND_PARIS: launch_coop_k<KMAX, KIND_PARIS>(a, st); return;
+		case KIND_VAN_TOLL: launch_coop_k<KMAX, KIND_VAN_TOLL>(a, st); return;
+		default: break;
+		}
+	}
+	launch_coop_k<KMAX, -1>(a, st);
+}
+
+void launch_step(int cls, int kmax, int bs, const StepArgs& a, hipStream_t st, int coop_kind = -1) {
 	if (cls == CLS_COOP) {
 		switch (kmax) {
-		case 12: launch_coop_t<12>(a, st); break;
-		case 16: launch_coop_t<16>(a, st); break;
-		case 32: launch_coop_t<32>(a, st); break;
-		case 64: launch_coop_t<64>(a, st); break;
-		default: launch_coop_t<128>(a, st); break;
+		case 12: launch_coop_t<12>(a, st, coop_kind); break;
+		case 16: launch_coop_t<16>(a, st, coop_kind); break;
+		case 32: launch_coop_t<32>(a, st, coop_kind); break;
+		case 64: launch_coop_t<64>(a, st, coop_kind); break;
+		default: launch_coop_t<128>(a, st, coop_kind); break;
 		}
 		return;
 	}
@@ -214,6 +239,7 @@ struct umans_ctx {
 	// policy classes: one step-kernel instance per class over a bucket of agents
 	std::vector<int> h_policy;           // policy of each uploaded agent
 	int cls_count[NUM_CLS] = {};
+	int coop_kind = -1;                     // the one cost of every cooperative-class policy, or -1
 	int cls_off[NUM_CLS] = {};              // first sorted index of each class
 	int cls_slot[NUM_CLS] = {};             // slot of each present class in the sort key
 	int nslots = 1;
@@ -677,6 +703,10 @@ int take_flags(umans_ctx* c, unsigned mask) {
 // get consecutive slots of the sort key, so class q occupies sorted indices
 // [cls_off[q], cls_off[q] + cls_count[q]).
 int update_classes(umans_ctx* c) {
+	c->coop_kind = -2;
+	for (const DevPolicy& P : c->hpol)
+		if (P.cls == CLS_COOP) c->coop_kind = (P.n_cost == 1 && (c->coop_kind == -2 || c->coop_kind == P.kind[0])) ? P.kind[0] : -1;
+	if (c->coop_kind < 0) c->coop_kind = -1;
 	for (int q = 0; q < NUM_CLS; ++q) c->cls_count[q] = 0;
 	// with obstacles every agent runs the generic class, the only one that reads edges
 	for (int p : c->h_policy) c->cls_count[c->n_edges > 0 ? CLS_GENERIC : c->hpol[p].cls]++;
@@ -935,7 +965,7 @@ int enqueue_step(umans_ctx* c, bool events, const State* ysave = nullptr, unsign
 			if (int rc = c->hip(hipStreamWaitEvent(sq, c->fork_ev, 0), "fork")) return rc;
 		}
 		if (q == CLS_COOP || kmax <= 32 || c->tier_first == 2) {
-			launch_step(q, kmax, class_block(q, kmax, bs, c->cls_block), aq, sq);
+			launch_step(q, kmax, class_block(q, kmax, bs, c->cls_block), aq, sq, c->coop_kind);
 		} else if (c->tier_first == 3) {
 			// Long lists: every lane collects its neighbours unsorted into its row of the
 			// list buffers and the wave ranks them; the agents with more than kLongCap in

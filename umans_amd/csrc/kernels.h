@@ -1633,9 +1633,26 @@ UD int regular_count(const DevPolicy& P) {
 	return P.s_include_base ? 1 + (P.s_speed_samples - 1) * P.s_angle_samples : P.s_speed_samples * P.s_angle_samples;
 }
 
-template <int KMAX>
+// The cost of candidate v for cost q of policy Q: KIND >= 0 when every policy of the
+// class has that one cost (only its code is compiled in: 127 instead of 181 registers for
+// Karamouzas, twice the resident waves), -1 for any mix.
+template <int KMAX, int KIND>
+UD float coop_cost(const DevPolicy& Q, int q, const Self& b, const NbList& L, const Prep<KMAX, CLS_GENERIC>& pr, V2 v) {
+	if constexpr (KIND == KIND_KARAMOUZAS) return karamouzas_cost(Q.params[q], Q.range[q], b, L, v, pr.ttcp[q]);
+	else if constexpr (KIND == KIND_RVO) return rvo_cost(Q.params[q], Q.range[q], b, L, v);
+	else if constexpr (KIND == KIND_MOUSSAID) return moussaid_cost(Q.params[q], Q.range[q], b, L, v);
+	else if constexpr (KIND == KIND_TTCA_DCA) return ttcadca_cost(Q.params[q], Q.range[q], b, L, v);
+	else if constexpr (KIND == KIND_PARIS) return paris_cost(Q.params[q], Q.range[q], b, L, v);
+	else if constexpr (KIND == KIND_VAN_TOLL) return vantoll_cost(Q.params[q], Q.range[q], b, L, v);
+	else return cost_of<KMAX, CLS_GENERIC>(Q, q, b, L, pr, v);
+}
+
+// APW agents per wave (64 or 32): fewer agents per wave make more waves, for crowds too
+// small to fill the SIMDs at 64 (the candidates of the wave's agents still cover all lanes).
+template <int KMAX, int APW, int KIND>
 __global__ __launch_bounds__(64) void k_step_coop(StepArgs A) {
 	constexpr int BS = 64;
+	static_assert(APW == 64 || APW == 32, "agents per wave");
 	constexpr bool kShort = KMAX <= kShortList;
 	constexpr int KC = short_buffer<KMAX>();
 	__shared__ float s_d2[kShort ? 1 : KMAX * BS];
@@ -1649,9 +1666,9 @@ __global__ __launch_bounds__(64) void k_step_coop(StepArgs A) {
 	__shared__ int s_cnt[BS], s_pi[BS], s_items[BS + 1];
 	__shared__ unsigned long long s_key[BS];
 	const int tid = threadIdx.x;
-	const int t = xcd_tile(blockIdx.x, gridDim.x) * BS + tid;
+	const int t = xcd_tile(blockIdx.x, gridDim.x) * APW + tid;
 	const int n_here = A.agent_count ? (int)*A.agent_count : A.n;
-	const bool active = t < n_here;
+	const bool active = tid < APW && t < n_here;
 	// one wave per block: a block without agents (a launch sized for a count known only
 	// on the device) leaves at once, before any barrier
 	if (__ballot(active) == 0ull) return;
@@ -1780,7 +1797,7 @@ __global__ __launch_bounds__(64) void k_step_coop(StepArgs A) {
 		const V2 v = regular_candidate(Q, c, base, bdir, dL);
 		const NbList Lb{s_nb + ag, BS, bcnt, S.pos, S.vel, S.prm, S.pref, bW, bH, b.pos, nullptr, 0};
 		float tc = 0.0f;
-		for (int q = 0; q < Q.n_cost; ++q) tc += Q.coeff[q] * cost_of<KMAX, CLS_GENERIC>(Q, q, b, Lb, pr, v);
+		for (int q = 0; q < Q.n_cost; ++q) tc += Q.coeff[q] * coop_cost<KMAX, KIND>(Q, q, b, Lb, pr, v);
 		const unsigned long long key = cost_key(tc, c);
 		if (key < local) local = key;
 	}
