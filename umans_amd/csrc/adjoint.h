@@ -250,9 +250,28 @@ UD D2 phantom_pos(const NbList& L, unsigned u) {
 	return p;
 }
 
-__global__ __launch_bounds__(128) void k_step_adjoint(AdjArgs A) {
-	const int k = blockIdx.x * blockDim.x + threadIdx.x;
-	if (k >= A.n) return;
+// Every lane gets the sum over the wave (a butterfly: each lane adds the same operands,
+// so the lanes agree bit for bit).
+UD double wave_sum(double v) {
+	for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
+	return v;
+}
+UD float lane_value(float v, int l) { return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), l)); }
+
+// The longest list the adjoint keeps (the step's saved copy of a list is at most this
+// long), and the entries per lane: entry e of an agent's list is handled by lane e mod 64.
+constexpr int kAdjListMax = 128;
+constexpr int kAdjSlots = kAdjListMax / 64;
+
+// One wave per agent: the lanes share the agent's neighbour list (entries lane,
+// lane + 64), so a 10k-agent crowd with ~80 neighbours each fills 10k waves instead of
+// 157, and each lane evaluates at most kAdjSlots pairs per pass. The float pass that
+// decides the clamp branches adds the pair forces in list order, as the forward step
+// does; the double values are summed over the wave.
+__global__ __launch_bounds__(256) void k_step_adjoint(AdjArgs A) {
+	const int lane = threadIdx.x & 63;
+	const int k = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
+	if (k >= A.n) return;                       // whole waves
 	const State& S = A.in;
 	const int4 meta = S.meta[k];
 	const int pi = meta.y & kPolicyMask;
@@ -266,57 +285,100 @@ __global__ __launch_bounds__(128) void k_step_adjoint(AdjArgs A) {
 	a.radius = prm.x; a.pref_speed = prm.y; a.max_speed = prm.z; a.max_accel = prm.w;
 	a.dt = wi.dt;
 	a.relaxation = P.relaxation;
-	const int cnt = A.nb_count[k];
+	const int cnt = A.nb_count[k];             // at most A.stride <= kUncappedMax
 	const NbList L{A.nb + (size_t)k * A.stride, 1, cnt, S.pos, S.vel, S.prm, S.pref, wi.W, wi.H, a.pos};
 	const unsigned row = (unsigned)meta.z;
 	const double dt = (double)a.dt;
 	const D2 pi_d = dv2(a.pos), vi_d = dv2(a.vel);
 
+	// this lane's entries
+	bool has[kAdjSlots];
+	Nb o[kAdjSlots];
+	D2 opos[kAdjSlots];
+	unsigned orow[kAdjSlots];
+#pragma unroll
+	for (int s = 0; s < kAdjSlots; ++s) {
+		const int e = lane + 64 * s;
+		has[s] = e < cnt;
+		if (has[s]) {
+			o[s] = L.get(e);
+			opos[s] = phantom_pos(L, L.raw(e));
+			orow[s] = row_of(S, o[s].j);
+		}
+	}
+
 	const D2 pbar_out = dv2(A.adj_pos_out[row]);
 	const D2 vbar_out = dv2(A.adj_vel_out[row]);
-	// p' = p + v' dt
-	D2 pbar = pbar_out;
+	// p' = p + v' dt; v' = next + C dt / 80
 	const D2 vtot = vbar_out + pbar_out * dt;   // adjoint of v'
-	D2 vbar{0.0, 0.0}, prefbar{0.0, 0.0};
-	// v' = next + C dt / 80
 	const D2 nextbar = vtot;
 	const D2 Cbar = vtot * (dt / 80.0);
+	// lane-local parts of the agent's own adjoints (summed over the wave at the end) and
+	// the wave-uniform parts (added once)
+	D2 pbar_l{0.0, 0.0}, vbar_l{0.0, 0.0};
+	double p0bar_l = 0.0, p1bar_l = 0.0;
+	D2 pbar_u = pbar_out, vbar_u{0.0, 0.0}, prefbar_u{0.0, 0.0};
 
 	// contact forces (policy.cpp:107-141): C_j = f (R - s) u, u = d / s, s = |d|
 	if (P.contact_scale > 0.0f) {
 		const double f = (double)P.contact_scale;
-		for (int e = 0; e < cnt; ++e) {
-			const Nb o = L.get(e);
-			const V2 d32 = a.pos - o.pos;
+#pragma unroll
+		for (int s = 0; s < kAdjSlots; ++s) {
+			if (!has[s]) continue;
+			const V2 d32 = a.pos - o[s].pos;
 			const float s32 = mag(d32);
-			const float inter32 = a.radius + o.radius - s32;
+			const float inter32 = a.radius + o[s].radius - s32;
 			if (!(inter32 > 0.0f) || !(s32 > 0.0f)) continue;
-			const D2 d = pi_d - phantom_pos(L, L.raw(e));
-			const double s = dmag(d);
-			const D2 u = d * (1.0 / s);
-			const double inter = (double)a.radius + (double)o.radius - s;
+			const D2 d = pi_d - opos[s];
+			const double sd = dmag(d);
+			const D2 u = d * (1.0 / sd);
+			const double inter = (double)a.radius + (double)o[s].radius - sd;
 			const double cu = ddot(Cbar, u);
 			// J = f [ (R - s)/s (I - u u^T) - u u^T ], symmetric
-			const D2 jd = f * ((inter / s) * (Cbar - cu * u) - cu * u);
-			pbar = pbar + jd;
-			atomic_add2(&A.adj_pos_in[row_of(S, o.j)], -jd);
+			const D2 jd = f * ((inter / sd) * (Cbar - cu * u) - cu * u);
+			pbar_l = pbar_l + jd;
+			atomic_add2(&A.adj_pos_in[orow[s]], -jd);
 		}
 	}
 
 	if (!is_zero(a.pref) && P.mode == MODE_GRADIENT) {
 		// the forward step's float acceleration and its clamp decisions (k_step, force
-		// gradient class: target = v + F / m dt, gradient = (v - target) / dt)
+		// gradient class: pair forces added in list order, target = v + F / m dt,
+		// gradient = (v - target) / dt), and which pairs exert a force
 		V2 F32[4];
-		for (int c = 0; c < P.n_cost; ++c) F32[c] = V2{0.0f, 0.0f};
-		for (int e = 0; e < cnt; ++e) {
-			const Nb o = L.get(e);
-			for (int c = 0; c < P.n_cost; ++c) {
-				const int kind = P.kind[c];
-				if (kind == KIND_GOAL_REACHING || !(o.dsq <= P.range[c] * P.range[c])) continue;
-				F32[c] = F32[c] + ((kind == KIND_POWER_LAW) ? power_law_pair(P.params[c], a, o)
-				                                            : social_force_pair(P.params[c], a, o, L.pref_of(o.j),
-				                                                                P.view_cos));
+		unsigned active = 0;                    // bit c * kAdjSlots + s
+		float sfscale[4][kAdjSlots];
+		for (int c = 0; c < P.n_cost; ++c) {
+			F32[c] = V2{0.0f, 0.0f};
+			const int kind = P.kind[c];
+			if (kind == KIND_GOAL_REACHING) continue;
+			const float rsq = P.range[c] * P.range[c];
+			V2 f[kAdjSlots];
+#pragma unroll
+			for (int s = 0; s < kAdjSlots; ++s) {
+				f[s] = V2{0.0f, 0.0f};
+				sfscale[c][s] = 1.0f;
+				if (!has[s] || !(o[s].dsq <= rsq)) continue;
+				bool act;
+				if (kind == KIND_POWER_LAW) {
+					f[s] = power_law_pair(P.params[c], a, o[s]);
+					act = pl_active(P.params[c], a, o[s]);
+				} else {
+					const V2 opref = L.pref_of(o[s].j);
+					f[s] = social_force_pair(P.params[c], a, o[s], opref, P.view_cos);
+					act = sf_active(P.params[c], a, o[s], opref, P.view_cos, sfscale[c][s]);
+				}
+				if (act) active |= 1u << (c * kAdjSlots + s);
 			}
+			// list order (a pair outside the range adds +0, which changes no float sum
+			// that starts at +0)
+			V2 F{0.0f, 0.0f};
+#pragma unroll
+			for (int s = 0; s < kAdjSlots; ++s) {
+				const int m = min(64, cnt - 64 * s);
+				for (int l = 0; l < m; ++l) F = F + V2{lane_value(f[s].x, l), lane_value(f[s].y, l)};
+			}
+			F32[c] = F;
 		}
 		V2 tg32{0.0f, 0.0f};
 		for (int c = 0; c < P.n_cost; ++c) {
@@ -329,7 +391,7 @@ __global__ __launch_bounds__(128) void k_step_adjoint(AdjArgs A) {
 		const bool acc_clamped = !(mag(acc32) <= a.max_accel);
 		const V2 w32 = a.vel + clamp_vec(acc32, a.max_accel) * a.dt;
 		const bool speed_clamped = !(mag(w32) <= a.max_speed);
-		if (A.clamps) A.clamps[row] = (unsigned char)((acc_clamped ? 1 : 0) | (speed_clamped ? 2 : 0));
+		if (A.clamps && lane == 0) A.clamps[row] = (unsigned char)((acc_clamped ? 1 : 0) | (speed_clamped ? 2 : 0));
 
 		// the same map in double: forces of the active pairs, acceleration, clamps
 		const double tr = (double)smax(a.relaxation, a.dt);
@@ -340,24 +402,22 @@ __global__ __launch_bounds__(128) void k_step_adjoint(AdjArgs A) {
 			if (kind == KIND_GOAL_REACHING) {
 				Fc = (dv2(a.pref) - vi_d) * (1.0 / tr);
 			} else {
-				for (int e = 0; e < cnt; ++e) {
-					const Nb o = L.get(e);
-					if (!(o.dsq <= P.range[c] * P.range[c])) continue;
-					const unsigned u = L.raw(e);
+				D2 part{0.0, 0.0};
+#pragma unroll
+				for (int s = 0; s < kAdjSlots; ++s) {
+					if (!(active >> (c * kAdjSlots + s) & 1u)) continue;
 					if (kind == KIND_SOCIAL_FORCES) {
-						float scale;
-						if (!sf_active(P.params[c], a, o, L.pref_of(o.j), P.view_cos, scale)) continue;
-						SFd s;
-						sf_eval(P.params[c], pi_d - phantom_pos(L, u), dv2(o.vel), dv2(L.pref_of(o.j)), scale, s);
-						Fc = Fc + s.g * s.R;
-					} else if (kind == KIND_POWER_LAW) {
-						if (!pl_active(P.params[c], a, o)) continue;
-						PLd s;
-						pl_eval(P.params[c], pi_d - phantom_pos(L, u), vi_d - dv2(o.vel),
-						        (double)o.radius + (double)a.radius, s);
-						Fc = Fc + s.comp1 * s.comp2;
+						SFd q;
+						sf_eval(P.params[c], pi_d - opos[s], dv2(o[s].vel), dv2(L.pref_of(o[s].j)), sfscale[c][s], q);
+						part = part + q.g * q.R;
+					} else {
+						PLd q;
+						pl_eval(P.params[c], pi_d - opos[s], vi_d - dv2(o[s].vel),
+						        (double)o[s].radius + (double)a.radius, q);
+						part = part + q.comp1 * q.comp2;
 					}
 				}
+				Fc = D2{wave_sum(part.x), wave_sum(part.y)};
 			}
 			acc = acc + ((double)P.coeff[c] / (double)P.params[c][0]) * Fc;
 		}
@@ -366,77 +426,108 @@ __global__ __launch_bounds__(128) void k_step_adjoint(AdjArgs A) {
 		const D2 w = vi_d + accc * dt;
 		// reverse
 		const D2 wbar = clamp_vjp(w, (double)a.max_speed, speed_clamped, nextbar);
-		vbar = vbar + wbar;
+		vbar_u = vbar_u + wbar;
 		const D2 accbar = clamp_vjp(acc, amax, acc_clamped, wbar * dt);
-		double p0bar = 0.0, p1bar = 0.0;
 		for (int c = 0; c < P.n_cost; ++c) {
 			const int kind = P.kind[c];
 			const D2 Fbar = ((double)P.coeff[c] / (double)P.params[c][0]) * accbar;
 			if (kind == KIND_GOAL_REACHING) {
-				prefbar = prefbar + Fbar * (1.0 / tr);
-				vbar = vbar - Fbar * (1.0 / tr);
+				prefbar_u = prefbar_u + Fbar * (1.0 / tr);
+				vbar_u = vbar_u - Fbar * (1.0 / tr);
 				continue;
 			}
-			for (int e = 0; e < cnt; ++e) {
-				const Nb o = L.get(e);
-				if (!(o.dsq <= P.range[c] * P.range[c])) continue;
-				const unsigned u = L.raw(e);
-				const unsigned rj = row_of(S, o.j);
+#pragma unroll
+			for (int s = 0; s < kAdjSlots; ++s) {
+				if (!(active >> (c * kAdjSlots + s) & 1u)) continue;
 				if (kind == KIND_SOCIAL_FORCES) {
-					float scale;
-					if (!sf_active(P.params[c], a, o, L.pref_of(o.j), P.view_cos, scale)) continue;
-					SFd s;
-					const D2 vj = dv2(o.vel);
-					sf_eval(P.params[c], pi_d - phantom_pos(L, u), vj, dv2(L.pref_of(o.j)), scale, s);
+					SFd q;
+					const D2 vj = dv2(o[s].vel);
+					sf_eval(P.params[c], pi_d - opos[s], vj, dv2(L.pref_of(o[s].j)), sfscale[c][s], q);
 					D2 Rbar, vjbar, prefjbar;
 					double V0b, sgb;
-					sf_vjp(P.params[c], s, vj, Fbar, Rbar, vjbar, prefjbar, V0b, sgb);
-					pbar = pbar + Rbar;
-					atomic_add2(&A.adj_pos_in[rj], -Rbar);
-					atomic_add2(&A.adj_vel_in[rj], vjbar);
-					atomic_add2(&A.adj_pref[rj], prefjbar);
-					if (A.param_kind == KIND_SOCIAL_FORCES) { p0bar += V0b; p1bar += sgb; }
-				} else if (kind == KIND_POWER_LAW) {
-					if (!pl_active(P.params[c], a, o)) continue;
-					PLd s;
-					pl_eval(P.params[c], pi_d - phantom_pos(L, u), vi_d - dv2(o.vel),
-					        (double)o.radius + (double)a.radius, s);
+					sf_vjp(P.params[c], q, vj, Fbar, Rbar, vjbar, prefjbar, V0b, sgb);
+					pbar_l = pbar_l + Rbar;
+					atomic_add2(&A.adj_pos_in[orow[s]], -Rbar);
+					atomic_add2(&A.adj_vel_in[orow[s]], vjbar);
+					atomic_add2(&A.adj_pref[orow[s]], prefjbar);
+					if (A.param_kind == KIND_SOCIAL_FORCES) { p0bar_l += V0b; p1bar_l += sgb; }
+				} else {
+					PLd q;
+					pl_eval(P.params[c], pi_d - opos[s], vi_d - dv2(o[s].vel), (double)o[s].radius + (double)a.radius,
+					        q);
 					D2 xbar, vrbar;
 					double kb, t0b;
-					pl_vjp(P.params[c], s, Fbar, xbar, vrbar, kb, t0b);
-					pbar = pbar + xbar;
-					vbar = vbar + vrbar;
-					atomic_add2(&A.adj_pos_in[rj], -xbar);
-					atomic_add2(&A.adj_vel_in[rj], -vrbar);
-					if (A.param_kind == KIND_POWER_LAW) { p0bar += kb; p1bar += t0b; }
+					pl_vjp(P.params[c], q, Fbar, xbar, vrbar, kb, t0b);
+					pbar_l = pbar_l + xbar;
+					vbar_l = vbar_l + vrbar;
+					atomic_add2(&A.adj_pos_in[orow[s]], -xbar);
+					atomic_add2(&A.adj_vel_in[orow[s]], -vrbar);
+					if (A.param_kind == KIND_POWER_LAW) { p0bar_l += kb; p1bar_l += t0b; }
 				}
 			}
 		}
-		double* pc = A.adj_params + (size_t)stat_copy() * A.param_stride;
-		if (p0bar != 0.0) atomicAdd(&pc[2 * pi + 0], p0bar);
-		if (p1bar != 0.0) atomicAdd(&pc[2 * pi + 1], p1bar);
 	}
-	atomic_add2(&A.adj_pos_in[row], pbar);
-	atomic_add2(&A.adj_vel_in[row], vbar);
-	atomic_add2(&A.adj_pref[row], prefbar);
+	const D2 pbar = pbar_u + D2{wave_sum(pbar_l.x), wave_sum(pbar_l.y)};
+	const D2 vbar = vbar_u + D2{wave_sum(vbar_l.x), wave_sum(vbar_l.y)};
+	const double p0bar = wave_sum(p0bar_l), p1bar = wave_sum(p1bar_l);
+	if (lane == 0) {
+		if (p0bar != 0.0 || p1bar != 0.0) {
+			double* pc = A.adj_params + (size_t)stat_copy() * A.param_stride;
+			if (p0bar != 0.0) atomicAdd(&pc[2 * pi + 0], p0bar);
+			if (p1bar != 0.0) atomicAdd(&pc[2 * pi + 1], p1bar);
+		}
+		atomic_add2(&A.adj_pos_in[row], pbar);
+		atomic_add2(&A.adj_vel_in[row], vbar);
+		atomic_add2(&A.adj_pref[row], prefbar_u);
+	}
 }
 
-// pref = ps (g - p) / |g - p| unless the goal is reached: adjoint to the position.
-__global__ __launch_bounds__(256) void k_pref_adjoint(State in, int n, const double2* adj_pref, double2* adj_pos) {
+// Minimum-image difference p - t in a toric world, plain difference otherwise.
+UD D2 loss_diff(const WorldInfo& wi, float2 p, float2 t) {
+	double dx = (double)p.x - (double)t.x, dy = (double)p.y - (double)t.y;
+	if (wi.world_type == 1) {
+		if (dx > 0.5 * (double)wi.W) dx -= (double)wi.W; else if (dx < -0.5 * (double)wi.W) dx += (double)wi.W;
+		if (dy > 0.5 * (double)wi.H) dy -= (double)wi.H; else if (dy < -0.5 * (double)wi.H) dy += (double)wi.H;
+	}
+	return D2{dx, dy};
+}
+
+// After the reverse step of step t, per agent of its sorted input state `in`:
+//  * the adjoint of the preferred velocity, pref = ps (g - p) / |g - p| unless the goal
+//    is reached (agent.cpp:52-59, decided as k_reorder decides it), moved to the position;
+//  * with `target` (the targets of step t - 1), the loss gradient scale * 2 (p - target)
+//    of step t - 1's output added: that output is step t's input state;
+//  * the preferred-velocity adjoint and the buffers clr0, clr1 (the next reverse step
+//    accumulates into them) cleared.
+// Every row is one agent's, handled by one thread.
+__global__ __launch_bounds__(256) void k_adj_link(State in, int n, const WorldInfo* worlds, double2* adj_pref,
+                                                 double2* adj_pos, const float2* target, double scale, double2* clr0,
+                                                 double2* clr1) {
 	const int k = blockIdx.x * blockDim.x + threadIdx.x;
 	if (k >= n) return;
-	const unsigned row = (unsigned)in.meta[k].z;
+	const int4 meta = in.meta[k];
+	const unsigned row = (unsigned)meta.z;
 	const float4 prm = in.prm[k];
-	// the forward step's decision (k_reorder)
-	if (sqr_mag(v2(in.goal[k]) - v2(in.pos[k])) <= prm.x * prm.x) return;
-	const D2 pb = dv2(adj_pref[row]);
-	const D2 d = dv2(in.goal[k]) - dv2(in.pos[k]);
-	const double m = dmag(d);
-	const D2 u = d * (1.0 / m);
-	const double pu = ddot(pb, u);
-	const double s = (double)prm.y / m;
-	// d pref / d p = -(ps / m) (I - u u^T)
-	atomic_add2(&adj_pos[row], -s * (pb - pu * u));
+	const float2 p32 = in.pos[k];
+	D2 p = dv2(adj_pos[row]);
+	if (!(sqr_mag(v2(in.goal[k]) - v2(p32)) <= prm.x * prm.x)) {
+		const D2 pb = dv2(adj_pref[row]);
+		const D2 d = dv2(in.goal[k]) - dv2(p32);
+		const double m = dmag(d);
+		const D2 u = d * (1.0 / m);
+		const double pu = ddot(pb, u);
+		const double s = (double)prm.y / m;
+		// d pref / d p = -(ps / m) (I - u u^T)
+		p = p - s * (pb - pu * u);
+	}
+	if (target) {
+		const D2 d = loss_diff(worlds[meta.y >> kPolicyBits], p32, target[row]);
+		p = p + (scale * 2.0) * d;
+	}
+	adj_pos[row] = make_double2(p.x, p.y);
+	adj_pref[row] = make_double2(0.0, 0.0);
+	clr0[row] = make_double2(0.0, 0.0);
+	clr1[row] = make_double2(0.0, 0.0);
 }
 
 // Loss of one step's output: sum over agents of |p - target|^2 with the minimum-image
@@ -448,19 +539,13 @@ __global__ __launch_bounds__(256) void k_loss(State out, int n, const WorldInfo*
 	double l = 0.0;
 	if (k < n) {
 		const int4 meta = out.meta[k];
-		const WorldInfo wi = worlds[meta.y >> kPolicyBits];
 		const unsigned row = (unsigned)meta.z;
-		const float2 p = out.pos[k], t = target[row];
-		double dx = (double)p.x - (double)t.x, dy = (double)p.y - (double)t.y;
-		if (wi.world_type == 1) {
-			if (dx > 0.5 * (double)wi.W) dx -= (double)wi.W; else if (dx < -0.5 * (double)wi.W) dx += (double)wi.W;
-			if (dy > 0.5 * (double)wi.H) dy -= (double)wi.H; else if (dy < -0.5 * (double)wi.H) dy += (double)wi.H;
-		}
-		l = dx * dx + dy * dy;
+		const D2 d = loss_diff(worlds[meta.y >> kPolicyBits], out.pos[k], target[row]);
+		l = d.x * d.x + d.y * d.y;
 		if (grad) {
 			double2 g = grad[row];
-			g.x += scale * 2.0 * dx;
-			g.y += scale * 2.0 * dy;
+			g.x += scale * 2.0 * d.x;
+			g.y += scale * 2.0 * d.y;
 			grad[row] = g;
 		}
 	}
@@ -471,12 +556,6 @@ __global__ __launch_bounds__(256) void k_loss(State out, int n, const WorldInfo*
 __global__ void k_fill_f(float* p, long long n, float v) {
 	const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
 	if (i < n) p[i] = v;
-}
-
-// y += x (adjoint vectors)
-__global__ void k_axpy2(double2* y, const double2* x, int n) {
-	const int i = blockIdx.x * blockDim.x + threadIdx.x;
-	if (i < n) { y[i].x += x[i].x; y[i].y += x[i].y; }
 }
 
 // the adjoint vectors handed out in float

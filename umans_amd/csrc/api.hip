@@ -25,6 +25,7 @@ namespace {
 constexpr int kMaxPolicies = 1 << kPolicyBits;
 constexpr int kMaxWorlds = (1 << (31 - kPolicyBits)) - 1;
 constexpr int kUncappedMax = 128;
+static_assert(kUncappedMax <= kAdjListMax, "the adjoint's lanes cover the longest saved list");
 constexpr int kRecordUncapped = 512;   // neighbour-record entries per agent with no cap
 constexpr int kHugeBlocks = 256;       // blocks (one wave each) of the very-long-list launch
 
@@ -280,7 +281,6 @@ struct umans_ctx {
 	float2* targets = nullptr;           // [steps][n], upload-row order
 	unsigned* seg_nb = nullptr;          // [ck][cap][KMAX]
 	int* seg_cnt = nullptr;              // [ck][cap]
-	double2* seg_lg = nullptr;           // [ck][n] loss gradient of each step's output
 	double2* adj_buf = nullptr;          // 5 x n: p, v, p', v', pref
 	float2* adj_out = nullptr;           // n: an adjoint vector handed out in float
 	unsigned char* adj_clamps = nullptr; // [steps][n] clamp branches of the reverse pass (row order)
@@ -375,14 +375,14 @@ struct umans_ctx {
 		for (DeviceState& d : seg_states) d.release();
 		ckpts.clear();
 		seg_states.clear();
-		(void)hipFree(targets); (void)hipFree(seg_nb); (void)hipFree(seg_cnt); (void)hipFree(seg_lg); (void)hipFree(adj_buf);
+		(void)hipFree(targets); (void)hipFree(seg_nb); (void)hipFree(seg_cnt); (void)hipFree(adj_buf);
 		(void)hipFree(adj_out);
 		(void)hipFree(adj_clamps);
 		adj_out = nullptr;
 		adj_clamps = nullptr;
 		adj_clamps_ready = false;
 		(void)hipFree(adj_params); (void)hipFree(dloss);
-		targets = nullptr; seg_nb = nullptr; seg_cnt = nullptr; seg_lg = nullptr; adj_buf = nullptr;
+		targets = nullptr; seg_nb = nullptr; seg_cnt = nullptr; adj_buf = nullptr;
 		adj_params = nullptr; dloss = nullptr;
 		adj_ready = false;
 		adj_steps = adj_ckpt = adj_n = 0;
@@ -1981,7 +1981,6 @@ int umans_adjoint_forward(umans_ctx* c, const umans_adjoint_desc* d, const float
 	if ((rc = c->hip(dalloc(&c->targets, (size_t)T * n), "alloc targets"))) return rc;
 	if ((rc = c->hip(dalloc(&c->seg_nb, (size_t)segn * n * kmax), "alloc lists"))) return rc;
 	if ((rc = c->hip(dalloc(&c->seg_cnt, (size_t)segn * n), "alloc lists"))) return rc;
-	if ((rc = c->hip(dalloc(&c->seg_lg, (size_t)segn * n), "alloc loss grad"))) return rc;
 	if ((rc = c->hip(dalloc(&c->adj_buf, (size_t)5 * n), "alloc adjoint"))) return rc;
 	if ((rc = c->hip(dalloc(&c->adj_out, (size_t)n), "alloc adjoint"))) return rc;
 	if ((rc = c->hip(dalloc(&c->adj_clamps, (size_t)T * n), "alloc adjoint"))) return rc;
@@ -1994,8 +1993,8 @@ int umans_adjoint_forward(umans_ctx* c, const umans_adjoint_desc* d, const float
 	fill_f(c, c->dloss, 2);
 	for (int t = 0; t < T; ++t) {
 		if (t % ck == 0 && (rc = c->hip(copy_state(c->ckpts[t / ck].s, c->X.s, n, c->stream), "checkpoint"))) return rc;
-		if ((rc = enqueue_step(c, false))) return rc;
-		launch_loss(c, c->targets + (size_t)t * n, c->dloss, nullptr, 0.0f);
+		if ((rc = enqueue_step(c, false, nullptr, nullptr, nullptr, t + 1 < T))) return rc;
+		launch_loss(c, c->targets + (size_t)t * n, c->dloss, nullptr, 0.0);
 		c->steps += 1;
 		c->sim_time += (double)c->worlds[0].delta_time;
 	}
@@ -2039,16 +2038,13 @@ int umans_adjoint_backward(umans_ctx* c, float* d_params, float* d_pos0, float* 
 		for (int t = t0; t < t1; ++t) {
 			const int s = t - t0;
 			if ((rc = enqueue_step(c, false, &c->seg_states[s].s, c->seg_nb + (size_t)s * n * kmax,
-			                       c->seg_cnt + (size_t)s * n))) return rc;
-			fill_f(c, c->seg_lg + (size_t)s * n, 4LL * n);
-			launch_loss(c, c->targets + (size_t)t * n, nullptr, c->seg_lg + (size_t)s * n, scale);
+			                       c->seg_cnt + (size_t)s * n, t + 1 < t1))) return rc;
 		}
+		// the loss gradient of the last step's output; those of the earlier steps are
+		// added by k_adj_link from the next step's input state
+		if (seg == nseg - 1) launch_loss(c, c->targets + (size_t)(T - 1) * n, nullptr, ap, scale);
 		for (int t = t1 - 1; t >= t0; --t) {
 			const int s = t - t0;
-			hipLaunchKernelGGL(k_axpy2, dim3(ceil_div(n, 256)), dim3(256), 0, c->stream, ap, c->seg_lg + (size_t)s * n, n);
-			fill_f(c, ap2, 4LL * n);
-			fill_f(c, av2, 4LL * n);
-			fill_f(c, apref, 4LL * n);
 			AdjArgs A{};
 			A.in = c->seg_states[s].s;
 			A.nb_count = c->seg_cnt + (size_t)s * n;
@@ -2066,9 +2062,10 @@ int umans_adjoint_backward(umans_ctx* c, float* d_params, float* d_pos0, float* 
 			A.param_stride = pstride;
 			A.param_kind = c->adj_param_kind;
 			A.clamps = c->adj_clamps + (size_t)t * n;
-			hipLaunchKernelGGL(k_step_adjoint, dim3(ceil_div(n, 128)), dim3(128), 0, c->stream, A);
-			hipLaunchKernelGGL(k_pref_adjoint, dim3(ceil_div(n, 256)), dim3(256), 0, c->stream, c->seg_states[s].s, n,
-			                   (const double2*)apref, ap2);
+			hipLaunchKernelGGL(k_step_adjoint, dim3(ceil_div(n, 4)), dim3(256), 0, c->stream, A);
+			hipLaunchKernelGGL(k_adj_link, dim3(ceil_div(n, 256)), dim3(256), 0, c->stream, c->seg_states[s].s, n,
+			                   c->dworlds, apref, ap2, t > 0 ? c->targets + (size_t)(t - 1) * n : nullptr, scale,
+			                   ap, av);
 			std::swap(ap, ap2);
 			std::swap(av, av2);
 		}
