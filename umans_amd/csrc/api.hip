@@ -3,6 +3,7 @@
 #include "umans_gpu.h"
 #include "adjoint.h"
 #include "population.h"
+#include "step_launch.h"
 
 #include <sys/stat.h>
 
@@ -82,82 +83,15 @@ bool block_supported(int kmax, int bs) {
 }
 int default_block(int kmax) { return kmax <= 32 ? 128 : 64; }
 
-template <int KMAX, int BS, int CLS>
-void launch_step_t(const StepArgs& a, hipStream_t st) {
-	const int blocks = (a.n + BS - 1) / BS;
-	if (blocks > 0) hipLaunchKernelGGL((k_step<KMAX, BS, CLS>), dim3(blocks), dim3(BS), 0, st, a);
-}
-
-template <int CLS>
-void launch_step_c(int kmax, int bs, const StepArgs& a, hipStream_t st) {
-	switch (kmax) {
-	case 12:
-		if (bs == 64) launch_step_t<12, 64, CLS>(a, st);
-		else if (bs == 256) launch_step_t<12, 256, CLS>(a, st);
-		else launch_step_t<12, 128, CLS>(a, st);
-		break;
-	case 16:
-		if (bs == 64) launch_step_t<16, 64, CLS>(a, st);
-		else if (bs == 256) launch_step_t<16, 256, CLS>(a, st);
-		else launch_step_t<16, 128, CLS>(a, st);
-		break;
-	case 32:
-		if (bs == 64) launch_step_t<32, 64, CLS>(a, st);
-		else launch_step_t<32, 128, CLS>(a, st);
-		break;
-	case 64: launch_step_t<64, 64, CLS>(a, st); break;
-	case 0: launch_step_t<0, 64, CLS>(a, st); break;     // long lists, ranked per wave
-	default: launch_step_t<128, 64, CLS>(a, st); break;
-	}
-}
-
-// Agents per wave of the cooperative kernel: 64 while that makes at least ~4 waves per
-// SIMD (1024 SIMDs), else 32 (twice the waves, half the candidates per lane).
-int coop_agents_per_wave(int n) { return n >= 262144 ? 64 : 32; }
-
-template <int KMAX, int KIND>
-void launch_coop_k(const StepArgs& a, hipStream_t st) {
-	const int apw = coop_agents_per_wave(a.n);
-	const int blocks = (a.n + apw - 1) / apw;
-	if (blocks <= 0) return;
-	if (apw == 64) hipLaunchKernelGGL((k_step_coop<KMAX, 64, KIND>), dim3(blocks), dim3(64), 0, st, a);
-	else hipLaunchKernelGGL((k_step_coop<KMAX, 32, KIND>), dim3(blocks), dim3(64), 0, st, a);
-}
-
-// kind: the one cost of every cooperative policy, or -1; instances of single kinds for
-// lists of up to 16 entries only
-template <int KMAX>
-void launch_coop_t(const StepArgs& a, hipStream_t st, int kind) {
-	if constexpr (KMAX <= 16) {
-		switch (kind) {
-		case KIND_KARAMOUZAS: launch_coop_k<KMAX, KIND_KARAMOUZAS>(a, st); return;
-		case KIND_RVO: launch_coop_k<KMAX, KIND_RVO>(a, st); return;
-		case KIND_MOUSSAID: launch_coop_k<KMAX, KIND_MOUSSAID>(a, st); return;
-		case KIND_TTCA_DCA: launch_coop_k<KMAX, KIND_TTCA_DCA>(a, st); return;
-		case KIND_PARIS: launch_coop_k<KMAX, KIND_PARIS>(a, st); return;
-		case KIND_VAN_TOLL: launch_coop_k<KMAX, KIND_VAN_TOLL>(a, st); return;
-		default: break;
-		}
-	}
-	launch_coop_k<KMAX, -1>(a, st);
-}
-
 void launch_step(int cls, int kmax, int bs, const StepArgs& a, hipStream_t st, int coop_kind = -1) {
-	if (cls == CLS_COOP) {
-		switch (kmax) {
-		case 12: launch_coop_t<12>(a, st, coop_kind); break;
-		case 16: launch_coop_t<16>(a, st, coop_kind); break;
-		case 32: launch_coop_t<32>(a, st, coop_kind); break;
-		case 64: launch_coop_t<64>(a, st, coop_kind); break;
-		default: launch_coop_t<128>(a, st, coop_kind); break;
-		}
-		return;
-	}
+	static_assert(CLS_FORCE_GRAD == 0 && CLS_FORCE_SAMPLE == 1 && CLS_ORCA == 2 && CLS_GENERIC == 3, "class numbers");
+	const bool short_list = kmax == 12 || kmax == 16;
 	switch (cls) {
-	case CLS_FORCE_GRAD: launch_step_c<CLS_FORCE_GRAD>(kmax, bs, a, st); break;
-	case CLS_FORCE_SAMPLE: launch_step_c<CLS_FORCE_SAMPLE>(kmax, bs, a, st); break;
-	case CLS_ORCA: launch_step_c<CLS_ORCA>(kmax, bs, a, st); break;
-	default: launch_step_c<CLS_GENERIC>(kmax, bs, a, st); break;
+	case CLS_COOP: launch_step_coop(kmax, a, st, coop_kind); break;
+	case CLS_FORCE_GRAD: (short_list ? launch_step_c0_p0 : launch_step_c0_p1)(kmax, bs, a, st); break;
+	case CLS_FORCE_SAMPLE: (short_list ? launch_step_c1_p0 : launch_step_c1_p1)(kmax, bs, a, st); break;
+	case CLS_ORCA: (short_list ? launch_step_c2_p0 : launch_step_c2_p1)(kmax, bs, a, st); break;
+	default: (short_list ? launch_step_c3_p0 : launch_step_c3_p1)(kmax, bs, a, st); break;
 	}
 }
 

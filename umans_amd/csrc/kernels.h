@@ -24,10 +24,12 @@
 namespace umans {
 
 // ------------------------------------------------------------------ utilities
+#ifndef UMANS_STEP_KERNELS_ONLY
 __global__ void k_fill(unsigned* p, int n, unsigned v) {
 	const int i = blockIdx.x * blockDim.x + threadIdx.x;
 	if (i < n) p[i] = v;
 }
+#endif
 
 UD int ordered_key(float f) {
 	const int i = __float_as_int(f);
@@ -54,6 +56,7 @@ UD int wave_max_i(int v) {
 // the grid set-up reduces them.
 constexpr int kBoxCopies = 8;
 constexpr int kBoxLine = 32;
+#ifndef UMANS_STEP_KERNELS_ONLY
 __global__ void k_bbox_init(int* bbox, int nworlds) {
 	const int i = blockIdx.x * blockDim.x + threadIdx.x;
 	if (i < nworlds * kBoxCopies) {
@@ -62,6 +65,7 @@ __global__ void k_bbox_init(int* bbox, int nworlds) {
 		b[2] = (int)0x80000000; b[3] = (int)0x80000000;
 	}
 }
+#endif
 // Bounding boxes of the infinite worlds: agent i's position added to its world's box
 // (every lane of the wave takes part; lanes with i >= n add nothing). Agents of a batch
 // are grouped by world, so a wave whose lanes share one world reduces first and issues
@@ -96,19 +100,24 @@ UD void bbox_add(const float2* pos, const int4* meta, const WorldInfo* worlds, i
 		atomicMax(&b[2], mxx); atomicMax(&b[3], mxy);
 	}
 }
+#ifndef UMANS_STEP_KERNELS_ONLY
 __global__ void k_copy_grids(const GridInfo* src, GridInfo* dst, int nworlds) {
 	const int i = blockIdx.x * blockDim.x + threadIdx.x;
 	if (i < nworlds) dst[i] = src[i];
 }
+#endif
+#ifndef UMANS_STEP_KERNELS_ONLY
 __global__ __launch_bounds__(256) void k_bbox(const float2* pos, const int4* meta, const WorldInfo* worlds,
                                              int n, int* bbox) {
 	bbox_add(pos, meta, worlds, n, blockIdx.x * blockDim.x + threadIdx.x, bbox);
 }
+#endif
 // One thread per world. Infinite worlds: a grid over the bounding box with cells of
 // at least cell_size metres and at most cell_cap cells. Toric worlds: the world
 // rectangle centred on the origin with cells at least cell_size wide.
 // Also clears the step's counters (zero0[0..n0), zero1[0..n1)), saving their launches.
 // reset_box: the boxes, once read, start again empty (the next reorder fills them).
+#ifndef UMANS_STEP_KERNELS_ONLY
 __global__ void k_grid_setup(const WorldInfo* worlds, int* bbox, GridInfo* grids, int nworlds, unsigned* zero0,
                              int n0, unsigned* zero1, int n1, int reset_box) {
 	const int wi = blockIdx.x * blockDim.x + threadIdx.x;
@@ -157,6 +166,7 @@ __global__ void k_grid_setup(const WorldInfo* worlds, int* bbox, GridInfo* grids
 	r.ncells = r.ncx * r.ncy;
 	grids[wi] = r;
 }
+#endif
 
 // Cell index along one axis: monotone non-decreasing in x, clamped into the grid
 // (non-finite coordinates land in cell 0).
@@ -170,6 +180,7 @@ UD int bin_axis(float x, float o, float inv, int nc) {
 // numbers each class's agents in cell order (the class kernels' work lists); the state
 // itself is stored cell-major, every class of a cell together, so the neighbour search
 // walks one contiguous range per row of cells whatever the classes.
+#ifndef UMANS_STEP_KERNELS_ONLY
 __global__ __launch_bounds__(256) void k_count(const float2* pos, const int4* meta, int n, const WorldInfo* worlds,
                                               const GridInfo* grids, const DevPolicy* pol, const int* cls_slot,
                                               int slot_cells, unsigned* cell_count, unsigned* cell_of) {
@@ -185,6 +196,7 @@ __global__ __launch_bounds__(256) void k_count(const float2* pos, const int4* me
 	cell_of[k] = (unsigned)c;
 	atomicAdd(&cell_count[c], 1u);
 }
+#endif
 
 // k_count's work for agent k of the next step, done by the step kernel that computed its
 // new position (same key: the class slot of the launch, the world's fixed grid).
@@ -242,6 +254,7 @@ UD unsigned ld_status(unsigned* p) {
 UD void st_status(unsigned* p, unsigned v) {
 	__hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
+#ifndef UMANS_STEP_KERNELS_ONLY
 __global__ __launch_bounds__(kScanThreads) void k_scan_lookback(const unsigned* in, unsigned* out, unsigned* ticket,
                                                                 unsigned* status, int n, int nb, unsigned* zero0,
                                                                 int n0, unsigned* zero1, int n1, int spin_limit) {
@@ -339,6 +352,7 @@ __global__ __launch_bounds__(kScanThreads) void k_scan_lookback(const unsigned* 
 			if (base + i < n) out[base + i] = v[i];
 	}
 }
+#endif
 
 // --------------------------------------------------------------------- reorder
 // Cell-major start of cell c from the class-major scan cs of the (slot, cell) counts:
@@ -362,6 +376,7 @@ UD unsigned cell_first_of(const unsigned* cs, unsigned c, int nslots, unsigned s
 // their infinite worlds, from which the next step's grids are set up: one step old, they
 // may leave an agent outside its grid, which the clamped binning puts in an edge cell,
 // so lists and results do not depend on it.
+#ifndef UMANS_STEP_KERNELS_ONLY
 __global__ __launch_bounds__(256) void k_reorder(State src, State dst, int n, const unsigned* cell_of,
                                                 unsigned* cell_count, const unsigned* class_start, int nslots,
                                                 int slot_cells, unsigned* cell_first, int* class_list,
@@ -399,6 +414,7 @@ __global__ __launch_bounds__(256) void k_reorder(State src, State dst, int n, co
 	if (!(sqr_mag(toGoal) <= prm.x * prm.x)) pref = normalized(toGoal) * prm.y;
 	dst.pref[d] = f2(pref);
 }
+#endif
 
 // ------------------------------------------------------------ neighbour search
 // All agents j with double d2 = (qx - x_j)^2 + (qy - y_j)^2 < double(float(r*r)), the
@@ -1478,6 +1494,7 @@ struct HugeArgs {
 	Line* lines;             // 2 * kHugeMax per block: half-planes, projected lines
 };
 
+#ifndef UMANS_STEP_KERNELS_ONLY
 __global__ __launch_bounds__(64) void k_step_huge(HugeArgs H) {
 	__shared__ unsigned long long s_key[kHugeMax];
 	__shared__ unsigned s_u[kHugeMax];
@@ -1597,6 +1614,7 @@ __global__ __launch_bounds__(64) void k_step_huge(HugeArgs H) {
 		__syncthreads();          // the next agent reuses the LDS lists
 	}
 }
+#endif
 
 
 // --------------------------------------------------------- cooperative sampling
