@@ -260,6 +260,31 @@ def bidirectional_leg(args):
     return out
 
 
+def descent_leg(args, tuning):
+    """The 1M-agent mixed crowd with its SocialForces class descending 20 iterations of
+    step 0.1 per frame (the gradient method's descent extension, umans_policy::
+    gradient_iterations; the reference has only the single step, so there is no CPU
+    baseline), at the launch tuning of the headline run."""
+    from umans_amd import gpu
+    sc = S.mixed()
+    pols = [dict(p) for p in sc.policies]
+    pols[1].update(gradient_iterations=20, gradient_step=0.1)
+    w = gpu.World(sc.world_type, sc.width, sc.height, sc.dt, pols, sc.max_neighbors)
+    w.upload(sc.agents)
+    if tuning:
+        w.set_tuning(tuning["cell_factor"], tuning["block_size"], list_capacity=tuning.get("list_capacity", 0),
+                     class_block=tuning.get("class_block") or (0, 0, 0, 0, 0))
+    steps = 20
+    t, prof = gpu_run(w, steps, 3, True, 5)
+    w.close()
+    return {"metric": "agent-steps/s at 1M agents", "value": sc.n * steps / t, "unit": "agent-steps/s",
+            "workload": sc.name, "steps": steps, "ms_per_step": t * 1e3 / steps,
+            "phases_ms": {"grid": prof["grid_ms"] / max(prof["steps"], 1),
+                          "step": prof["step_ms"] / max(prof["steps"], 1)},
+            "config": {"policies": "as the headline, with the SocialForces class descending 20 iterations of "
+                                   "step 0.1 per frame (GradientIterations=20, GradientStepSize=0.1)"}}
+
+
 def batched_leg(args):
     from umans_amd import gpu
     sw, flops = workload("sweep")
@@ -453,6 +478,7 @@ def main():
             out["cpu_baseline"] = cb
     if not args.no_extras and mixed:
         print("# timing the bidirectional, batched-sweep and adjoint legs", file=sys.stderr, flush=True)
+        out["descent"] = descent_leg(args, tuning)
         out["bidirectional"] = bidirectional_leg(args)
         out["batched"] = batched_leg(args)
         out["adjoint"] = adjoint_leg(args)

@@ -76,6 +76,17 @@ typedef struct {
 	float cost_coeff[UMANS_MAX_COSTS];
 	float cost_range[UMANS_MAX_COSTS];
 	float cost_params[UMANS_MAX_COSTS][UMANS_MAX_PARAMS];
+	/* Extension, not in the reference (gradient method only; XML attributes
+	 * GradientIterations and GradientStepSize on <Policy>, which the reference's loader
+	 * ignores). With gradient_iterations >= 2 the method descends
+	 *   v <- v - (gradient_step * dt) * sum_c coeff_c grad C_c(v)
+	 * that many times from the current velocity, each gradient evaluated at the current
+	 * iterate as CostFunction::GetGradient evaluates it, and takes the acceleration
+	 * (v_N - v) / dt before the clamps. 0 or 1 keeps the reference's single step,
+	 * acceleration = -sum_c coeff_c grad C_c(v) (policy.cpp:78-89); one iteration with
+	 * gradient_step 1 is that step up to rounding. */
+	int32_t gradient_iterations;
+	float gradient_step;
 } umans_policy;
 
 /* World description. world_type 0 = WorldInfinite (worldInfinite.cpp), 1 = WorldToric

@@ -31,7 +31,8 @@ class Policy(C.Structure):
                 ("s_random_samples", C.c_int32), ("s_include_base", C.c_int32),
                 ("relaxation_time", C.c_float), ("contact_force_scale", C.c_float),
                 ("n_cost", C.c_int32), ("cost_kind", C.c_int32 * 4), ("cost_coeff", C.c_float * 4),
-                ("cost_range", C.c_float * 4), ("cost_params", (C.c_float * 8) * 4)]
+                ("cost_range", C.c_float * 4), ("cost_params", (C.c_float * 8) * 4),
+                ("gradient_iterations", C.c_int32), ("gradient_step", C.c_float)]
 
 
 _F = C.POINTER(C.c_float)
@@ -112,6 +113,8 @@ def make_policies(packed):
             setattr(q, k, int(p[k]))
         for k in ("s_angle", "relaxation_time", "contact_force_scale"):
             setattr(q, k, float(p[k]))
+        q.gradient_iterations = int(p.get("gradient_iterations", 1))
+        q.gradient_step = float(p.get("gradient_step", 1.0))
         for c in range(p["n_cost"]):
             q.cost_kind[c] = int(p["cost_kind"][c])
             q.cost_coeff[c] = float(p["cost_coeff"][c])

@@ -1025,6 +1025,18 @@ int step_impl(const oracle_world& w, const oracle_policy* pols, int npol, int n,
 				V2<T> tg;
 				for (int c = 0; c < pol.n_cost; ++c) tg = tg + (T)pol.cost_coeff[c] * ce.gradient(c, a.vel);
 				acc = T(-1) * tg;
+				if (pol.gradient_iterations >= 2) {
+					// descent extension (no counterpart in the reference): each iteration's
+					// gradient at the current iterate, the first one being tg
+					const T eta = (T)pol.gradient_step * dt;
+					V2<T> v = a.vel - tg * eta;
+					for (int it = 1; it < pol.gradient_iterations; ++it) {
+						V2<T> g;
+						for (int c = 0; c < pol.n_cost; ++c) g = g + (T)pol.cost_coeff[c] * ce.gradient(c, v);
+						v = v - g * eta;
+					}
+					acc = (v - a.vel) / dt;
+				}
 			} else {
 				V2<T> best;
 				std::vector<std::pair<int, T>> fns;

@@ -57,6 +57,10 @@ typedef struct {
 	float cost_coeff[4];
 	float cost_range[4];
 	float cost_params[4][8];
+	int32_t gradient_iterations;  /* >= 2: the gradient method's descent extension (not in the
+	                                 reference): v <- v - (gradient_step dt) sum coeff grad C(v),
+	                                 acceleration (v_N - v) / dt; 0 or 1: policy.cpp:78-89 */
+	float gradient_step;
 } oracle_policy;
 
 /* Per-agent arrays (length n; 2-vectors interleaved x,y). */

@@ -157,3 +157,17 @@ def test_packed_policies_round_trip_through_reference_xml(tmp_path):
                                        rtol=1e-7, err_msg=k)
         np.testing.assert_allclose(np.asarray(got["cost_params"][:n], np.float32),
                                    np.asarray(want["cost_params"][:n], np.float32), rtol=1e-7)
+
+
+def test_gradient_descent_attributes(tmp_path):
+    """GradientIterations / GradientStepSize (the gradient method's descent extension)
+    are read from <Policy>; without them a policy keeps the reference's single step."""
+    path = _write(tmp_path, '<Simulation delta_time="0.05"><World/><Policies>'
+                  '<Policy id="0" OptimizationMethod="gradient" GradientIterations="20" GradientStepSize="0.1">'
+                  '<costfunction name="SocialForcesAvoidance"/><costfunction name="GoalReachingForce"/></Policy>'
+                  '<Policy id="1" OptimizationMethod="gradient"><costfunction name="GoalReachingForce"/></Policy>'
+                  '</Policies><Agents><Agent><pos x="0" y="0"/><Policy id="0"/></Agent></Agents></Simulation>')
+    sc, _ = gpu.load_scenario(path)
+    assert sc.policies[0]["gradient_iterations"] == 20
+    assert sc.policies[0]["gradient_step"] == pytest.approx(0.1)
+    assert sc.policies[1]["gradient_iterations"] == 1 and sc.policies[1]["gradient_step"] == 1.0

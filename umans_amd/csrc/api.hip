@@ -443,8 +443,19 @@ int make_dev_policy(const umans_policy& p, DevPolicy& d, std::string& err) {
 	d.contact_scale = p.contact_force_scale;
 
 	umans_policy s = p;   // effective sampling parameters
+	d.grad_iters = 1;
+	d.grad_step = 1.0f;
 	if (p.method == UMANS_METHOD_GRADIENT) {
 		d.mode = MODE_GRADIENT;
+		if (p.gradient_iterations >= 2) {
+			if (p.gradient_iterations > kMaxGradientIterations || !std::isfinite(p.gradient_step)) {
+				err = "gradient descent needs 2.." + std::to_string(kMaxGradientIterations) +
+				      " iterations and a finite step";
+				return UMANS_EINVAL;
+			}
+			d.grad_iters = p.gradient_iterations;
+			d.grad_step = p.gradient_step;
+		}
 	} else if (p.method == UMANS_METHOD_SAMPLING) {
 		d.mode = MODE_SAMPLE;
 	} else {
@@ -1328,7 +1339,7 @@ int agents_per_world(umans_ctx* c, std::vector<int>& per_world) {
 
 // ------------------------------------------------------------------ adjoint
 bool adjoint_policy_ok(const DevPolicy& p) {
-	if (p.mode != MODE_GRADIENT) return false;
+	if (p.mode != MODE_GRADIENT || p.grad_iters != 1) return false;
 	for (int c = 0; c < p.n_cost; ++c)
 		if (p.kind[c] != KIND_GOAL_REACHING && p.kind[c] != KIND_POWER_LAW && p.kind[c] != KIND_SOCIAL_FORCES) return false;
 	return true;

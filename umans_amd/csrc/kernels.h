@@ -1304,6 +1304,18 @@ UD void finish_agent(const StepArgs& A, int k, int4 meta, int pi, int wix, const
 					V2 tg{0.0f, 0.0f};
 					for_costs<CLS>(P, [&](int c) { tg = tg + P.coeff[c] * gradient_of<KMAX, CLS>(P, c, a, L, pr, a.vel); });
 					acc = -1.0f * tg;
+					if (P.grad_iters > 1) {
+						// the descent extension (umans_policy::gradient_iterations): the
+						// first iteration's gradient is the one just summed
+						const float eta = P.grad_step * a.dt;
+						V2 v = a.vel - tg * eta;
+						for (int it = 1; it < P.grad_iters; ++it) {
+							V2 g{0.0f, 0.0f};
+							for_costs<CLS>(P, [&](int c) { g = g + P.coeff[c] * gradient_of<KMAX, CLS>(P, c, a, L, pr, v); });
+							v = v - g * eta;
+						}
+						acc = (v - a.vel) / a.dt;
+					}
 				}
 			} else {
 				V2 best;

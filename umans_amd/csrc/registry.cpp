@@ -108,6 +108,8 @@ extern "C" int umans_policy_init(umans_policy* p, int method) {
 	p->relaxation_time = 0.0f;
 	p->contact_force_scale = 5000.0f;
 	p->n_cost = 0;
+	p->gradient_iterations = 1;
+	p->gradient_step = 1.0f;
 	return UMANS_OK;
 }
 

@@ -264,6 +264,10 @@ bool load_policy(umans_scenario& sc, const Node* e) {
 	}
 	qfloat(e, "RelaxationTime", p.relaxation_time);
 	qfloat(e, "ContactForceScale", p.contact_force_scale);
+	// the gradient method's descent extension (umans_policy::gradient_iterations); the
+	// reference's loader ignores these attributes
+	qint(e, "GradientIterations", p.gradient_iterations);
+	qfloat(e, "GradientStepSize", p.gradient_step);
 	// cost functions: the first <costfunction> child, then every following sibling
 	// element (crowdSimulator.cpp:394-403 walks NextSiblingElement() without a name)
 	bool started = false;

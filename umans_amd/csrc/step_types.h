@@ -29,6 +29,7 @@ constexpr unsigned kIndexMask = (1u << kSegShift) - 1u;
 constexpr int kStatCopies = 64;
 constexpr int kStatLine = 16;
 constexpr int kMaxAngleSamples = 64;
+constexpr int kMaxGradientIterations = 1000;   // the gradient method's descent extension
 
 // A policy as the kernels read it. The sampling fields are the effective ones (for a
 // global policy that falls back to sampling: SamplingParameters::
@@ -45,6 +46,8 @@ struct DevPolicy {
 	int orca_cost;           // index of the ORCA cost or -1
 	int cls;                 // kernel class (CLS_* in kernels.h)
 	double view_cos;         // SocialForces view test: (float)acos(c) < 100 deg  <=>  c > view_cos
+	int grad_iters;          // gradient method: 1 = the reference's single step, >= 2 descent
+	float grad_step;         // descent step in frames (umans_policy::gradient_step)
 	int kind[4];
 	float coeff[4];          // effective coefficients (1 for a single-cost global fallback)
 	float range[4];

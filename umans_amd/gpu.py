@@ -50,7 +50,8 @@ class PolicyC(C.Structure):
                 ("s_random_samples", C.c_int32), ("s_include_base", C.c_int32),
                 ("relaxation_time", C.c_float), ("contact_force_scale", C.c_float),
                 ("n_cost", C.c_int32), ("cost_kind", C.c_int32 * 4), ("cost_coeff", C.c_float * 4),
-                ("cost_range", C.c_float * 4), ("cost_params", (C.c_float * 8) * 4)]
+                ("cost_range", C.c_float * 4), ("cost_params", (C.c_float * 8) * 4),
+                ("gradient_iterations", C.c_int32), ("gradient_step", C.c_float)]
 
 
 class AgentsC(C.Structure):
@@ -177,6 +178,8 @@ def policy_struct(packed):
         setattr(q, k, int(packed[k]))
     for k in ("s_angle", "relaxation_time", "contact_force_scale"):
         setattr(q, k, float(packed[k]))
+    q.gradient_iterations = int(packed.get("gradient_iterations", 1))
+    q.gradient_step = float(packed.get("gradient_step", 1.0))
     for c in range(4):
         q.cost_kind[c] = int(packed["cost_kind"][c])
         q.cost_coeff[c] = float(packed["cost_coeff"][c])
@@ -494,7 +497,8 @@ def packed_from_struct(q):
     """Packed-policy dict (umans_amd.registry layout) from a umans_policy struct."""
     d = {k: getattr(q, k) for k in ("method", "s_type", "s_base", "s_base_dir", "s_radius", "s_angle",
                                     "s_speed_samples", "s_angle_samples", "s_random_samples", "s_include_base",
-                                    "relaxation_time", "contact_force_scale", "n_cost")}
+                                    "relaxation_time", "contact_force_scale", "n_cost",
+                                    "gradient_iterations", "gradient_step")}
     d["cost_kind"] = list(q.cost_kind)
     d["cost_coeff"] = list(q.cost_coeff)
     d["cost_range"] = list(q.cost_range)

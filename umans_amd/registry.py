@@ -46,7 +46,7 @@ def cost_defaults(name):
 
 
 def pack_policy(method="gradient", costs=(), sampling=None, relaxation_time=0.0,
-                contact_force_scale=5000.0):
+                contact_force_scale=5000.0, gradient_iterations=1, gradient_step=1.0):
     """Build a packed policy (dict with the umans_policy fields).
 
     costs: sequence of (name, coeff, {param: value}) — `range` may appear among the params.
@@ -54,6 +54,9 @@ def pack_policy(method="gradient", costs=(), sampling=None, relaxation_time=0.0,
       (SamplingType, SamplingBase, SamplingBaseDirection, SamplingRadius, SamplingAngle,
        SpeedSamples, AngleSamples, RandomSamples, IncludeBaseAsSample); defaults from
       policy.h:63-71. Defaults of RelaxationTime 0 and ContactForceScale 5000: policy.h:121-125.
+    gradient_iterations, gradient_step: the gradient method's descent extension (not in
+      the reference; 1 keeps its single step): v <- v - gradient_step * dt * sum coeff grad C(v),
+      gradient_iterations times, then acceleration (v_N - v) / dt.
     """
     s = dict(sampling or {})
     p = {
@@ -69,6 +72,7 @@ def pack_policy(method="gradient", costs=(), sampling=None, relaxation_time=0.0,
         "s_include_base": int(str(s.get("IncludeBaseAsSample", "false")).lower() in ("1", "true")),
         "relaxation_time": float(relaxation_time),
         "contact_force_scale": float(contact_force_scale),
+        "gradient_iterations": int(gradient_iterations), "gradient_step": float(gradient_step),
         "n_cost": 0, "cost_kind": [], "cost_coeff": [], "cost_range": [], "cost_params": [],
     }
     if len(costs) > 4:
