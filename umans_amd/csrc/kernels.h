@@ -542,14 +542,15 @@ UD unsigned key_entry(unsigned long long k) {   // sorted index | segment << kSe
 
 // Long lists, step 1 (one lane per agent): the same queries and exact test as
 // find_neighbors, but every candidate in range is appended, unsorted, to the lane's row of
-// the collection buffer (packed keys, below), at most kLongCap of them.
+// the collection buffer (packed keys, below; a row is kLongCap contiguous keys, so a lane
+// fills whole lines of its own and a row is read back in one coalesced access), at most
+// kLongCap of them.
 // Returns the number in range (all of them, as find_neighbors' total).
 UD int collect_neighbors(const StepArgs& A, const WorldInfo& wi, const GridInfo& g, int self, V2 p, float range,
                          int row) {
 	const State& S = A.in;
 	const double r2 = (double)(range * range);
 	const float rs = range * 1.0001f + 1e-3f;
-	const size_t st = (size_t)A.long_stride;
 	int n = 0;
 	for (int s = 0; s < 5; ++s) {
 		float dx = 0.0f, dy = 0.0f;
@@ -574,7 +575,7 @@ UD int collect_neighbors(const StepArgs& A, const WorldInfo& wi, const GridInfo&
 				const double ddy = qy - (double)pj.y;
 				const double d2 = ddx * ddx + ddy * ddy;
 				if (!(d2 < r2) || (int)j == self) continue;
-				if (n < kLongCap) A.long_key[(size_t)n * st + row] = long_key(s, d2, j);
+				if (n < kLongCap) A.long_key[(size_t)row * kLongCap + n] = long_key(s, d2, j);
 				++n;
 			}
 		}
@@ -585,8 +586,8 @@ UD int collect_neighbors(const StepArgs& A, const WorldInfo& wi, const GridInfo&
 // Long lists, step 2 (the whole wave, in uniform control flow): the wave sorts its rows
 // with bitonic networks across lanes, in the canonical order (segment, d2, agent ID),
 // and writes each row's first kcap entries into long_u in that order. Rows of up to 16,
-// 32 and 64 entries are sorted four, two and one at a time (one entry per lane); rows of
-// up to 128 one at a time with two entries per lane. The networks sort the packed keys;
+// 32, 64 and 128 entries are sorted eight, four, two and one at a time, two entries per
+// lane (rank_rows). The networks sort the packed keys;
 // a pass in which two neighbouring entries of a row share segment and float(d2) sorts
 // its rows again by (segment, exact d2, agent ID), the doubles recomputed from the
 // positions as the search computed them. n_row: the row's entry count (0 for rows the
@@ -797,82 +798,70 @@ template <int J>
 UD unsigned long long xor_lane_u64(unsigned long long v) {
 	return ((unsigned long long)xor_lane<J>((unsigned)(v >> 32)) << 32) | xor_lane<J>((unsigned)v);
 }
-// Bitonic sort, ascending, of one key per lane over blocks of S lanes, with every stride
-// known at compile time: stage (W, J) of the merge of blocks of W.
-template <int S, int W, int J>
-UD void bitonic_stage(unsigned long long& k, int lane) {
-	const unsigned long long kp = xor_lane_u64<J>(k);
-	const bool take_min = (((lane & (S - 1)) & W) == 0) == ((lane & J) == 0);
-	const bool mine_lower = k < kp;
-	k = (take_min == mine_lower) ? k : kp;
-	if constexpr (J > 1) bitonic_stage<S, W, J / 2>(k, lane);
-	else if constexpr (W < S) bitonic_stage<S, 2 * W, W>(k, lane);
-}
-template <int S>
-UD void bitonic_keys_fixed(unsigned long long& k) {
-	bitonic_stage<S, 2, 1>(k, threadIdx.x & 63);
-}
-// The same network on full entries (exact d2, index | segment, ID; canonical order).
-template <int S, int W, int J>
-UD void bitonic_exact_stage(double& d, unsigned& u, int& id, int lane) {
-	const double dp = __longlong_as_double((long long)xor_lane_u64<J>((unsigned long long)__double_as_longlong(d)));
-	const unsigned up = xor_lane<J>(u);
-	const int ip = (int)xor_lane<J>((unsigned)id);
-	const bool take_min = (((lane & (S - 1)) & W) == 0) == ((lane & J) == 0);
-	const bool take = take_min ? key_less(dp, up, ip, d, u, id) : key_less(d, u, id, dp, up, ip);
-	d = take ? dp : d;
-	u = take ? up : u;
-	id = take ? ip : id;
-	if constexpr (J > 1) bitonic_exact_stage<S, W, J / 2>(d, u, id, lane);
-	else if constexpr (W < S) bitonic_exact_stage<S, 2 * W, W>(d, u, id, lane);
-}
-template <int S>
-UD void bitonic_exact_fixed(double& d, unsigned& u, int& id) {
-	bitonic_exact_stage<S, 2, 1>(d, u, id, threadIdx.x & 63);
-}
-// 128 entries, two per lane (entry lane and lane + 64): the stride-64 exchanges stay in
-// the lane.
-template <int W, int J>
-UD void bitonic128_stage(unsigned long long (&k)[2], int lane) {
-	if constexpr (J == 64) {
-		const unsigned long long lo = k[0] < k[1] ? k[0] : k[1];
-		const unsigned long long hi = k[0] < k[1] ? k[1] : k[0];
-		k[0] = lo;
-		k[1] = hi;
+// Bitonic sort, ascending, of rows of SZ entries held E per lane: entry x of a row is
+// element q of lane l of the row's group of L = SZ / E lanes, x = q * L + l. Stage
+// (W, J) of the merge of blocks of W compares x with x ^ J: inside the lane (element
+// q ^ J / L) for J >= L, with lane l ^ J otherwise. Keys: the packed 64-bit keys.
+template <int SZ, int E, int W, int J>
+UD void rank_stage(unsigned long long (&k)[E], int l) {
+	constexpr int L = SZ / E;
+	if constexpr (J >= L) {
+		constexpr int D = J / L;
+#pragma unroll
+		for (int q = 0; q < E; ++q) {
+			if (q & D) continue;
+			const bool asc = ((q * L + l) & W) == 0;
+			const unsigned long long lo = k[q] < k[q + D] ? k[q] : k[q + D];
+			const unsigned long long hi = k[q] < k[q + D] ? k[q + D] : k[q];
+			k[q] = asc ? lo : hi;
+			k[q + D] = asc ? hi : lo;
+		}
 	} else {
 #pragma unroll
-		for (int q = 0; q < 2; ++q) {
+		for (int q = 0; q < E; ++q) {
 			const unsigned long long kp = xor_lane_u64<J>(k[q]);
-			const bool take_min = (((lane + 64 * q) & W) == 0) == ((lane & J) == 0);
+			const int x = q * L + l;
+			const bool take_min = ((x & W) == 0) == ((x & J) == 0);
 			k[q] = (take_min == (k[q] < kp)) ? k[q] : kp;
 		}
 	}
-	if constexpr (J > 1) bitonic128_stage<W, J / 2>(k, lane);
-	else if constexpr (W < 128) bitonic128_stage<2 * W, W>(k, lane);
+	if constexpr (J > 1) rank_stage<SZ, E, W, J / 2>(k, l);
+	else if constexpr (W < SZ) rank_stage<SZ, E, 2 * W, W>(k, l);
 }
-template <int W, int J>
-UD void bitonic128_exact_stage(double (&d)[2], unsigned (&u)[2], int (&id)[2], int lane) {
-	if constexpr (J == 64) {
-		if (key_less(d[1], u[1], id[1], d[0], u[0], id[0])) {
-			const double td = d[0]; d[0] = d[1]; d[1] = td;
-			const unsigned tu = u[0]; u[0] = u[1]; u[1] = tu;
-			const int ti = id[0]; id[0] = id[1]; id[1] = ti;
+// The same network on full entries (exact d2, index | segment, ID): the canonical order.
+template <int SZ, int E, int W, int J>
+UD void rank_exact_stage(double (&d)[E], unsigned (&u)[E], int (&id)[E], int l) {
+	constexpr int L = SZ / E;
+	if constexpr (J >= L) {
+		constexpr int D = J / L;
+#pragma unroll
+		for (int q = 0; q < E; ++q) {
+			if (q & D) continue;
+			const bool asc = ((q * L + l) & W) == 0;
+			const bool sw = asc ? key_less(d[q + D], u[q + D], id[q + D], d[q], u[q], id[q])
+			                    : key_less(d[q], u[q], id[q], d[q + D], u[q + D], id[q + D]);
+			if (sw) {
+				const double td = d[q]; d[q] = d[q + D]; d[q + D] = td;
+				const unsigned tu = u[q]; u[q] = u[q + D]; u[q + D] = tu;
+				const int ti = id[q]; id[q] = id[q + D]; id[q + D] = ti;
+			}
 		}
 	} else {
 #pragma unroll
-		for (int q = 0; q < 2; ++q) {
+		for (int q = 0; q < E; ++q) {
 			const double dp = __longlong_as_double((long long)xor_lane_u64<J>((unsigned long long)__double_as_longlong(d[q])));
 			const unsigned up = xor_lane<J>(u[q]);
 			const int ip = (int)xor_lane<J>((unsigned)id[q]);
-			const bool take_min = (((lane + 64 * q) & W) == 0) == ((lane & J) == 0);
+			const int x = q * L + l;
+			const bool take_min = ((x & W) == 0) == ((x & J) == 0);
 			const bool take = take_min ? key_less(dp, up, ip, d[q], u[q], id[q]) : key_less(d[q], u[q], id[q], dp, up, ip);
 			d[q] = take ? dp : d[q];
 			u[q] = take ? up : u[q];
 			id[q] = take ? ip : id[q];
 		}
 	}
-	if constexpr (J > 1) bitonic128_exact_stage<W, J / 2>(d, u, id, lane);
-	else if constexpr (W < 128) bitonic128_exact_stage<2 * W, W>(d, u, id, lane);
+	if constexpr (J > 1) rank_exact_stage<SZ, E, W, J / 2>(d, u, id, l);
+	else if constexpr (W < SZ) rank_exact_stage<SZ, E, 2 * W, W>(d, u, id, l);
 }
 
 // Full entry of key k of the row whose agent stands at (px, py) in a world of width W
@@ -893,108 +882,108 @@ UD void exact_entry(const State& S, float px, float py, float W, float H, unsign
 	d = ddx * ddx + ddy * ddy;
 	id = S.meta[u & kIndexMask].x;
 }
-// The rows of more than 2^(LG-1) (LG = 4: any) and at most 2^LG entries, 64 >> LG rows
-// per pass, one entry per lane. The next pass's keys are loaded before this pass sorts,
-// so their latency is spent sorting.
-template <int LG>
+
+// The rows of more than 2^(LG-1) (LG = 4: any) and at most 2^LG entries, E entries per
+// lane, so 64 E >> LG rows per pass: a wave ranks its 64 rows in few passes, and a pass's
+// dependent chain of cross-lane stages serves several rows. The keys of the next
+// kRankAhead passes are loaded before a pass sorts, so their latency is spent sorting.
+constexpr int kRankAhead = 2;
+template <int LG, int E>
 UD void rank_rows(const StepArgs& A, int row, int n_row, int kcap, V2 p, const WorldInfo& wi, int lane) {
-	constexpr int SZ = 1 << LG, G = 64 >> LG;
+	constexpr int SZ = 1 << LG, L = SZ / E, G = 64 / L;
+	static_assert(L >= 1 && L <= 64 && G * L == 64, "rows of SZ entries, E per lane");
 	const size_t st = (size_t)A.long_stride;
 	const State& S = A.in;
 	unsigned long long m = __ballot(n_row > (LG == 4 ? 0 : SZ / 2) && n_row <= SZ);
 	if (!m) return;
-	const int i = lane & (SZ - 1);
-	// a pass's rows: the next G set bits of m (uniform), lane group g = lane >> LG
-	auto select = [&](int& src, int& n, int& rw) {
+	const int l = lane & (L - 1);
+	// a pass's rows: the next G set bits of m (uniform), lane group g = lane / L; its
+	// keys: entries q L + l of the lane group's row (kKeyPad where there is none)
+	auto select = [&](int& src, int& n, int& rw, unsigned long long (&key)[E]) {
 		int rsel = -1;
 		for (int g = 0; g < G; ++g) {
 			const int r = m ? __builtin_ctzll(m) : -1;
 			if (m) m &= m - 1;
-			if ((lane >> LG) == g) rsel = r;
+			if (lane / L == g) rsel = r;
 		}
 		src = rsel < 0 ? 0 : rsel;          // every lane shuffles
 		const int n_src = __shfl(n_row, src);
 		rw = __shfl(row, src);
 		n = rsel >= 0 ? n_src : 0;
-	};
-	int src, n, rw;
-	select(src, n, rw);
-	unsigned long long kn = i < n ? A.long_key[(size_t)i * st + rw] : kKeyPad;
-	for (;;) {
-		const int csrc = src, cn = n, crw = rw;
-		unsigned long long k = kn;
-		const bool more = m != 0ull;
-		if (more) {
-			select(src, n, rw);
-			kn = i < n ? A.long_key[(size_t)i * st + rw] : kKeyPad;
+#pragma unroll
+		for (int q = 0; q < E; ++q) {
+			const int x = q * L + l;
+			key[q] = x < n ? A.long_key[(size_t)rw * kLongCap + x] : kKeyPad;
 		}
-		bitonic_keys_fixed<SZ>(k);
-		// ties sit next to each other once sorted
-		const unsigned long long nx = shfl_u64(k, (lane + 1) & 63);
+	};
+	// the passes in flight: slot 0 is sorted next (uniform flags)
+	int qsrc[kRankAhead], qn[kRankAhead], qrw[kRankAhead];
+	unsigned long long qk[kRankAhead][E];
+	bool qhave[kRankAhead];
+#pragma unroll
+	for (int d = 0; d < kRankAhead; ++d) {
+		qhave[d] = m != 0ull;
+		qsrc[d] = 0; qn[d] = 0; qrw[d] = 0;
+#pragma unroll
+		for (int q = 0; q < E; ++q) qk[d][q] = kKeyPad;
+		if (qhave[d]) select(qsrc[d], qn[d], qrw[d], qk[d]);
+	}
+	while (qhave[0]) {
+		const int csrc = qsrc[0], cn = qn[0], crw = qrw[0];
+		unsigned long long k[E];
+#pragma unroll
+		for (int q = 0; q < E; ++q) k[q] = qk[0][q];
+#pragma unroll
+		for (int d = 0; d + 1 < kRankAhead; ++d) {
+			qsrc[d] = qsrc[d + 1]; qn[d] = qn[d + 1]; qrw[d] = qrw[d + 1]; qhave[d] = qhave[d + 1];
+#pragma unroll
+			for (int q = 0; q < E; ++q) qk[d][q] = qk[d + 1][q];
+		}
+		qhave[kRankAhead - 1] = m != 0ull;
+		if (qhave[kRankAhead - 1])
+			select(qsrc[kRankAhead - 1], qn[kRankAhead - 1], qrw[kRankAhead - 1], qk[kRankAhead - 1]);
+		rank_stage<SZ, E, 2, 1>(k, l);
+		// ties sit next to each other once sorted: the entry after x = q L + l is element q
+		// of lane l + 1, or for l = L - 1 element q + 1 of the group's first lane
 		const int keep = cn < kcap ? cn : kcap;
-		if (__ballot(i + 1 < SZ && same_float_key(k, nx))) {
+		bool tie = false;
+#pragma unroll
+		for (int q = 0; q < E; ++q) {
+			const unsigned long long a = shfl_u64(k[q], (lane + 1) & 63);
+			const unsigned long long b = q + 1 < E ? shfl_u64(k[q + 1 < E ? q + 1 : q], lane - l) : kKeyPad;
+			const unsigned long long nx = l < L - 1 ? a : b;
+			tie = tie || (q * L + l + 1 < SZ && same_float_key(k[q], nx));
+		}
+		if (__ballot(tie)) {
 			const float px = __shfl(p.x, csrc), py = __shfl(p.y, csrc);
 			const float W = __shfl(wi.W, csrc), H = __shfl(wi.H, csrc);
-			double d;
-			unsigned u;
-			int id;
-			exact_entry(S, px, py, W, H, k, d, u, id);
-			bitonic_exact_fixed<SZ>(d, u, id);
-			if (i < keep) A.long_u[(size_t)i * st + crw] = u;
-		} else if (i < keep) {
-			A.long_u[(size_t)i * st + crw] = key_entry(k);
+			double d[E];
+			unsigned u[E];
+			int id[E];
+#pragma unroll
+			for (int q = 0; q < E; ++q) exact_entry(S, px, py, W, H, k[q], d[q], u[q], id[q]);
+			rank_exact_stage<SZ, E, 2, 1>(d, u, id, l);
+#pragma unroll
+			for (int q = 0; q < E; ++q)
+				if (q * L + l < keep) A.long_u[(size_t)(q * L + l) * st + crw] = u[q];
+		} else {
+#pragma unroll
+			for (int q = 0; q < E; ++q)
+				if (q * L + l < keep) A.long_u[(size_t)(q * L + l) * st + crw] = key_entry(k[q]);
 		}
-		if (!more) break;
 	}
 }
 
 UD void rank_long_lists(const StepArgs& A, int row, int n_row, int kcap, V2 p, const WorldInfo& wi) {
 	const int lane = threadIdx.x & 63;
-	const size_t st = (size_t)A.long_stride;
-	const State& S = A.in;
 	// the rows were written by their own lanes: make those stores visible to the wave
 	// (one wave per workgroup, so workgroup scope is the wave)
 	__builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup");
-	// rows of up to 16, 32 and 64 entries: 4, 2 and 1 per pass, one entry per lane
-	rank_rows<4>(A, row, n_row, kcap, p, wi, lane);
-	rank_rows<5>(A, row, n_row, kcap, p, wi, lane);
-	rank_rows<6>(A, row, n_row, kcap, p, wi, lane);
-	// rows of up to 128 entries: one per pass, two entries per lane
-	unsigned long long m = __ballot(n_row > 64);
-	while (m) {
-		const int r = __builtin_ctzll(m);
-		m &= m - 1;
-		const int n = __shfl(n_row, r), rw = __shfl(row, r);
-		unsigned long long k[2];
-#pragma unroll
-		for (int q = 0; q < 2; ++q) {
-			const int e = lane + 64 * q;
-			k[q] = e < n ? A.long_key[(size_t)e * st + rw] : kKeyPad;
-		}
-		bitonic128_stage<2, 1>(k, lane);
-		// sorted position of k[q] is lane + 64 q: the next entry of k[0] on lane 63 is k[1]
-		// of lane 0
-		const unsigned long long n0 = shfl_u64(k[0], (lane + 1) & 63);
-		const unsigned long long n1 = shfl_u64(k[1], (lane + 1) & 63);
-		const unsigned long long f1 = shfl_u64(k[1], 0);
-		const bool tie = same_float_key(k[0], lane < 63 ? n0 : f1) || (lane < 63 && same_float_key(k[1], n1));
-		const int keep = n < kcap ? n : kcap;
-		if (__ballot(tie)) {
-			const float px = __shfl(p.x, r), py = __shfl(p.y, r);
-			const float W = __shfl(wi.W, r), H = __shfl(wi.H, r);
-			double d[2];
-			unsigned u[2];
-			int id[2];
-			exact_entry(S, px, py, W, H, k[0], d[0], u[0], id[0]);
-			exact_entry(S, px, py, W, H, k[1], d[1], u[1], id[1]);
-			bitonic128_exact_stage<2, 1>(d, u, id, lane);
-			if (lane < keep) A.long_u[(size_t)lane * st + rw] = u[0];
-			if (lane + 64 < keep) A.long_u[(size_t)(lane + 64) * st + rw] = u[1];
-		} else {
-			if (lane < keep) A.long_u[(size_t)lane * st + rw] = key_entry(k[0]);
-			if (lane + 64 < keep) A.long_u[(size_t)(lane + 64) * st + rw] = key_entry(k[1]);
-		}
-	}
+	// rows of up to 16, 32, 64 and 128 entries, two entries per lane: 8, 4, 2 and 1 per pass
+	rank_rows<4, 2>(A, row, n_row, kcap, p, wi, lane);
+	rank_rows<5, 2>(A, row, n_row, kcap, p, wi, lane);
+	rank_rows<6, 2>(A, row, n_row, kcap, p, wi, lane);
+	rank_rows<7, 2>(A, row, n_row, kcap, p, wi, lane);
 	// the sorted rows are read back by their own lanes
 	__builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup");
 }

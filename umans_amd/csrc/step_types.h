@@ -146,9 +146,10 @@ struct StepArgs {
 	int next_key_base;
 	// long lists (the step kernel instance with KMAX = 0): lane t collects its in-range
 	// candidates unsorted into row base + t of long_key (entry e at
-	// [e * long_stride + row], packed as segment | float(d2) | index), at most kLongCap
+	// [row * kLongCap + e], packed as segment | float(d2) | index), at most kLongCap
 	// of them; the wave then sorts each row and writes its entries, in order, into
-	// long_u (index | segment, same layout)
+	// long_u (index | segment; entry e at [e * long_stride + row], so a wave's lanes
+	// read their e-th entries together)
 	unsigned long long* long_key;
 	unsigned* long_u;
 	int long_stride;
