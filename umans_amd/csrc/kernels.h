@@ -661,32 +661,32 @@ UD void sort_net(unsigned long long* k) {
 // A full buffer drops its canonical last entry: KC - 1 >= kcap entries order before it,
 // so it is in no list, nor is any later candidate whose (segment, float) is above its
 // own. Returns the dropped key; the caller's count goes down by one.
-template <int KC>
+template <int KC, int CS = 64>
 UD unsigned long long drop_last(const State& S, V2 p, const WorldInfo& wi, unsigned long long* buf) {
 	int mi = 0;
 	unsigned long long mk = buf[0];
 #pragma unroll
 	for (int i = 1; i < KC; ++i) {
-		const unsigned long long v = buf[i * 64];
+		const unsigned long long v = buf[i * CS];
 		// (segment, float) decides; equal ones by exact d2 and ID (rare)
 		bool later = (v >> 28) > (mk >> 28);
 		if (same_float_key(v, mk)) later = entry_before(S, p, wi, key_entry(mk), key_entry(v));
 		mi = later ? i : mi;
 		mk = later ? v : mk;
 	}
-	buf[mi * 64] = buf[(KC - 1) * 64];
+	buf[mi * CS] = buf[(KC - 1) * CS];
 	return mk;
 }
 
-// Sorts the lane's cnt (<= KC) buffered keys (buf[i * 64]) into k[0..KC), pads after
+// Sorts the lane's cnt (<= KC) buffered keys (buf[i * CS]) into k[0..KC), pads after
 // them, puts runs of equal segment and float that reach into the kept entries in exact
 // d2 and ID order, and returns min(cnt, kcap).
-template <int KC>
+template <int KC, int CS = 64>
 UD int sort_short(const State& S, V2 p, const WorldInfo& wi, unsigned long long* buf, int cnt, int kcap,
                   unsigned long long* k) {
 #pragma unroll
 	for (int i = 0; i < KC; ++i) {
-		const unsigned long long v = buf[i * 64];
+		const unsigned long long v = buf[i * CS];
 		k[i] = i < cnt ? v : kKeyPad;
 	}
 	sort_net<KC>(k);
@@ -697,24 +697,24 @@ UD int sort_short(const State& S, V2 p, const WorldInfo& wi, unsigned long long*
 		if (tie) {
 #pragma unroll
 			for (int i = 0; i < KC; ++i)
-				if (i < cnt) buf[i * 64] = k[i];
+				if (i < cnt) buf[i * CS] = k[i];
 			for (int i = 0; i + 1 < cnt;) {
 				int r = i + 1;
-				while (r < cnt && same_float_key(buf[i * 64], buf[r * 64])) ++r;
+				while (r < cnt && same_float_key(buf[i * CS], buf[r * CS])) ++r;
 				for (int x = i + 1; x < r; ++x) {
-					const unsigned long long v = buf[x * 64];
+					const unsigned long long v = buf[x * CS];
 					int y = x;
-					while (y > i && entry_before(S, p, wi, key_entry(v), key_entry(buf[(y - 1) * 64]))) {
-						buf[y * 64] = buf[(y - 1) * 64];
+					while (y > i && entry_before(S, p, wi, key_entry(v), key_entry(buf[(y - 1) * CS]))) {
+						buf[y * CS] = buf[(y - 1) * CS];
 						--y;
 					}
-					buf[y * 64] = v;
+					buf[y * CS] = v;
 				}
 				i = r;
 			}
 #pragma unroll
 			for (int i = 0; i < KC; ++i) {
-				const unsigned long long v = buf[i * 64];
+				const unsigned long long v = buf[i * CS];
 				k[i] = i < cnt ? v : kKeyPad;
 			}
 		}
@@ -723,10 +723,10 @@ UD int sort_short(const State& S, V2 p, const WorldInfo& wi, unsigned long long*
 }
 
 // find_neighbors for KMAX <= kShortList: the same queries, exact test and canonical list.
-// buf: the lane's column of KMAX + 4 keys (stride 64, a wave's own region); the finished
-// list (index | segment, stride 64) is written over the start of the wave's region, lane
-// by lane as the keys were.
-template <int KMAX>
+// buf: the lane's column of KMAX + 4 keys (stride CS, the columns of the wave's lanes
+// side by side); the finished list (index | segment, stride CS) is written over the
+// start of the wave's region, lane by lane as the keys were.
+template <int KMAX, int CS = 64>
 UD int find_short(const StepArgs& A, const WorldInfo& wi, const GridInfo& g, int self, V2 p, float range,
                   unsigned long long* buf, unsigned* list, int& total) {
 	constexpr int KC = short_buffer<KMAX>();
@@ -764,23 +764,23 @@ UD int find_short(const StepArgs& A, const WorldInfo& wi, const GridInfo& g, int
 				total += hit ? 1 : 0;
 				const unsigned long long key = long_key(s, d2, j);
 				if (hit && (key >> 28) <= (lim >> 28)) {
-					buf[cnt * 64] = key;
+					buf[cnt * CS] = key;
 					++cnt;
 				}
 				if (__ballot(cnt == KC) != 0ull) {
 					if (cnt == KC) {
-						lim = drop_last<KC>(S, p, wi, buf);
+						lim = drop_last<KC, CS>(S, p, wi, buf);
 						--cnt;
 					}
 				}
 			}
 		}
 	}
-	cnt = sort_short<KC>(S, p, wi, buf, cnt, kcap, k);
+	cnt = sort_short<KC, CS>(S, p, wi, buf, cnt, kcap, k);
 	// the list overlays the keys of the wave's lanes: every lane has read its keys
 	__builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
 #pragma unroll
-	for (int i = 0; i < KMAX; ++i) list[i * 64] = key_entry(k[i]);
+	for (int i = 0; i < KMAX; ++i) list[i * CS] = key_entry(k[i]);
 	return cnt;
 }
 // The value of lane ^ J (every lane active): DPP for J = 1, 2 (quad permutes) and 8 (a
@@ -1670,20 +1670,21 @@ UD float coop_cost(const DevPolicy& Q, int q, const Self& b, const NbList& L, co
 // small to fill the SIMDs at 64 (the candidates of the wave's agents still cover all lanes).
 template <int KMAX, int APW, int KIND>
 __global__ __launch_bounds__(64) void k_step_coop(StepArgs A) {
-	constexpr int BS = 64;
-	static_assert(APW == 64 || APW == 32, "agents per wave");
+	static_assert(APW == 64 || APW == 32 || APW == 16, "agents per wave");
 	constexpr bool kShort = KMAX <= kShortList;
 	constexpr int KC = short_buffer<KMAX>();
-	__shared__ float s_d2[kShort ? 1 : KMAX * BS];
-	__shared__ unsigned s_nbl[kShort ? 1 : KMAX * BS];
-	__shared__ unsigned long long s_buf[kShort ? KC * BS : 1];
-	// the lists, lane-interleaved (stride BS = 64): short lists are written over the keys
+	// per-agent arrays hold the wave's APW agents (lanes tid < APW): with 32 agents per
+	// wave half the LDS of 64, so twice the waves fit a CU
+	__shared__ float s_d2[kShort ? 1 : KMAX * APW];
+	__shared__ unsigned s_nbl[kShort ? 1 : KMAX * APW];
+	__shared__ unsigned long long s_buf[kShort ? KC * APW : 1];
+	// the lists, agent-interleaved (stride APW): short lists are written over the keys
 	unsigned* const s_nb = kShort ? reinterpret_cast<unsigned*>(s_buf) : s_nbl;
-	__shared__ float s_self[12][BS];    // pos, vel, pref, radius, pref speed, max speed, max accel, dt, relaxation
-	__shared__ float s_ttcp[4][BS];
-	__shared__ float s_wh[2][BS];
-	__shared__ int s_cnt[BS], s_pi[BS], s_items[BS + 1];
-	__shared__ unsigned long long s_key[BS];
+	__shared__ float s_self[12][APW];    // pos, vel, pref, radius, pref speed, max speed, max accel, dt, relaxation
+	__shared__ float s_ttcp[4][APW];
+	__shared__ float s_wh[2][APW];
+	__shared__ int s_cnt[APW], s_pi[APW], s_items[APW + 1];
+	__shared__ unsigned long long s_key[APW];
 	const int tid = threadIdx.x;
 	const int t = xcd_tile(blockIdx.x, gridDim.x) * APW + tid;
 	const int n_here = A.agent_count ? (int)*A.agent_count : A.n;
@@ -1712,9 +1713,9 @@ __global__ __launch_bounds__(64) void k_step_coop(StepArgs A) {
 		a.dt = wi.dt;
 		a.relaxation = A.pol[pi].relaxation;
 		if constexpr (kShort)
-			cnt = find_short<KMAX>(A, wi, g, k, a.pos, A.pol[pi].query_range, s_buf + tid, s_nb + tid, total);
+			cnt = find_short<KMAX, APW>(A, wi, g, k, a.pos, A.pol[pi].query_range, s_buf + tid, s_nb + tid, total);
 		else
-			cnt = find_neighbors<KMAX, BS>(A, wi, g, k, a.pos, A.pol[pi].query_range, s_d2 + tid, s_nb + tid, total);
+			cnt = find_neighbors<KMAX, APW>(A, wi, g, k, a.pos, A.pol[pi].query_range, s_d2 + tid, s_nb + tid, total);
 	}
 	// with no cap, a list longer than KMAX is redone whole by k_step_huge; the lane still
 	// takes part in the wave's barriers but has no candidates
@@ -1734,14 +1735,15 @@ __global__ __launch_bounds__(64) void k_step_coop(StepArgs A) {
 		A.rec_count[k] = cnt;
 		const int m = cnt < A.rec_stride ? cnt : A.rec_stride;
 		for (int e = 0; e < m; ++e) {
-			const unsigned u = s_nb[e * BS + tid];
+			const unsigned u = s_nb[e * APW + tid];
 			const long long o = (long long)k * A.rec_stride + e;
 			A.rec_id[o] = S.meta[u & kIndexMask].x;
 			A.rec_d2[o] = entry_d2(S, a.pos, wi, u);
 			A.rec_seg[o] = (int)(u >> kSegShift);
 		}
 	}
-	const NbList L{s_nb + tid, BS, cnt, S.pos, S.vel, S.prm, S.pref, wi.W, wi.H, a.pos, nullptr, 0};
+	// (lanes tid >= APW hold no agent and never read their list)
+	const NbList L{s_nb + (tid < APW ? tid : 0), APW, cnt, S.pos, S.vel, S.prm, S.pref, wi.W, wi.H, a.pos, nullptr, 0};
 	// Policy::ComputeNewVelocity (policy.cpp:43-76): agents with a zero preferred
 	// velocity keep a zero new velocity and have no candidates
 	const bool sample = active && !to_huge && !is_zero(a.pref);
@@ -1753,25 +1755,27 @@ __global__ __launch_bounds__(64) void k_step_coop(StepArgs A) {
 	}
 	const float self_v[12] = {a.pos.x, a.pos.y, a.vel.x, a.vel.y, a.pref.x, a.pref.y, a.radius, a.pref_speed,
 	                          a.max_speed, a.max_accel, a.dt, a.relaxation};
+	if (tid < APW) {
 #pragma unroll
-	for (int q = 0; q < 12; ++q) s_self[q][tid] = self_v[q];
+		for (int q = 0; q < 12; ++q) s_self[q][tid] = self_v[q];
 #pragma unroll
-	for (int q = 0; q < 4; ++q) s_ttcp[q][tid] = ttcp[q];
-	s_wh[0][tid] = wi.W;
-	s_wh[1][tid] = wi.H;
-	s_cnt[tid] = cnt;
-	s_pi[tid] = pi;
-	s_key[tid] = ~0ull;
-	// exclusive prefix sum of the candidate counts over the wave
+		for (int q = 0; q < 4; ++q) s_ttcp[q][tid] = ttcp[q];
+		s_wh[0][tid] = wi.W;
+		s_wh[1][tid] = wi.H;
+		s_cnt[tid] = cnt;
+		s_pi[tid] = pi;
+		s_key[tid] = ~0ull;
+	}
+	// exclusive prefix sum of the candidate counts over the wave (lanes tid >= APW have none)
 	int incl = ns;
 	for (int o = 1; o < 64; o <<= 1) {
 		const int y = __shfl_up(incl, o);
 		if (tid >= o) incl += y;
 	}
-	s_items[tid + 1] = incl;
+	if (tid < APW) s_items[tid + 1] = incl;
 	if (tid == 0) s_items[0] = 0;
 	__syncthreads();
-	const int n_items = s_items[BS];
+	const int n_items = s_items[APW];
 
 	int ag = -1;                                   // agent whose state the lane holds
 	unsigned long long local = ~0ull;
@@ -1814,7 +1818,7 @@ __global__ __launch_bounds__(64) void k_step_coop(StepArgs A) {
 		const DevPolicy& Q = A.pol[bpi];
 		const int c = it - s_items[ag];
 		const V2 v = regular_candidate(Q, c, base, bdir, dL);
-		const NbList Lb{s_nb + ag, BS, bcnt, S.pos, S.vel, S.prm, S.pref, bW, bH, b.pos, nullptr, 0};
+		const NbList Lb{s_nb + ag, APW, bcnt, S.pos, S.vel, S.prm, S.pref, bW, bH, b.pos, nullptr, 0};
 		float tc = 0.0f;
 		for (int q = 0; q < Q.n_cost; ++q) tc += Q.coeff[q] * coop_cost<KMAX, KIND>(Q, q, b, Lb, pr, v);
 		const unsigned long long key = cost_key(tc, c);

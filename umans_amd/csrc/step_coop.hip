@@ -9,7 +9,7 @@ namespace {
 
 // Agents per wave of the cooperative kernel: 64 while that makes at least ~4 waves per
 // SIMD (1024 SIMDs), else 32 (twice the waves, half the candidates per lane).
-int coop_agents_per_wave(int n) { return n >= 262144 ? 64 : 32; }
+int coop_agents_per_wave(int n) { return n >= 262144 ? 64 : (n >= 131072 ? 32 : 16); }
 
 template <int KMAX, int KIND>
 void launch_coop_k(const StepArgs& a, hipStream_t st) {
@@ -17,7 +17,8 @@ void launch_coop_k(const StepArgs& a, hipStream_t st) {
 	const int blocks = (a.n + apw - 1) / apw;
 	if (blocks <= 0) return;
 	if (apw == 64) hipLaunchKernelGGL((k_step_coop<KMAX, 64, KIND>), dim3(blocks), dim3(64), 0, st, a);
-	else hipLaunchKernelGGL((k_step_coop<KMAX, 32, KIND>), dim3(blocks), dim3(64), 0, st, a);
+	else if (apw == 32) hipLaunchKernelGGL((k_step_coop<KMAX, 32, KIND>), dim3(blocks), dim3(64), 0, st, a);
+	else hipLaunchKernelGGL((k_step_coop<KMAX, 16, KIND>), dim3(blocks), dim3(64), 0, st, a);
 }
 
 // kind: the one cost of every cooperative policy, or -1; instances of single kinds for
