@@ -24,6 +24,24 @@
 namespace umans {
 
 // ------------------------------------------------------------------ utilities
+// Workgroups go to the 8 XCDs round-robin (workgroup b to XCD b mod 8). Each XCD is
+// given a contiguous run of tiles instead, so the agents its CUs hold are neighbours in
+// the sorted order and its L2 keeps the cells they gather from (the step kernels) and the
+// per-cell counters and starts they update and read (the reorder).
+#ifndef UMANS_XCD_REMAP
+#define UMANS_XCD_REMAP 1
+#endif
+constexpr int kNumXcd = 8;
+UD int xcd_tile(int b, int nb) {
+#if UMANS_XCD_REMAP
+	const int per = nb / kNumXcd, rem = nb % kNumXcd;
+	const int x = b % kNumXcd, i = b / kNumXcd;
+	return x * per + (x < rem ? x : rem) + i;
+#else
+	(void)nb;
+	return b;
+#endif
+}
 #ifndef UMANS_STEP_KERNELS_ONLY
 __global__ void k_fill(unsigned* p, int n, unsigned v) {
 	const int i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -382,7 +400,7 @@ __global__ __launch_bounds__(256) void k_reorder(State src, State dst, int n, co
                                                 int slot_cells, unsigned* cell_first, int* class_list,
                                                 unsigned* scan_state, int n_scan_state, const WorldInfo* worlds,
                                                 int* bbox) {
-	const int k = blockIdx.x * blockDim.x + threadIdx.x;
+	const int k = xcd_tile(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
 	if (k < n_scan_state) scan_state[k] = 0u;
 	if (bbox) bbox_add(src.pos, src.meta, worlds, n, k, bbox);
 	const unsigned sc = (unsigned)slot_cells;
@@ -1160,24 +1178,6 @@ UD V2 sample_best(const DevPolicy& P, const Self& a, const NbList& L, const Prep
 }
 
 // ------------------------------------------------------------------ the step
-// Workgroups go to the 8 XCDs round-robin (workgroup b to XCD b mod 8). Each XCD is
-// given a contiguous run of tiles instead, so the agents its CUs hold are neighbours in
-// the sorted order and its L2 keeps the cells they gather from.
-#ifndef UMANS_XCD_REMAP
-#define UMANS_XCD_REMAP 1
-#endif
-constexpr int kNumXcd = 8;
-UD int xcd_tile(int b, int nb) {
-#if UMANS_XCD_REMAP
-	const int per = nb / kNumXcd, rem = nb % kNumXcd;
-	const int x = b % kNumXcd, i = b / kNumXcd;
-	return x * per + (x < rem ? x : rem) + i;
-#else
-	(void)nb;
-	return b;
-#endif
-}
-
 // The rest of one DoStep for agent k once its neighbour list is known (entry e at
 // lst[e * lstride], cnt of them; obstacle edges obs[0 .. n_obs)): the neighbour record,
 // Policy::ComputeNewVelocity, contact forces, integration, the outputs and the next
