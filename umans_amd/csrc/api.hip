@@ -83,6 +83,11 @@ bool block_supported(int kmax, int bs) {
 }
 int default_block(int kmax) { return kmax <= 32 ? 128 : 64; }
 
+// Agents per wave of the long-list step instance: 64 while that makes at least ~4 waves
+// per SIMD (1024 SIMDs), else 32 or 16 — more waves for small crowds, and fewer rows for
+// each wave to rank. The lists and results do not depend on it.
+int long_agents_per_wave(int n) { return n >= 262144 ? 64 : (n >= 131072 ? 32 : 16); }
+
 void launch_step(int cls, int kmax, int bs, const StepArgs& a, hipStream_t st, int coop_kind = -1) {
 	static_assert(CLS_FORCE_GRAD == 0 && CLS_FORCE_SAMPLE == 1 && CLS_ORCA == 2 && CLS_GENERIC == 3, "class numbers");
 	const bool short_list = kmax == 12 || kmax == 16;
@@ -920,6 +925,7 @@ int enqueue_step(umans_ctx* c, bool events, const State* ysave = nullptr, unsign
 			al.long_key = c->long_key;
 			al.long_u = c->long_u;
 			al.long_stride = c->cap;
+			al.long_apw = long_agents_per_wave(aq.n);
 			al.tier_final = 0;
 			al.ovf_list = c->ovf[0];
 			al.ovf_count = c->ovf_count + 2 * q;

@@ -1379,10 +1379,12 @@ __global__ __launch_bounds__(BS) void k_step(StepArgs A) {
 	// The ORCA half-planes live in private memory: kept in LDS they bound the class to
 	// ~2 waves per SIMD, and the private copy (L1-resident) measured faster at 6.
 	const int tid = threadIdx.x;
-	const int t = xcd_tile(blockIdx.x, gridDim.x) * BS + tid;
+	// long lists: apw agents per wave, lanes tid >= apw hold none
+	const int apw = kLong && A.long_apw > 0 ? A.long_apw : BS;
+	const int t = xcd_tile(blockIdx.x, gridDim.x) * apw + tid;
 	const int n_here = A.agent_count ? (int)*A.agent_count : A.n;
 	const int base = A.dev_base ? (int)*A.dev_base : A.base;
-	const bool active = t < n_here;
+	const bool active = tid < apw && t < n_here;
 	// a wave without agents (most of a launch over a list counted on the device) leaves
 	// at once: it takes part in no wave-level step and adds nothing to the stats
 	if (__ballot(active) == 0ull) return;

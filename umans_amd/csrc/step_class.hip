@@ -13,7 +13,8 @@ namespace {
 
 template <int KMAX, int BS>
 void launch_t(const StepArgs& a, hipStream_t st) {
-	const int blocks = (a.n + BS - 1) / BS;
+	const int apw = KMAX == 0 && a.long_apw > 0 ? a.long_apw : BS;   // agents per block
+	const int blocks = (a.n + apw - 1) / apw;
 	if (blocks > 0) hipLaunchKernelGGL((k_step<KMAX, BS, UMANS_CLS>), dim3(blocks), dim3(BS), 0, st, a);
 }
 

@@ -153,6 +153,9 @@ struct StepArgs {
 	unsigned long long* long_key;
 	unsigned* long_u;
 	int long_stride;
+	// agents per wave of the long-list instance (16, 32 or 64; 0 = 64): fewer agents per
+	// wave give small crowds more waves and each wave fewer rows to rank
+	int long_apw;
 };
 
 // Capacity of a collected long list; an agent with more candidates in range is handed
