@@ -83,6 +83,8 @@ def test_scheduled_agents_may_share_an_id():
     _, d = load_states()
     sc, meta = gpu.load_scenario(IDS_SCENARIO)
     assert sc.agents["id"].tolist() == [0, 1, 1, 1, 2]
+    # the loader says that this backend keeps agents with one ID distinct
+    assert any("Agent ID 1 is shared by 3 agents" in w for w in meta["warnings"])
     st = np.asarray(meta["start_time"])
     now = [i for i in range(len(st)) if st[i] <= 0]
     later = sorted((i for i in range(len(st)) if st[i] > 0), key=lambda i: st[i])

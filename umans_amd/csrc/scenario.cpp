@@ -444,6 +444,17 @@ bool load_file(umans_scenario& sc, const std::string& path, int depth) {
 			    return each_block_child(b, "Agent", [&](const Node* e) { return load_agent(sc, e, ids); });
 		    }))
 			return false;
+		// Scheduled agents can share an ID (see IdAlloc). The reference then resolves every
+		// neighbour with that ID to the agent listed last (GetAgent_const,
+		// worldBase.cpp:51-60) and excludes them all from each other's lists by ID
+		// (AgentKDTree.cpp:48-50); this backend keeps them distinct agents.
+		std::map<int32_t, int> seen;
+		for (int32_t v : sc.ids) ++seen[v];
+		for (const auto& kv : seen)
+			if (kv.second > 1)
+				sc.warnings.push_back("Agent ID " + std::to_string(kv.first) + " is shared by " + std::to_string(kv.second) +
+				                      " agents: the reference resolves their neighbour entries to the one listed "
+				                      "last; this backend keeps them distinct");
 	}
 	if (const Node* ob = world->child("Obstacles")) {          // crowdSimulator.cpp:496-514
 		if (!load_block(sc, ob, folder, "Obstacles", [&](const Node* b) {
