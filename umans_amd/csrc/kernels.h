@@ -55,17 +55,41 @@ UD int ordered_key(float f) {
 }
 UD float from_ordered_key(int k) { return __int_as_float(k >= 0 ? k : (k ^ 0x7fffffff)); }
 
+// The value of lane ^ J (every lane active): DPP for J = 1, 2 (quad permutes) and 8 (a
+// rotation by 8 within rows of 16), ds_swizzle for 4 and 16 (within 32 lanes), a
+// permute for 32 — the first three never leave the VALU.
+template <int J>
+UD unsigned xor_lane(unsigned v) {
+	if constexpr (J == 1) return (unsigned)__builtin_amdgcn_update_dpp(0, (int)v, 0xB1, 0xF, 0xF, true);
+	else if constexpr (J == 2) return (unsigned)__builtin_amdgcn_update_dpp(0, (int)v, 0x4E, 0xF, 0xF, true);
+	else if constexpr (J == 8) return (unsigned)__builtin_amdgcn_update_dpp(0, (int)v, 0x128, 0xF, 0xF, true);
+	else if constexpr (J == 4 || J == 16) return (unsigned)__builtin_amdgcn_ds_swizzle((int)v, 0x1F | (J << 10));
+	else return (unsigned)__shfl_xor((int)v, J);
+}
+// Minimum / maximum over the wave (every lane active), in every lane: within each row of
+// 16 lanes by the exchanges above (three DPP moves and one swizzle), then the four rows'
+// results read as scalars — instead of six lane permutes through the LDS crossbar.
 UD int wave_min_i(int v) {
-	for (int o = 32; o > 0; o >>= 1) v = min(v, __shfl_xor(v, o));
-	return v;
+	v = min(v, (int)xor_lane<1>((unsigned)v));
+	v = min(v, (int)xor_lane<2>((unsigned)v));
+	v = min(v, (int)xor_lane<4>((unsigned)v));
+	v = min(v, (int)xor_lane<8>((unsigned)v));
+	const int r0 = __builtin_amdgcn_readlane(v, 0), r1 = __builtin_amdgcn_readlane(v, 16);
+	const int r2 = __builtin_amdgcn_readlane(v, 32), r3 = __builtin_amdgcn_readlane(v, 48);
+	return min(min(r0, r1), min(r2, r3));
 }
 // The copy of the step counters a wave updates (its wave index in the launch).
 UD int stat_copy() {
 	return (int)((blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6)) & (kStatCopies - 1));
 }
 UD int wave_max_i(int v) {
-	for (int o = 32; o > 0; o >>= 1) v = max(v, __shfl_xor(v, o));
-	return v;
+	v = max(v, (int)xor_lane<1>((unsigned)v));
+	v = max(v, (int)xor_lane<2>((unsigned)v));
+	v = max(v, (int)xor_lane<4>((unsigned)v));
+	v = max(v, (int)xor_lane<8>((unsigned)v));
+	const int r0 = __builtin_amdgcn_readlane(v, 0), r1 = __builtin_amdgcn_readlane(v, 16);
+	const int r2 = __builtin_amdgcn_readlane(v, 32), r3 = __builtin_amdgcn_readlane(v, 48);
+	return max(max(r0, r1), max(r2, r3));
 }
 
 // Per world w, kBoxCopies copies of (ordered keys of min x, min y, max x, max y), each
@@ -800,17 +824,6 @@ UD int find_short(const StepArgs& A, const WorldInfo& wi, const GridInfo& g, int
 #pragma unroll
 	for (int i = 0; i < KMAX; ++i) list[i * CS] = key_entry(k[i]);
 	return cnt;
-}
-// The value of lane ^ J (every lane active): DPP for J = 1, 2 (quad permutes) and 8 (a
-// rotation by 8 within rows of 16), ds_swizzle for 4 and 16 (within 32 lanes), a
-// permute for 32 — the first three never leave the VALU.
-template <int J>
-UD unsigned xor_lane(unsigned v) {
-	if constexpr (J == 1) return (unsigned)__builtin_amdgcn_update_dpp(0, (int)v, 0xB1, 0xF, 0xF, true);
-	else if constexpr (J == 2) return (unsigned)__builtin_amdgcn_update_dpp(0, (int)v, 0x4E, 0xF, 0xF, true);
-	else if constexpr (J == 8) return (unsigned)__builtin_amdgcn_update_dpp(0, (int)v, 0x128, 0xF, 0xF, true);
-	else if constexpr (J == 4 || J == 16) return (unsigned)__builtin_amdgcn_ds_swizzle((int)v, 0x1F | (J << 10));
-	else return (unsigned)__shfl_xor((int)v, J);
 }
 template <int J>
 UD unsigned long long xor_lane_u64(unsigned long long v) {
