@@ -664,8 +664,15 @@ UD bool same_float_key(unsigned long long a, unsigned long long b) {
 // differ; runs of equal segment and float are put in exact-d2 and ID order before
 // anything is cut. This replaces an insertion per candidate, whose shift loop ran as
 // long as the wave's longest shift and cost more instructions than the rest of the step.
+// The buffer holds KMAX + 4 keys, at least MIN: a lane whose buffer fills drops its
+// canonical last entry, a scan of the whole buffer, so a larger buffer drops less often at
+// the price of a longer sort network and more LDS. The step kernel (one agent per lane)
+// measured fastest with 20 keys for caps of both 12 and 16 (a crowd with ~12.6 agents in
+// range: 24 keys at cap 16 was slower than 20, 16 at cap 12 too); the cooperative kernel
+// keeps KMAX + 4.
 constexpr int kShortList = 16;
-template <int KMAX> constexpr int short_buffer() { return KMAX + 4; }
+constexpr int kStepMinKeys = 20, kCoopMinKeys = 0;
+template <int KMAX, int MIN> constexpr int short_buffer() { return KMAX + 4 > MIN ? KMAX + 4 : MIN; }
 
 // Batcher's odd-even merge sort for n keys: the network for the next power of two with
 // the comparators that touch a slot >= n left out (those slots would hold +inf).
@@ -765,13 +772,13 @@ UD int sort_short(const State& S, V2 p, const WorldInfo& wi, unsigned long long*
 }
 
 // find_neighbors for KMAX <= kShortList: the same queries, exact test and canonical list.
-// buf: the lane's column of KMAX + 4 keys (stride CS, the columns of the wave's lanes
-// side by side); the finished list (index | segment, stride CS) is written over the
-// start of the wave's region, lane by lane as the keys were.
-template <int KMAX, int CS = 64>
+// buf: the lane's column of short_buffer<KMAX, MIN>() keys (stride CS, the columns of
+// the wave's lanes side by side); the finished list (index | segment, stride CS) is
+// written over the start of the wave's region, lane by lane as the keys were.
+template <int KMAX, int MIN, int CS = 64>
 UD int find_short(const StepArgs& A, const WorldInfo& wi, const GridInfo& g, int self, V2 p, float range,
                   unsigned long long* buf, unsigned* list, int& total) {
-	constexpr int KC = short_buffer<KMAX>();
+	constexpr int KC = short_buffer<KMAX, MIN>();
 	const State& S = A.in;
 	const double r2 = (double)(range * range);
 	const float rs = range * 1.0001f + 1e-3f;
@@ -1384,7 +1391,7 @@ __global__ __launch_bounds__(BS) void k_step(StepArgs A) {
 	constexpr int KL = kLong ? kLongCap : KMAX;    // the longest list a lane can hold
 	static_assert(!kLong || BS == 64, "long lists are ranked per wave: one wave per block");
 	constexpr bool kShort = !kLong && KMAX <= kShortList;
-	constexpr int KC = short_buffer<KMAX>();
+	constexpr int KC = short_buffer<KMAX, kStepMinKeys>();
 	__shared__ float s_d2[kLong || kShort ? 1 : KMAX * BS];
 	__shared__ unsigned s_nb[kLong || kShort ? 1 : KMAX * BS];
 	// short lists: each wave's region of KC keys per lane, its list written over the start
@@ -1420,7 +1427,7 @@ __global__ __launch_bounds__(BS) void k_step(StepArgs A) {
 			total = collect_neighbors(A, wi, g, k, a.pos, A.pol[pi].query_range, base + t);
 		} else if constexpr (kShort) {
 			unsigned long long* region = s_buf + (tid >> 6) * (KC * 64);
-			cnt = find_short<KMAX>(A, wi, g, k, a.pos, A.pol[pi].query_range, region + (tid & 63),
+			cnt = find_short<KMAX, kStepMinKeys>(A, wi, g, k, a.pos, A.pol[pi].query_range, region + (tid & 63),
 			                       reinterpret_cast<unsigned*>(region) + (tid & 63), total);
 		} else {
 			cnt = find_neighbors<KMAX, BS>(A, wi, g, k, a.pos, A.pol[pi].query_range, s_d2 + tid, s_nb + tid, total);
@@ -1687,7 +1694,7 @@ template <int KMAX, int APW, int KIND>
 __global__ __launch_bounds__(64) void k_step_coop(StepArgs A) {
 	static_assert(APW == 64 || APW == 32 || APW == 16, "agents per wave");
 	constexpr bool kShort = KMAX <= kShortList;
-	constexpr int KC = short_buffer<KMAX>();
+	constexpr int KC = short_buffer<KMAX, kCoopMinKeys>();
 	// per-agent arrays hold the wave's APW agents (lanes tid < APW): with 32 agents per
 	// wave half the LDS of 64, so twice the waves fit a CU
 	__shared__ float s_d2[kShort ? 1 : KMAX * APW];
@@ -1728,7 +1735,7 @@ __global__ __launch_bounds__(64) void k_step_coop(StepArgs A) {
 		a.dt = wi.dt;
 		a.relaxation = A.pol[pi].relaxation;
 		if constexpr (kShort)
-			cnt = find_short<KMAX, APW>(A, wi, g, k, a.pos, A.pol[pi].query_range, s_buf + tid, s_nb + tid, total);
+			cnt = find_short<KMAX, kCoopMinKeys, APW>(A, wi, g, k, a.pos, A.pol[pi].query_range, s_buf + tid, s_nb + tid, total);
 		else
 			cnt = find_neighbors<KMAX, APW>(A, wi, g, k, a.pos, A.pol[pi].query_range, s_d2 + tid, s_nb + tid, total);
 	}
