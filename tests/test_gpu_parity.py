@@ -154,6 +154,52 @@ def test_mixed_crowd_100k_one_step():
     np.testing.assert_array_equal(out["id"], a["id"])
 
 
+@pytest.mark.parametrize("dense", [False, True])
+def test_force_sampling_ranking_and_ties(dense):
+    """Regular sampling of the force-model costs (PowerLaw + GoalReachingForce,
+    costFunction.cpp:66-164) on crowds built to tie: agents standing at their goals with
+    zero or tiny velocities (targets at the centre of a ring of equal candidates, decided
+    by the first candidate in loop order), agents walking straight at their goals (a grid
+    symmetric about the target), and random ones; isolated (no neighbours, so no
+    transcendental in the costs) or dense. The kernel ranks candidates by
+    sum(coeff |v - target|^2) and leaves near-ties to the exact cost loop: every chosen
+    velocity must equal the FP32 oracle's (a different candidate is off by a sampling
+    step, 0.3 m/s)."""
+    pol = S.powerlaw_sampling_policy()
+    n = 4000
+    r = np.random.default_rng(77 + dense)
+    a = S._agents(n)
+    side = int(np.ceil(np.sqrt(n)))
+    spacing = 1.2 if dense else 20.0
+    g = np.stack(np.meshgrid(np.arange(side), np.arange(side)), -1).reshape(-1, 2)[:n]
+    a["pos"][:] = ((g - side / 2) * spacing + r.uniform(-0.1, 0.1, (n, 2))).astype(np.float32)
+    grp = np.arange(n) % 4
+    a["goal"][:] = a["pos"]                                             # groups 0, 1: at the goal
+    a["vel"][grp == 1] = r.uniform(-1e-3, 1e-3, ((grp == 1).sum(), 2)).astype(np.float32)
+    d = r.normal(size=(n, 2)).astype(np.float32)
+    d /= np.linalg.norm(d, axis=1, keepdims=True)
+    walk = grp == 2                                                     # straight at the goal
+    a["goal"][walk] = (a["pos"][walk] + 30.0 * d[walk]).astype(np.float32)
+    a["vel"][walk] = (1.2 * d[walk]).astype(np.float32)
+    rnd = grp == 3
+    a["goal"][rnd] = (a["pos"][rnd] + r.uniform(-20, 20, (rnd.sum(), 2))).astype(np.float32)
+    a["vel"][rnd] = r.uniform(-1.5, 1.5, (rnd.sum(), 2)).astype(np.float32)
+    a["viewdir"] = S.initial_viewdir(a["pos"], a["goal"])
+    out, _, _ = gpu_step(0, 0.0, 0.0, 0.1, [pol], a, max_neighbors=12, record=False)
+    wd = O.make_world(0, 0.0, 0.0, 0.1, 12)
+    o32 = O.step(wd, [pol], a, precision=0, nthreads=8)
+    o64 = O.step(wd, [pol], a, precision=1, nthreads=8)
+    e32 = rel_err(out["vel"], o32["vel"])
+    e64 = rel_err(out["vel"], o64["vel64"])
+    if dense:
+        # PowerLaw's pair force calls expf: the chosen candidate may differ from the FP32
+        # reference's only where the FP32 and FP64 choices differ too
+        ok = (e64 <= REL_TOL) | (e32 <= 1e-6)
+    else:
+        ok = e32 <= 1e-6
+    assert ok.all(), f"agents {np.where(~ok)[0][:8]} (groups {grp[~ok][:8]}): e32 {e32[~ok][:4]} e64 {e64[~ok][:4]}"
+
+
 @pytest.mark.parametrize("name", example_names())
 def test_gpu_on_reference_examples(name):
     """Example scenarios of the reference read by the XML loader and stepped on the GPU

@@ -1153,6 +1153,37 @@ UD V2 sample_best(const DevPolicy& P, const Self& a, const NbList& L, const Prep
 			// loop's order; a cost not below MaxFloat never wins.
 			const int S = P.s_speed_samples, NA = P.s_angle_samples, inc = P.s_include_base;
 			const float dL = radius / (float)(inc ? (S - 1) : S);
+			// Ranked without the divisions: every cost of the class is a positive
+			// multiple of 0.5 |v - target_c|^2 / dt, so sum(coeff_c |v - target_c|^2)
+			// orders the candidates as the float cost does wherever two of them differ by
+			// more than the rounding of either (each within 2^-20 relative of the exact
+			// value, all terms non-negative). When the best and the second best are 2^-16
+			// apart or closer (ties, symmetric grids), or the coefficients or dt leave the
+			// range where that bound holds, the exact loop below decides.
+			bool ranked = P.n_cost >= 1 && a.dt >= 0x1p-20f && a.dt <= 0x1p20f;
+			for_costs<CLS>(P, [&](int c) { ranked = ranked && P.coeff[c] >= 0x1p-20f && P.coeff[c] <= 0x1p20f; });
+			if (ranked) {
+				float s1 = __builtin_inff(), s2 = __builtin_inff();
+				V2 v1{0.0f, 0.0f};
+				auto rank_at = [&](V2 v) {
+					float s = 0.0f;
+					for_costs<CLS>(P, [&](int c) { s += P.coeff[c] * sqr_mag(v - pr.target[c]); });
+					const bool lt1 = s < s1, lt2 = s < s2;
+					s2 = lt1 ? s1 : (lt2 ? s : s2);
+					v1 = lt1 ? v : v1;
+					s1 = lt1 ? s : s1;
+				};
+				for (int k = 0; k < NA; ++k) {
+					const V2 dir = rotate_cs(bdir, P.ang_cos[k], P.ang_sin[k]);
+					if (inc && k == 0) rank_at(base + dir * (dL * 0.0f));
+					for (int s = inc ? 2 : 1; s <= S; ++s) rank_at(base + dir * (dL * (float)(inc ? s - 1 : s)));
+				}
+				if (s2 <= kMaxFloat && s2 >= 0x1p-60f && s2 > s1 * (1.0f + 0x1p-16f)) {
+					float tc = 0.0f;
+					for_costs<CLS>(P, [&](int c) { tc += P.coeff[c] * cost_of<KMAX, CLS>(P, c, a, L, pr, v1); });
+					return tc < kMaxFloat ? v1 : V2{0.0f, 0.0f};
+				}
+			}
 			int bestIdx = 0x7fffffff;
 			auto eval_at = [&](V2 v, int idx) {
 				float tc = 0.0f;
