@@ -784,7 +784,8 @@ UD int find_short(const StepArgs& A, const WorldInfo& wi, const GridInfo& g, int
 	const float rs = range * 1.0001f + 1e-3f;
 	const int kcap = A.kcap < KMAX ? A.kcap : KMAX;
 	unsigned long long k[KC];
-	unsigned long long lim = kKeyPad;   // (segment, float) above that of a dropped key: in no list
+	// (segment, float) of the last dropped key (key >> 28): a candidate above it is in no list
+	unsigned long long limhi = kKeyPad >> 28;
 	int cnt = 0;
 	total = 0;
 	for (int s = 0; s < 5; ++s) {
@@ -812,13 +813,10 @@ UD int find_short(const StepArgs& A, const WorldInfo& wi, const GridInfo& g, int
 				const bool hit = d2 < r2 && (int)j != self;
 				total += hit ? 1 : 0;
 				const unsigned long long key = long_key(s, d2, j);
-				if (hit && (key >> 28) <= (lim >> 28)) {
+				if (hit && (key >> 28) <= limhi) {
 					buf[cnt * CS] = key;
-					++cnt;
-				}
-				if (__ballot(cnt == KC) != 0ull) {
-					if (cnt == KC) {
-						lim = drop_last<KC, CS>(S, p, wi, buf);
+					if (++cnt == KC) {
+						limhi = drop_last<KC, CS>(S, p, wi, buf) >> 28;
 						--cnt;
 					}
 				}
@@ -1127,6 +1125,38 @@ UD V2 gradient_of(const DevPolicy& P, int c, const Self& a, const NbList& L, con
 	return V2{0.0f, 0.0f};
 }
 
+// The regular grid of the force-model sampling ranked by sum(coeff_c |v - target_c|^2)
+// over the NC costs (the costs without their common factor 0.5 / dt): true, with the
+// smallest in v1, when the second smallest lies more than 2^-16 above it (and above
+// 2^-60), so that the float costs order the two the same way; false for a near-tie.
+template <int NC>
+UD bool rank_force_samples(const DevPolicy& P, const V2* target, V2 base, V2 bdir, float dL, int S, int NA, int inc,
+                           V2& v1) {
+	float coeff[NC];
+	V2 tg[NC];
+#pragma unroll
+	for (int c = 0; c < NC; ++c) {
+		coeff[c] = P.coeff[c];
+		tg[c] = target[c];
+	}
+	float s1 = __builtin_inff(), s2 = __builtin_inff();
+	auto rank_at = [&](V2 v) {
+		float s = 0.0f;
+#pragma unroll
+		for (int c = 0; c < NC; ++c) s += coeff[c] * sqr_mag(v - tg[c]);
+		const bool lt1 = s < s1, lt2 = s < s2;
+		s2 = lt1 ? s1 : (lt2 ? s : s2);
+		v1 = lt1 ? v : v1;
+		s1 = lt1 ? s : s1;
+	};
+	for (int k = 0; k < NA; ++k) {
+		const V2 dir = rotate_cs(bdir, P.ang_cos[k], P.ang_sin[k]);
+		if (inc && k == 0) rank_at(base + dir * (dL * 0.0f));
+		for (int s = inc ? 2 : 1; s <= S; ++s) rank_at(base + dir * (dL * (float)(inc ? s - 1 : s)));
+	}
+	return s2 <= kMaxFloat && s2 >= 0x1p-60f && s2 > s1 * (1.0f + 0x1p-16f);
+}
+
 // CostFunction::ApproximateGlobalMinimumBySampling (costFunction.cpp:66-164): the first
 // strict minimum of sum(coeff * cost) wins; (0,0) if no candidate beats MaxFloat.
 template <int KMAX, int CLS>
@@ -1163,22 +1193,16 @@ UD V2 sample_best(const DevPolicy& P, const Self& a, const NbList& L, const Prep
 			bool ranked = P.n_cost >= 1 && a.dt >= 0x1p-20f && a.dt <= 0x1p20f;
 			for_costs<CLS>(P, [&](int c) { ranked = ranked && P.coeff[c] >= 0x1p-20f && P.coeff[c] <= 0x1p20f; });
 			if (ranked) {
-				float s1 = __builtin_inff(), s2 = __builtin_inff();
+				// one body per cost count, so the loop evaluates only the policy's costs
 				V2 v1{0.0f, 0.0f};
-				auto rank_at = [&](V2 v) {
-					float s = 0.0f;
-					for_costs<CLS>(P, [&](int c) { s += P.coeff[c] * sqr_mag(v - pr.target[c]); });
-					const bool lt1 = s < s1, lt2 = s < s2;
-					s2 = lt1 ? s1 : (lt2 ? s : s2);
-					v1 = lt1 ? v : v1;
-					s1 = lt1 ? s : s1;
-				};
-				for (int k = 0; k < NA; ++k) {
-					const V2 dir = rotate_cs(bdir, P.ang_cos[k], P.ang_sin[k]);
-					if (inc && k == 0) rank_at(base + dir * (dL * 0.0f));
-					for (int s = inc ? 2 : 1; s <= S; ++s) rank_at(base + dir * (dL * (float)(inc ? s - 1 : s)));
+				bool decided = false;
+				switch (P.n_cost) {
+				case 1: decided = rank_force_samples<1>(P, pr.target, base, bdir, dL, S, NA, inc, v1); break;
+				case 2: decided = rank_force_samples<2>(P, pr.target, base, bdir, dL, S, NA, inc, v1); break;
+				case 3: decided = rank_force_samples<3>(P, pr.target, base, bdir, dL, S, NA, inc, v1); break;
+				default: decided = rank_force_samples<4>(P, pr.target, base, bdir, dL, S, NA, inc, v1); break;
 				}
-				if (s2 <= kMaxFloat && s2 >= 0x1p-60f && s2 > s1 * (1.0f + 0x1p-16f)) {
+				if (decided) {
 					float tc = 0.0f;
 					for_costs<CLS>(P, [&](int c) { tc += P.coeff[c] * cost_of<KMAX, CLS>(P, c, a, L, pr, v1); });
 					return tc < kMaxFloat ? v1 : V2{0.0f, 0.0f};
