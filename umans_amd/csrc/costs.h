@@ -40,12 +40,10 @@ struct NbList {
 		const int j = (int)(u & kIndexMask);
 		const unsigned seg = u >> kSegShift;
 		const float2 pj = pos[j];
-		// positionOffset = -displacement (worldToric.cpp:36,47), added in float
-		float ox = -0.0f, oy = -0.0f;
-		if (seg == 1) ox = -W;
-		else if (seg == 2) ox = W;
-		else if (seg == 3) oy = -H;
-		else if (seg == 4) oy = H;
+		// positionOffset = -displacement (worldToric.cpp:36,47), added in float; selects,
+		// not branches (-0 adds nothing, whatever the sign of the coordinate)
+		const float ox = seg == 1 ? -W : (seg == 2 ? W : -0.0f);
+		const float oy = seg == 3 ? -H : (seg == 4 ? H : -0.0f);
 		Nb n;
 		n.pos = V2{pj.x + ox, pj.y + oy};
 		n.vel = v2(vel[j]);

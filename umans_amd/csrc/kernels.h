@@ -1743,10 +1743,16 @@ UD float coop_cost(const DevPolicy& Q, int q, const Self& b, const NbList& L, co
 	else return cost_of<KMAX, CLS_GENERIC>(Q, q, b, L, pr, v);
 }
 
+// Resident waves per SIMD asked of the compiler: the Karamouzas instance with 16 agents
+// per wave (113 VGPRs, 4 waves) measured 5 % faster held to 72 VGPRs and 7 waves (140
+// bytes of spills; 6 and 8 waves gained less or lost). With 32 or 64 agents per wave
+// their LDS caps the waves below that anyway; the other instances keep their allocation.
+constexpr int coop_waves_per_simd(int kind, int apw) { return kind == KIND_KARAMOUZAS && apw == 16 ? 7 : 1; }
+
 // APW agents per wave (64 or 32): fewer agents per wave make more waves, for crowds too
 // small to fill the SIMDs at 64 (the candidates of the wave's agents still cover all lanes).
 template <int KMAX, int APW, int KIND>
-__global__ __launch_bounds__(64) void k_step_coop(StepArgs A) {
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(coop_waves_per_simd(KIND, APW)))) void k_step_coop(StepArgs A) {
 	static_assert(APW == 64 || APW == 32 || APW == 16, "agents per wave");
 	constexpr bool kShort = KMAX <= kShortList;
 	constexpr int KC = short_buffer<KMAX, kCoopMinKeys>();
@@ -1897,7 +1903,13 @@ __global__ __launch_bounds__(64) void k_step_coop(StepArgs A) {
 		const V2 v = regular_candidate(Q, c, base, bdir, dL);
 		const NbList Lb{s_nb + ag, APW, bcnt, S.pos, S.vel, S.prm, S.pref, bW, bH, b.pos, nullptr, 0};
 		float tc = 0.0f;
-		for (int q = 0; q < Q.n_cost; ++q) tc += Q.coeff[q] * coop_cost<KMAX, KIND>(Q, q, b, Lb, pr, v);
+		if constexpr (KIND >= 0) {
+			// the class's policies all have this one cost: index 0, so its per-agent terms
+			// stay in registers
+			tc += Q.coeff[0] * coop_cost<KMAX, KIND>(Q, 0, b, Lb, pr, v);
+		} else {
+			for (int q = 0; q < Q.n_cost; ++q) tc += Q.coeff[q] * coop_cost<KMAX, KIND>(Q, q, b, Lb, pr, v);
+		}
 		const unsigned long long key = cost_key(tc, c);
 		if (key < local) local = key;
 	}
