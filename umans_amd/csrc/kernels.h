@@ -611,15 +611,22 @@ UD int collect_neighbors(const StepArgs& A, const WorldInfo& wi, const GridInfo&
 		for (int cy = cy0; cy <= cy1; ++cy) {
 			const int crow = wi.cell_offset + cy * g.ncx;
 			const unsigned b = A.cell_start[crow + cx0], e = A.cell_start[crow + cx1 + 1];
-			for (unsigned j = b; j < e; ++j) {
-				const float2 pj = S.pos[j];
+			// two candidates a pass, both positions loaded first: one wait covers both
+			auto visit = [&](unsigned j, float2 pj) {
 				const double ddx = qx - (double)pj.x;
 				const double ddy = qy - (double)pj.y;
 				const double d2 = ddx * ddx + ddy * ddy;
-				if (!(d2 < r2) || (int)j == self) continue;
+				if (!(d2 < r2) || (int)j == self) return;
 				if (n < kLongCap) A.long_key[(size_t)row * kLongCap + n] = long_key(s, d2, j);
 				++n;
+			};
+			unsigned j = b;
+			for (; j + 1 < e; j += 2) {
+				const float2 p0 = S.pos[j], p1 = S.pos[j + 1];
+				visit(j, p0);
+				visit(j + 1, p1);
 			}
+			if (j < e) visit(j, S.pos[j]);
 		}
 	}
 	return n;
@@ -805,8 +812,8 @@ UD int find_short(const StepArgs& A, const WorldInfo& wi, const GridInfo& g, int
 		for (int cy = cy0; cy <= cy1; ++cy) {
 			const int row = wi.cell_offset + cy * g.ncx;
 			const unsigned b = A.cell_start[row + cx0], e = A.cell_start[row + cx1 + 1];
-			for (unsigned j = b; j < e; ++j) {
-				const float2 pj = S.pos[j];
+			// two candidates a pass, both positions loaded first: one wait covers both
+			auto visit = [&](unsigned j, float2 pj) {
 				const double ddx = qx - (double)pj.x;
 				const double ddy = qy - (double)pj.y;
 				const double d2 = ddx * ddx + ddy * ddy;
@@ -820,7 +827,14 @@ UD int find_short(const StepArgs& A, const WorldInfo& wi, const GridInfo& g, int
 						--cnt;
 					}
 				}
+			};
+			unsigned j = b;
+			for (; j + 1 < e; j += 2) {
+				const float2 p0 = S.pos[j], p1 = S.pos[j + 1];
+				visit(j, p0);
+				visit(j + 1, p1);
 			}
+			if (j < e) visit(j, S.pos[j]);
 		}
 	}
 	cnt = sort_short<KC, CS>(S, p, wi, buf, cnt, kcap, k);
