@@ -7,7 +7,9 @@ state, bit for bit, as one sorted-insertion launch at full capacity."""
 import numpy as np
 import pytest
 
+import oracle_py as O
 from cases import CASES
+from gpu_util import REL_TOL, rel_err
 from umans_amd import gpu
 from umans_amd import scenarios as S
 from umans_amd.registry import pack_policy
@@ -145,3 +147,42 @@ def test_ranked_lists_on_a_lattice(world_type, spacing, jitter):
     np.testing.assert_array_equal(seg, rseg)
     for k in ("pos", "vel", "viewdir", "rng"):
         np.testing.assert_array_equal(out[k], ref[k], err_msg=k)
+
+
+@pytest.mark.parametrize("world_type", [0, 1])
+@pytest.mark.parametrize("spacing", [0.75, 1.5])
+@pytest.mark.parametrize("jitter", [False, True])
+@pytest.mark.parametrize("cap", [12, 16])
+def test_capped_short_lists_on_a_lattice(world_type, spacing, jitter, cap):
+    """Capped lists (12 or 16, the per-lane LDS buffers) on the square lattice: lists of ~89
+    or ~22 agents in range, so every buffer fills and drops its canonical last entry many
+    times, and rings of 4 or 8 equal distances straddle the cut (with jitter, distances
+    equal as floats and unequal as doubles too). Lists bit-exact against the FP32 oracle
+    (the first `cap` of the canonical order: exact d2, then agent ID), velocities to the
+    parity bar."""
+    side = 36
+    pos = lattice(side, spacing, jitter, seed=7)
+    pol = S.powerlaw_sampling_policy()
+    sc = S.random_cloud(side * side, float(side * spacing), [pol], world_type=world_type, seed=41)
+    a = dict(sc.agents)
+    a["pos"] = pos
+    a["id"] = np.random.default_rng(9).permutation(side * side).astype(np.int32)
+    w = gpu.World(world_type, float(side * spacing), float(side * spacing), sc.dt, [pol], max_neighbors=cap)
+    w.upload(a)
+    w.record_neighbors(True)
+    w.step(1)
+    out, (cnt, ids, d2, seg), st = w.download(), w.neighbors(), w.stats()
+    w.close()
+    wd = O.make_world(world_type, float(side * spacing), float(side * spacing), sc.dt, cap)
+    o32 = O.step(wd, [pol], a, precision=0, list_stride=128)
+    o64 = O.step(wd, [pol], a, precision=1)
+    np.testing.assert_array_equal(cnt, o32["nb_count"])
+    m = np.arange(ids.shape[1])[None] < cnt[:, None]
+    np.testing.assert_array_equal(np.where(m, ids, -1), np.where(m, o32["nb_id"][:, :ids.shape[1]], -1))
+    np.testing.assert_array_equal(np.where(m, d2, 0.0), np.where(m, o32["nb_d2"][:, :ids.shape[1]], 0.0))
+    np.testing.assert_array_equal(np.where(m, seg, 0), np.where(m, o32["nb_seg"][:, :ids.shape[1]], 0))
+    assert st["overflow_agents"] == int((o32["nb_full"] > cap).sum()) > 0
+    e64 = rel_err(out["vel"], o64["vel64"])
+    e32 = rel_err(out["vel"], o32["vel"])
+    ok = (e64 <= REL_TOL) | (e32 <= 1e-6)
+    assert ok.all(), f"{(~ok).sum()} agents off: e64 {np.sort(e64)[-4:]}"
